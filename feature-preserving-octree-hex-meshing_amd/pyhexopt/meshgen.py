@@ -1,0 +1,171 @@
+"""Synthetic workloads for the hex-optimisation path.
+
+Hex corner order is the reference's `Hybrid::vs` order (global_types.h:216-226,
+`hex_ref_shape`): 0=(0,0,0) 1=(1,0,0) 2=(1,1,0) 3=(0,1,0) 4=(0,0,1) 5=(1,0,1) 6=(1,1,1)
+7=(0,1,1). Vertex arrays are (nV, 3) float64 row-major unless stated otherwise.
+"""
+import numpy as np
+
+HEX_CORNERS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                        [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=np.int64)
+# global_types.h:163-173
+HEX_TETRA_TABLE = np.array([[0, 3, 4, 1], [1, 0, 5, 2], [2, 1, 6, 3], [3, 2, 7, 0],
+                            [4, 7, 5, 0], [5, 4, 6, 1], [6, 5, 7, 2], [7, 6, 4, 3]], dtype=np.int64)
+
+
+def grid_hex(n, lo=-1.0, hi=1.0):
+    """Regular n^3 hex grid on [lo,hi]^3. Returns V (nV,3), H (nH,8) int32."""
+    t = np.linspace(lo, hi, n + 1)
+    z, y, x = np.meshgrid(t, t, t, indexing="ij")
+    V = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.float64)
+    idx = np.arange((n + 1) ** 3).reshape(n + 1, n + 1, n + 1)  # [z, y, x]
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    H = np.stack([idx[k + c[2], j + c[1], i + c[0]].ravel() for c in HEX_CORNERS], axis=1)
+    return V, H.astype(np.int32)
+
+
+def grid_boundary_mask(n):
+    t = np.arange(n + 1)
+    z, y, x = np.meshgrid(t, t, t, indexing="ij")
+    b = (x == 0) | (x == n) | (y == 0) | (y == n) | (z == 0) | (z == n)
+    return b.ravel()
+
+
+def jitter(V, interior_mask, sigma, seed=0):
+    rng = np.random.default_rng(seed)
+    out = V.copy()
+    ids = np.nonzero(interior_mask)[0]
+    out[ids] += rng.normal(0.0, sigma, size=(ids.size, 3))
+    return out
+
+
+def invert_hexes(V, H, frac, seed=0):
+    """Swap the positions of corners 0 and 1 in a seeded fraction of hexes (inverts them).
+    Hexes sharing a vertex may be picked; the swap is applied in hex order."""
+    rng = np.random.default_rng(seed)
+    nH = H.shape[0]
+    k = int(round(frac * nH))
+    pick = rng.choice(nH, size=k, replace=False)
+    out = V.copy()
+    for h in np.sort(pick):
+        a, b = H[h, 0], H[h, 1]
+        out[[a, b]] = out[[b, a]]
+    return out, np.sort(pick)
+
+
+def icosphere(level):
+    """Unit icosphere with 20*4^level triangles (outward orientation)."""
+    p = (1.0 + 5 ** 0.5) / 2.0
+    V = [[-1, p, 0], [1, p, 0], [-1, -p, 0], [1, -p, 0], [0, -1, p], [0, 1, p], [0, -1, -p], [0, 1, -p],
+         [p, 0, -1], [p, 0, 1], [-p, 0, -1], [-p, 0, 1]]
+    F = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    V = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in V]
+    for _ in range(level):
+        cache = {}
+        nF = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = V[a] + V[b]
+                V.append(m / np.linalg.norm(m))
+                cache[key] = len(V) - 1
+            return cache[key]
+
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nF += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        F = nF
+    return np.array(V), np.array(F, dtype=np.int32)
+
+
+def superellipsoid(level=5, power=4.0):
+    """|x|^p+|y|^p+|z|^p = 1 surface: icosphere vertices mapped radially p/||p||_power."""
+    V, F = icosphere(level)
+    nrm = (np.abs(V) ** power).sum(axis=1) ** (1.0 / power)
+    return V / nrm[:, None], F
+
+
+def cube_surface(n=16, lo=0.0, hi=1.0):
+    """Unit-cube surface, each face an n x n quad grid split along one diagonal
+    (12*n^2 triangles, outward orientation)."""
+    t = np.linspace(lo, hi, n + 1)
+    verts = {}
+    V = []
+
+    def vid(p):
+        key = tuple(np.round(p, 12))
+        if key not in verts:
+            verts[key] = len(V)
+            V.append(p)
+        return verts[key]
+
+    F = []
+    for axis in range(3):
+        for side in (0, 1):
+            u_ax, v_ax = [a for a in range(3) if a != axis]
+            for i in range(n):
+                for j in range(n):
+                    q = []
+                    for (di, dj) in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        p = np.zeros(3)
+                        p[axis] = hi if side else lo
+                        p[u_ax] = t[i + di]
+                        p[v_ax] = t[j + dj]
+                        q.append(vid(p))
+                    a, b, c, d = q
+                    tri = [[a, b, c], [a, c, d]]
+                    # orient outward: normal of (a,b,c) along +axis when side==1
+                    e1 = np.array(V[b]) - np.array(V[a])
+                    e2 = np.array(V[c]) - np.array(V[a])
+                    nz = np.cross(e1, e2)[axis]
+                    if (nz > 0) != bool(side):
+                        tri = [[a, c, b], [a, d, c]]
+                    F += tri
+    return np.array(V), np.array(F, dtype=np.int32)
+
+
+def torus_surface(R=1.0, r=0.35, nu=200, nv=100):
+    """Torus of radii (R, r) as an nu x nv parameter grid split into 2*nu*nv triangles."""
+    u = np.arange(nu) * 2 * np.pi / nu
+    v = np.arange(nv) * 2 * np.pi / nv
+    uu, vv = np.meshgrid(u, v, indexing="ij")
+    X = (R + r * np.cos(vv)) * np.cos(uu)
+    Y = (R + r * np.cos(vv)) * np.sin(uu)
+    Z = r * np.sin(vv)
+    V = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    idx = np.arange(nu * nv).reshape(nu, nv)
+    F = []
+    for i in range(nu):
+        for j in range(nv):
+            a, b = idx[i, j], idx[(i + 1) % nu, j]
+            c, d = idx[(i + 1) % nu, (j + 1) % nv], idx[i, (j + 1) % nv]
+            F += [[a, b, c], [a, c, d]]
+    return V, np.array(F, dtype=np.int32)
+
+
+def voxel_hex_mesh(inside_fn, lo, hi, n):
+    """Uniform-octree (single level) hex mesh: the cells of an n^3 grid on [lo,hi]^3 whose
+    centre satisfies inside_fn. Unused vertices are dropped. Returns V, H."""
+    V, H = grid_hex(n, lo, hi)
+    cen = V[H].mean(axis=1)
+    keep = inside_fn(cen)
+    H = H[keep]
+    used = np.unique(H)
+    remap = -np.ones(V.shape[0], dtype=np.int64)
+    remap[used] = np.arange(used.size)
+    return V[used], remap[H].astype(np.int32)
+
+
+def boundary_vertices(H, nV):
+    """Vertices on boundary quads (quads used by exactly one hex)."""
+    faces = np.array([[0, 1, 2, 3], [4, 7, 6, 5], [0, 4, 5, 1], [0, 3, 7, 4], [3, 2, 6, 7], [1, 5, 6, 2]])
+    Q = H[:, faces].reshape(-1, 4)
+    key = np.sort(Q, axis=1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    bq = Q[cnt[inv.ravel()] == 1]
+    mask = np.zeros(nV, dtype=bool)
+    mask[bq.ravel()] = True
+    return mask

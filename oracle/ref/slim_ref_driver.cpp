@@ -1,0 +1,285 @@
+// slim_ref_driver.cpp -- runs the reference's own hex-optimisation code (compiled from the
+// sources under the reference tree, see oracle/ref/Makefile) on a small binary input and
+// writes its outputs, so the tests can pin the CPU restatement and the HIP path to it.
+// Test infrastructure only; built in the development container, never shipped.
+//
+// Modes
+//   0  one optimization::slim_m_opt-equivalent call (slim_precompute + slim_solve2, fixed
+//      iteration branch, slim_m.cpp:1095-1132,1269-1288) with given soft constraints; the
+//      iterations are run one slim_solve2(data,1) call at a time so every iteration is logged.
+//   1  the deformation loop body (grid_hex_meshing.cpp:2945-2959) with 1 SLIM iteration per
+//      outer iteration: compute_referenceMesh(cube) -> tangent-plane projection of the given
+//      surface vertices via igl::signed_distance_pseudonormal -> slim_m_opt(ts,1,-1) ->
+//      lambda_T update.
+//   2  closest point + pseudonormal only (igl::AABB + normals as build_aabb_tree,
+//      grid_hex_meshing.cpp:4231-4248).
+#include "slim_m.h"
+#include "igl/AABB.h"
+#include "igl/per_face_normals.h"
+#include "igl/per_vertex_normals.h"
+#include "igl/per_edge_normals.h"
+#include "igl/signed_distance.h"
+#include <cstdio>
+#include <cstdint>
+#include <vector>
+#include <string>
+
+void update_weights_and_closest_rotations(SLIMData& s, const Eigen::MatrixXd& V, const Eigen::MatrixXi& F,
+                                          Eigen::MatrixXd& uv);
+double compute_energy_total(SLIMData& s, Eigen::MatrixXd& V_new);
+
+namespace {
+struct In {
+  FILE* f;
+  template <class T> T get() { T v; if (fread(&v, sizeof(T), 1, f) != 1) { fprintf(stderr, "short read\n"); exit(3); } return v; }
+  template <class T> std::vector<T> arr(size_t n) {
+    std::vector<T> v(n);
+    if (n && fread(v.data(), sizeof(T), n, f) != n) { fprintf(stderr, "short read\n"); exit(3); }
+    return v;
+  }
+};
+struct Out {
+  FILE* f;
+  void put(const std::string& name, const void* p, size_t bytes, char kind) {
+    uint32_t ln = (uint32_t)name.size();
+    fwrite(&ln, 4, 1, f);
+    fwrite(name.data(), 1, ln, f);
+    fwrite(&kind, 1, 1, f);
+    uint64_t nb = bytes;
+    fwrite(&nb, 8, 1, f);
+    if (bytes) fwrite(p, 1, bytes, f);
+  }
+  void putd(const std::string& n, const std::vector<double>& v) { put(n, v.data(), v.size() * 8, 'd'); }
+  void puti(const std::string& n, const std::vector<int>& v) { put(n, v.data(), v.size() * 4, 'i'); }
+};
+
+Eigen::MatrixXd rows3(const std::vector<double>& v, int n) {
+  Eigen::MatrixXd M(n, 3);
+  for (int i = 0; i < n; ++i) for (int d = 0; d < 3; ++d) M(i, d) = v[(size_t)i * 3 + d];
+  return M;
+}
+std::vector<double> flat3(const Eigen::MatrixXd& M) {
+  std::vector<double> v((size_t)M.rows() * 3);
+  for (int i = 0; i < M.rows(); ++i) for (int d = 0; d < 3; ++d) v[(size_t)i * 3 + d] = M(i, d);
+  return v;
+}
+
+struct Surface {
+  Eigen::MatrixXd V, FN, VN, EN;
+  Eigen::MatrixXi F, E;
+  Eigen::VectorXi EMAP;
+  igl::AABB<Eigen::MatrixXd, 3> tree;
+  void build() {
+    tree.init(V, F);
+    igl::per_face_normals(V, F, FN);
+    igl::per_vertex_normals(V, F, igl::PER_VERTEX_NORMALS_WEIGHTING_TYPE_ANGLE, FN, VN);
+    igl::per_edge_normals(V, F, igl::PER_EDGE_NORMALS_WEIGHTING_TYPE_UNIFORM, FN, EN, E, EMAP);
+  }
+};
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc != 3) { fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]); return 2; }
+  In in{fopen(argv[1], "rb")};
+  if (!in.f) return 2;
+  Out out{fopen(argv[2], "wb")};
+  int mode = in.get<int32_t>();
+
+  if (mode == 2) {
+    Surface s;
+    int nv = in.get<int32_t>(), nf = in.get<int32_t>(), nq = in.get<int32_t>();
+    s.V = rows3(in.arr<double>((size_t)nv * 3), nv);
+    auto F = in.arr<int32_t>((size_t)nf * 3);
+    s.F.resize(nf, 3);
+    for (int i = 0; i < nf; ++i) for (int d = 0; d < 3; ++d) s.F(i, d) = F[(size_t)i * 3 + d];
+    Eigen::MatrixXd P = rows3(in.arr<double>((size_t)nq * 3), nq);
+    s.build();
+    Eigen::VectorXd S; Eigen::VectorXi I; Eigen::MatrixXd C, N;
+    igl::signed_distance_pseudonormal(P, s.V, s.F, s.tree, s.FN, s.VN, s.EN, s.EMAP, S, I, C, N);
+    std::vector<double> Sv(S.data(), S.data() + nq);
+    std::vector<int> Iv(I.data(), I.data() + nq);
+    out.putd("S", Sv); out.puti("I", Iv); out.putd("C", flat3(C)); out.putd("N", flat3(N));
+    out.putd("FN", flat3(s.FN)); out.putd("VN", flat3(s.VN));
+    std::vector<double> fen((size_t)nf * 9);
+    for (int f = 0; f < nf; ++f)
+      for (int x = 0; x < 3; ++x)
+        for (int d = 0; d < 3; ++d) fen[(size_t)f * 9 + x * 3 + d] = s.EN(s.EMAP(nf * x + f), d);
+    out.putd("FEN", fen);
+    fclose(out.f);
+    return 0;
+  }
+
+  int nV = in.get<int32_t>(), nH = in.get<int32_t>();
+  int energy = in.get<int32_t>(), n_outer = in.get<int32_t>(), n_inner = in.get<int32_t>();
+  int cube = in.get<int32_t>(), projection = in.get<int32_t>();
+  double weight_opt = in.get<double>(), lamT = in.get<double>(), lamT_bound = in.get<double>();
+  double lamC = in.get<double>(), lamL = in.get<double>(), lamProj = in.get<double>();
+  auto V = in.arr<double>((size_t)nV * 3);
+  auto H = in.arr<int32_t>((size_t)nH * 8);
+  auto hflag = in.arr<int32_t>(nH);
+  int nb = in.get<int32_t>();
+  auto b = in.arr<int32_t>(nb);
+  auto bc = in.arr<double>((size_t)nb * 3);
+  int nTc = in.get<int32_t>();
+  auto idsT = in.arr<int32_t>(nTc);
+  auto normT = in.arr<double>((size_t)nTc * 3);
+  auto disT = in.arr<double>(nTc);
+  int nC = in.get<int32_t>();
+  auto idsC = in.arr<int32_t>(nC);
+  auto Cc = in.arr<double>((size_t)nC * 3);
+  int nL = in.get<int32_t>();
+  auto idsL = in.arr<int32_t>(nL);
+  auto axa = in.arr<double>((size_t)nL * 3);
+  auto org = in.arr<double>((size_t)nL * 3);
+  Surface surf;
+  int nsv = in.get<int32_t>(), nsf = in.get<int32_t>();
+  if (nsv > 0) {
+    surf.V = rows3(in.arr<double>((size_t)nsv * 3), nsv);
+    auto F = in.arr<int32_t>((size_t)nsf * 3);
+    surf.F.resize(nsf, 3);
+    for (int i = 0; i < nsf; ++i) for (int d = 0; d < 3; ++d) surf.F(i, d) = F[(size_t)i * 3 + d];
+    surf.build();
+  }
+
+  // Mesh and Tetralize_Set exactly as deformation() builds them (grid_hex_meshing.cpp:2890-2935)
+  Mesh m;
+  m.type = Mesh_type::Hex;
+  m.V.resize(3, nV);
+  for (int i = 0; i < nV; ++i) for (int d = 0; d < 3; ++d) m.V(d, i) = V[(size_t)i * 3 + d];
+  for (int h = 0; h < nH; ++h) {
+    Hybrid hy; hy.id = h;
+    for (int i = 0; i < 8; ++i) hy.vs.push_back(H[(size_t)h * 8 + i]);
+    m.Hs.push_back(hy);
+  }
+  Tetralize_Set ts;
+  ts.V = m.V.transpose();
+  ts.T.resize(nH * 8, 4);
+  for (int h = 0; h < nH; ++h)
+    for (int i = 0; i < 8; ++i)
+      for (int j = 0; j < 4; ++j) ts.T(h * 8 + i, j) = H[(size_t)h * 8 + hex_tetra_table[i][j]];
+  Feature_Constraints fc;
+  fc.lamda_C = lamC; fc.lamda_L = lamL; fc.lamda_T = lamT;
+  fc.ids_T.resize(nTc); fc.normal_T.resize(nTc, 3); fc.dis_T.resize(nTc);
+  for (int i = 0; i < nTc; ++i) {
+    fc.ids_T(i) = idsT[i];
+    for (int d = 0; d < 3; ++d) fc.normal_T(i, d) = normT[(size_t)i * 3 + d];
+    fc.dis_T(i) = disT[i];
+  }
+  fc.ids_C.resize(nC); fc.C.resize(nC, 3);
+  for (int i = 0; i < nC; ++i) { fc.ids_C(i) = idsC[i]; for (int d = 0; d < 3; ++d) fc.C(i, d) = Cc[(size_t)i * 3 + d]; }
+  fc.num_a = nL; fc.ids_L.resize(nL); fc.Axa_L.resize(nL, 3); fc.origin_L.resize(nL, 3);
+  for (int i = 0; i < nL; ++i) {
+    fc.ids_L(i) = idsL[i];
+    for (int d = 0; d < 3; ++d) { fc.Axa_L(i, d) = axa[(size_t)i * 3 + d]; fc.origin_L(i, d) = org[(size_t)i * 3 + d]; }
+  }
+  ts.energy_type = (SLIM_ENERGY)energy;
+  ts.UV = ts.V;
+  ts.fc = fc;
+  ts.projection = projection != 0;
+  ts.lamda_projection = lamProj;
+  ts.global = true;
+  ts.glue = false;
+  ts.lamda_glue = 0;
+  ts.lamda_region = 0;
+  ts.b.resize(nb); ts.bc.resize(nb, 3);
+  for (int i = 0; i < nb; ++i) { ts.b(i) = b[i]; for (int d = 0; d < 3; ++d) ts.bc(i, d) = bc[(size_t)i * 3 + d]; }
+
+  std::vector<bool> H_flag(nH, true);
+  for (int h = 0; h < nH; ++h) H_flag[h] = hflag[h] != 0;
+  std::vector<uint32_t> Hids;
+  for (int h = 0; h < nH; ++h) Hids.push_back(h);
+
+  std::vector<double> log;
+  std::vector<double> Vtrace;
+  if (mode == 0) {
+    compute_referenceMesh(ts.V, m.Hs, H_flag, Hids, ts.RT, cube != 0);
+    SLIMData data;
+    data.stop_threshold = 1.e-7;
+    data.soft_const_p = 1e1;
+    data.exp_factor = 5.0;
+    data.weight_opt = weight_opt;
+    slim_precompute(ts.V, ts.T, ts.UV, data, ts.energy_type, ts);
+    // G per tet from the assembled Dx/Dy/Dz
+    int nT = nH * 8;
+    std::vector<double> G((size_t)nT * 12, 0.0);
+    const Eigen::SparseMatrix<double>* D[3] = {&data.Dx, &data.Dy, &data.Dz};
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < D[c]->outerSize(); ++k)
+        for (Eigen::SparseMatrix<double>::InnerIterator it(*D[c], k); it; ++it) {
+          int t = it.row(), col = it.col();
+          for (int j = 0; j < 4; ++j)
+            if (ts.T(t, j) == col) G[(size_t)t * 12 + c * 4 + j] += it.value();
+        }
+    out.putd("G", G);
+    std::vector<double> e0 = {data.energy, data.engery_quality, data.engery_soft};
+    out.putd("E0", e0);
+    // local step at the start point (W and R per tet, row-major 3x3)
+    {
+      SLIMData d2 = data;
+      Eigen::MatrixXd uv = d2.V_o;
+      update_weights_and_closest_rotations(d2, d2.V, d2.F, uv);
+      std::vector<double> W((size_t)nT * 9), R((size_t)nT * 9);
+      const Eigen::VectorXd* Wc[9] = {&d2.W_11, &d2.W_12, &d2.W_13, &d2.W_21, &d2.W_22, &d2.W_23, &d2.W_31, &d2.W_32, &d2.W_33};
+      for (int t = 0; t < nT; ++t) {
+        for (int k = 0; k < 9; ++k) W[(size_t)t * 9 + k] = (*Wc[k])(t);
+        // Ri(t, b + 3c) = R(b, c)
+        for (int bb = 0; bb < 3; ++bb)
+          for (int c = 0; c < 3; ++c) R[(size_t)t * 9 + bb * 3 + c] = d2.Ri(t, bb + 3 * c);
+      }
+      out.putd("W0", W);
+      out.putd("R0", R);
+    }
+    for (int it = 0; it < n_inner; ++it) {
+      slim_solve2(data, 1, -1, false);
+      log.push_back(data.energy); log.push_back(data.engery_quality); log.push_back(data.engery_soft);
+      auto f = flat3(data.V_o);
+      Vtrace.insert(Vtrace.end(), f.begin(), f.end());
+    }
+    std::vector<double> aL(data.a_L.data(), data.a_L.data() + data.a_L.size());
+    out.putd("aL", aL);
+    ts.UV = data.V_o;
+  } else if (mode == 1) {
+    // ts.fc.ids_T lists the vertices to project (projection_smooth, grid_hex_meshing.cpp:3760-3781)
+    double LAMDA0 = lamT;
+    for (int it = 0; it < n_outer; ++it) {
+      compute_referenceMesh(ts.V, m.Hs, H_flag, Hids, ts.RT, cube != 0);
+      if (nsv > 0 && nTc > 0) {
+        Eigen::MatrixXd Ps(nTc, 3);
+        for (int i = 0; i < nTc; ++i) Ps.row(i) = ts.V.row(fc.ids_T(i));
+        Eigen::VectorXd S; Eigen::VectorXi I; Eigen::MatrixXd C, N;
+        igl::signed_distance_pseudonormal(Ps, surf.V, surf.F, surf.tree, surf.FN, surf.VN, surf.EN, surf.EMAP, S, I, C, N);
+        for (int j = 0; j < nTc; ++j) {
+          fc.normal_T.row(j) = N.row(j);
+          fc.dis_T[j] = N.row(j).dot(C.row(j));
+        }
+      }
+      ts.fc = fc;
+      SLIMData data;
+      data.stop_threshold = 1.e-7;
+      data.soft_const_p = 1e1;
+      data.exp_factor = 5.0;
+      data.weight_opt = weight_opt;
+      slim_precompute(ts.V, ts.T, ts.UV, data, ts.energy_type, ts);
+      for (int k = 0; k < n_inner; ++k) slim_solve2(data, 1, -1, false);
+      ts.UV = data.V_o;
+      ts.V = ts.UV;
+      log.push_back(data.energy); log.push_back(data.engery_quality); log.push_back(data.engery_soft);
+      log.push_back(fc.lamda_T);
+      fc.lamda_T = std::min(LAMDA0 * std::max(data.engery_quality / data.engery_soft * fc.lamda_T, 1.0), lamT_bound);
+      auto f = flat3(ts.UV);
+      Vtrace.insert(Vtrace.end(), f.begin(), f.end());
+    }
+  }
+  out.putd("log", log);
+  out.putd("Vtrace", Vtrace);
+  out.putd("UV", flat3(ts.UV));
+  Mesh_Quality mq;
+  m.V = ts.UV.transpose();
+  scaled_jacobian(m, mq);
+  std::vector<double> q = {mq.min_Jacobian, mq.ave_Jacobian};
+  out.putd("SJ", q);
+  std::vector<double> hj(mq.H_Js.data(), mq.H_Js.data() + mq.H_Js.size());
+  out.putd("SJ_hex", hj);
+  fclose(out.f);
+  return 0;
+}

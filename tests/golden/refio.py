@@ -1,0 +1,81 @@
+"""Input/output of the compiled-reference driver (oracle/_ref/slim_ref).
+
+Used only in the development container (where the reference tree exists) to write the
+golden fixtures under tests/golden/. GPU runs only read the committed fixtures.
+"""
+import os
+import struct
+import subprocess
+import tempfile
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REF_BIN = os.path.join(REPO, "oracle", "_ref", "slim_ref")
+
+
+def available():
+    return os.path.exists(REF_BIN)
+
+
+def _w(f, fmt, *v):
+    f.write(struct.pack("<" + fmt, *v))
+
+
+def _arr(f, a, dt):
+    f.write(np.ascontiguousarray(a, dtype=dt).tobytes())
+
+
+def read_out(path):
+    out = {}
+    with open(path, "rb") as f:
+        while True:
+            h = f.read(4)
+            if not h:
+                break
+            (ln,) = struct.unpack("<I", h)
+            name = f.read(ln).decode()
+            kind = f.read(1).decode()
+            (nb,) = struct.unpack("<Q", f.read(8))
+            raw = f.read(nb)
+            out[name] = np.frombuffer(raw, dtype=np.float64 if kind == "d" else np.int32).copy()
+    return out
+
+
+def run_slim(mode, V, H, energy=2, n_outer=1, n_inner=1, cube=True, projection=False, weight_opt=1.0,
+             lamT=0.0, lamT_bound=1e8, lamC=0.0, lamL=0.0, lamProj=1e5, hflag=None, b=None, bc=None,
+             idsT=None, normalT=None, disT=None, idsC=None, Cpos=None, idsL=None, axaL=None, originL=None,
+             surfV=None, surfF=None):
+    nV, nH = V.shape[0], H.shape[0]
+    e3 = np.zeros((0, 3)); ei = np.zeros(0, dtype=np.int32)
+    hflag = np.ones(nH, dtype=np.int32) if hflag is None else hflag
+    with tempfile.TemporaryDirectory() as td:
+        pin, pout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(pin, "wb") as f:
+            _w(f, "i", mode)
+            _w(f, "iiiiiii", nV, nH, energy, n_outer, n_inner, int(cube), int(projection))
+            _w(f, "dddddd", weight_opt, lamT, lamT_bound, lamC, lamL, lamProj)
+            _arr(f, V, "<f8"); _arr(f, H, "<i4"); _arr(f, hflag, "<i4")
+            for ids, arrs in ((b, (bc,)), (idsT, (normalT, disT)), (idsC, (Cpos,)), (idsL, (axaL, originL))):
+                ids = ei if ids is None else ids
+                _w(f, "i", len(ids)); _arr(f, ids, "<i4")
+                for a in arrs:
+                    if a is None:
+                        a = e3[:0] if ids is None else np.zeros((len(ids), 3))
+                    _arr(f, a, "<f8")
+            if surfV is None:
+                _w(f, "ii", 0, 0)
+            else:
+                _w(f, "ii", surfV.shape[0], surfF.shape[0]); _arr(f, surfV, "<f8"); _arr(f, surfF, "<i4")
+        subprocess.run([REF_BIN, pin, pout], check=True, stdout=subprocess.DEVNULL)
+        return read_out(pout)
+
+
+def run_aabb(V, F, P):
+    with tempfile.TemporaryDirectory() as td:
+        pin, pout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(pin, "wb") as f:
+            _w(f, "i", 2)
+            _w(f, "iii", V.shape[0], F.shape[0], P.shape[0])
+            _arr(f, V, "<f8"); _arr(f, F, "<i4"); _arr(f, P, "<f8")
+        subprocess.run([REF_BIN, pin, pout], check=True, stdout=subprocess.DEVNULL)
+        return read_out(pout)
