@@ -1,0 +1,597 @@
+// hexopt.cpp -- host side of libhexopt: the context, the SLIM iteration driver and the C ABI
+// declared in include/hexopt.h.
+//
+// The iteration follows slim_solve2's fixed-iteration branch (slim_m.cpp:1269-1288):
+//   local step (update_weights_and_closest_rotations)      -> launch_local_step
+//   global step (solve_weighted_arap: Eigen CG, Jacobi,    -> launch_apply + cg kernels,
+//                tol 1e-8, <= 20 iterations, warm start)     all state on the device
+//   igl::flip_avoiding_line_search + igl::line_search      -> max_step / energy kernels,
+//                                                             the accept test on the host
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hexopt.h"
+#include "kernels.h"
+#include "layout.h"
+
+namespace hexopt {
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t ensure(size_t nb) {
+    if (nb <= bytes && p) return hipSuccess;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    if (nb == 0) nb = 8;
+    hipError_t e = hipMalloc(&p, nb);
+    if (e == hipSuccess) bytes = nb;
+    return e;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+}  // namespace
+}  // namespace hexopt
+
+using namespace hexopt;
+
+struct hexopt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  MeshLayout L;
+  std::string err;
+  // layout on the device
+  DevBuf d_conn, d_tile_hex, d_tile_col, d_tile_lv, d_lv_gid, d_lv_slot, d_lidx, d_sv_gid, d_sv_off, d_sv_slots;
+  TileMesh tm{};
+  // positions (SoA, Morton numbering)
+  DevBuf d_rest, d_cur;
+  // reference operator
+  DevBuf d_invs, d_B, d_hflag, d_count;
+  int ref_mode = -1;
+  // SLIM state
+  DevBuf d_K, d_rhs, d_invd, d_x, d_r, d_z, d_p, d_tmp, d_halo, d_cg, d_part[8], d_scal;
+  // constraints
+  DevBuf d_cidx, d_S, d_b, d_cline, d_bids, d_bpos, d_cids, d_cpos, d_tids, d_tn, d_td, d_lv, d_lt, d_lo, d_rhs_a,
+      d_invd_a, d_aL;
+  SoftDev sd{};
+  int nL = 0;
+  std::vector<double> h_lt;  // nL*3 (caller copy for rhs/diag of the a_L rows)
+  // adjoint scratch
+  DevBuf d_X, d_G, d_v;
+};
+
+namespace {
+
+#define HIPCHK(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                      \
+      return HEXOPT_EHIP;                                                                   \
+    }                                                                                       \
+  } while (0)
+
+int fail(hexopt_ctx* c, int code, const char* msg) {
+  c->err = msg;
+  return code;
+}
+
+template <class T>
+int upload(hexopt_ctx* c, DevBuf& b, const std::vector<T>& v) {
+  HIPCHK(c, b.ensure(v.size() * sizeof(T)));
+  if (!v.empty()) HIPCHK(c, hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return HEXOPT_OK;
+}
+
+// caller rows (nV x 3) -> SoA Morton order
+void to_soa(const MeshLayout& L, const double* V, std::vector<double>& out) {
+  out.resize((size_t)3 * L.nV);
+  for (int i = 0; i < L.nV; ++i)
+    for (int d = 0; d < 3; ++d) out[(size_t)d * L.nV + i] = V[(size_t)L.vperm[i] * 3 + d];
+}
+void from_soa(const MeshLayout& L, const double* S, double* V) {
+  for (int i = 0; i < L.nV; ++i)
+    for (int d = 0; d < 3; ++d) V[(size_t)L.vperm[i] * 3 + d] = S[(size_t)d * L.nV + i];
+}
+
+int nblocks(long n) { long b = (n + 255) / 256; if (b < 1) b = 1; if (b > 2048) b = 2048; return (int)b; }
+
+RefOp refop(hexopt_ctx* c) { return RefOp{c->ref_mode, c->d_invs.as<double>(), c->d_B.as<double>()}; }
+
+// Scratch sized for the current mesh and constraints.
+int ensure_state(hexopt_ctx* c) {
+  const int nV = c->L.nV, nH = c->L.nH;
+  const size_t n = (size_t)3 * nV + c->nL;
+  HIPCHK(c, c->d_K.ensure((size_t)48 * nH * sizeof(double)));
+  for (DevBuf* b : {&c->d_rhs, &c->d_invd, &c->d_x, &c->d_r, &c->d_z, &c->d_p, &c->d_tmp})
+    HIPCHK(c, b->ensure(n * sizeof(double)));
+  HIPCHK(c, c->d_halo.ensure((size_t)6 * (c->L.nSlots + 1) * sizeof(double)));
+  HIPCHK(c, c->d_cg.ensure(cg_state_bytes()));
+  const size_t np = (size_t)std::max(std::max(std::max(c->L.nTiles, nblocks(n)), (nH + 255) / 256 + 1), 2048) + 64;
+  const size_t npf = (size_t)(c->L.nShared + 255) / 256 + 1;
+  for (int i = 0; i < 8; ++i) HIPCHK(c, c->d_part[i].ensure(std::max(np, npf) * sizeof(double)));
+  HIPCHK(c, c->d_scal.ensure(16 * sizeof(double)));
+  return HEXOPT_OK;
+}
+
+int energy_now(hexopt_ctx* c, const hexopt_params* prm, const double* dst, double step, const double* aL, double* eq,
+               double* es) {
+  const int nsb = nblocks(c->sd.nb + c->sd.nC + c->sd.nT + c->sd.nL);
+  double* out = c->d_scal.as<double>();
+  launch_energy(refop(c), c->d_cur.as<double>(), dst, step, c->L.nV, c->tm.conn, c->L.nH, prm->energy, prm->weight_opt,
+                prm->exp_factor, c->sd, aL, c->d_part[0].as<double>(), c->d_part[1].as<double>(), nsb, out, c->stream);
+  double h[2];
+  HIPCHK(c, hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *eq = h[0];
+  *es = h[1];
+  return HEXOPT_OK;
+}
+
+SoftDev soft_with(hexopt_ctx* c, const hexopt_params* prm) {
+  SoftDev s = c->sd;
+  s.lamP = prm->lamda_projection;
+  if (!prm->projection) s.nb = 0;
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hexopt_create_error(void) { return g_create_error.c_str(); }
+
+hexopt_ctx* hexopt_create(int nV, const double* V, int nH, const int32_t* hexes, int device) {
+  g_create_error.clear();
+  if (nV <= 0 || nH < 0 || !V || (nH > 0 && !hexes)) { g_create_error = "invalid mesh arguments"; return nullptr; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { g_create_error = "no HIP device"; return nullptr; }
+  if (device < 0 || device >= ndev) { g_create_error = "device index out of range"; return nullptr; }
+  hexopt_ctx* c = new hexopt_ctx();
+  c->device = device;
+  const char* e = nullptr;
+  if (!build_layout(nV, V, nH, hexes, c->L, &e)) { g_create_error = e; delete c; return nullptr; }
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    g_create_error = "hipStreamCreate failed";
+    delete c;
+    return nullptr;
+  }
+  const MeshLayout& L = c->L;
+  int rc = HEXOPT_OK;
+  rc |= upload(c, c->d_conn, L.conn);
+  rc |= upload(c, c->d_tile_hex, L.tile_hex);
+  rc |= upload(c, c->d_tile_col, L.tile_col);
+  rc |= upload(c, c->d_tile_lv, L.tile_lv);
+  rc |= upload(c, c->d_lv_gid, L.lv_gid);
+  rc |= upload(c, c->d_lv_slot, L.lv_slot);
+  rc |= upload(c, c->d_lidx, L.lidx);
+  rc |= upload(c, c->d_sv_gid, L.sv_gid);
+  rc |= upload(c, c->d_sv_off, L.sv_off);
+  rc |= upload(c, c->d_sv_slots, L.sv_slots);
+  std::vector<double> soa;
+  to_soa(L, V, soa);
+  rc |= upload(c, c->d_rest, soa);
+  rc |= upload(c, c->d_cur, soa);
+  if (rc != HEXOPT_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
+    g_create_error = "device upload failed: " + c->err;
+    hexopt_destroy(c);
+    return nullptr;
+  }
+  c->tm = TileMesh{L.nV, L.nH, L.nTiles, L.nShared, L.nSlots, c->d_conn.as<int>(), c->d_tile_hex.as<int>(),
+                   c->d_tile_col.as<uint16_t>(), c->d_tile_lv.as<int>(), c->d_lv_gid.as<int>(),
+                   c->d_lv_slot.as<int>(), c->d_lidx.as<uint16_t>(), c->d_sv_gid.as<int>(), c->d_sv_off.as<int>(),
+                   c->d_sv_slots.as<int>()};
+  hexopt_constraints none{};
+  if (hexopt_set_constraints(c, &none) != HEXOPT_OK) {
+    g_create_error = c->err;
+    hexopt_destroy(c);
+    return nullptr;
+  }
+  return c;
+}
+
+void hexopt_destroy(hexopt_ctx* c) {
+  if (!c) return;
+  if (c->stream) { (void)hipStreamSynchronize(c->stream); }
+  hipStream_t s = c->stream;
+  delete c;  // frees device buffers
+  if (s) (void)hipStreamDestroy(s);
+}
+
+const char* hexopt_last_error(hexopt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int hexopt_set_positions(hexopt_ctx* c, int which, const double* V) {
+  if (!c || !V || (which != HEXOPT_REST && which != HEXOPT_CURRENT)) return HEXOPT_EINVAL;
+  std::vector<double> soa;
+  to_soa(c->L, V, soa);
+  DevBuf& b = which == HEXOPT_REST ? c->d_rest : c->d_cur;
+  HIPCHK(c, hipMemcpyAsync(b.p, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HEXOPT_OK;
+}
+
+int hexopt_get_positions(hexopt_ctx* c, int which, double* V) {
+  if (!c || !V || (which != HEXOPT_REST && which != HEXOPT_CURRENT)) return HEXOPT_EINVAL;
+  std::vector<double> soa((size_t)3 * c->L.nV);
+  DevBuf& b = which == HEXOPT_REST ? c->d_rest : c->d_cur;
+  HIPCHK(c, hipMemcpyAsync(soa.data(), b.p, soa.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  from_soa(c->L, soa.data(), V);
+  return HEXOPT_OK;
+}
+
+double* hexopt_device_positions(hexopt_ctx* c, int which) {
+  if (!c) return nullptr;
+  return which == HEXOPT_REST ? c->d_rest.as<double>() : c->d_cur.as<double>();
+}
+
+int hexopt_reference(hexopt_ctx* c, const uint8_t* hflag, int cube) {
+  if (!c) return HEXOPT_EINVAL;
+  const int nH = c->L.nH, nV = c->L.nV;
+  const uint8_t* dfl = nullptr;
+  if (hflag) {
+    std::vector<uint8_t> hf(nH);
+    for (int i = 0; i < nH; ++i) hf[i] = hflag[c->L.hperm[i]];
+    if (upload(c, c->d_hflag, hf) != HEXOPT_OK) return HEXOPT_EHIP;
+    dfl = c->d_hflag.as<uint8_t>();
+  }
+  HIPCHK(c, c->d_invs.ensure((size_t)8 * nH * sizeof(double) + 8));
+  HIPCHK(c, c->d_count.ensure(sizeof(int)));
+  HIPCHK(c, hipMemsetAsync(c->d_count.p, 0, sizeof(int), c->stream));
+  launch_ref_cube(c->d_rest.as<double>(), nV, c->tm.conn, nH, dfl, cube, c->d_invs.as<double>(), c->d_count.as<int>(),
+                  c->stream);
+  int ngen = 0;
+  HIPCHK(c, hipMemcpyAsync(&ngen, c->d_count.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (ngen == 0) {
+    c->ref_mode = REF_CUBE;
+  } else {
+    HIPCHK(c, c->d_B.ensure((size_t)72 * nH * sizeof(double) + 8));
+    launch_ref_general(c->d_rest.as<double>(), nV, c->tm.conn, nH, dfl, cube, c->d_B.as<double>(), c->stream);
+    c->ref_mode = REF_GENERAL;
+  }
+  HIPCHK(c, hipGetLastError());
+  return HEXOPT_OK;
+}
+
+int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
+  if (!c || !cs) return HEXOPT_EINVAL;
+  const MeshLayout& L = c->L;
+  const int nV = L.nV;
+  auto chk = [&](int n, const int32_t* ids) {
+    for (int i = 0; i < n; ++i)
+      if (ids[i] < 0 || ids[i] >= nV) return false;
+    return true;
+  };
+  if (cs->nb < 0 || cs->nC < 0 || cs->nT < 0 || cs->nL < 0) return fail(c, HEXOPT_EINVAL, "negative constraint count");
+  if ((cs->nb && (!cs->b || !cs->bc || !chk(cs->nb, cs->b))) || (cs->nC && (!cs->ids_C || !cs->C || !chk(cs->nC, cs->ids_C))) ||
+      (cs->nT && (!cs->ids_T || !cs->normal_T || !cs->dis_T || !chk(cs->nT, cs->ids_T))) ||
+      (cs->nL && (!cs->ids_L || !cs->Axa_L || !cs->origin_L || !chk(cs->nL, cs->ids_L))))
+    return fail(c, HEXOPT_EINVAL, "invalid constraint arrays");
+  // per-vertex folded rows (projection rows are folded in at solve time only when enabled,
+  // so the projection part is kept in its own rows: index space [0,nV) for the rest,
+  // the projection contribution is added through the b list (see hexopt_slim_solve)).
+  std::vector<int> cidx(nV, -1);
+  std::vector<double> S, b;
+  std::vector<int> cline;
+  auto row = [&](int v) {
+    if (cidx[v] < 0) {
+      cidx[v] = (int)cline.size();
+      cline.push_back(-1);
+      for (int k = 0; k < 6; ++k) S.push_back(0);
+      for (int k = 0; k < 3; ++k) b.push_back(0);
+    }
+    return cidx[v];
+  };
+  for (int i = 0; i < cs->nC; ++i) {
+    int r = row(L.vinv[cs->ids_C[i]]);
+    for (int d = 0; d < 3; ++d) { S[r * 6 + d] += cs->lamda_C; b[r * 3 + d] += cs->lamda_C * cs->C[i * 3 + d]; }
+  }
+  for (int i = 0; i < cs->nT; ++i) {
+    int r = row(L.vinv[cs->ids_T[i]]);
+    const double* n = &cs->normal_T[i * 3];
+    const double l = cs->lamda_T;
+    S[r * 6 + 0] += l * n[0] * n[0]; S[r * 6 + 1] += l * n[1] * n[1]; S[r * 6 + 2] += l * n[2] * n[2];
+    S[r * 6 + 3] += l * n[0] * n[1]; S[r * 6 + 4] += l * n[0] * n[2]; S[r * 6 + 5] += l * n[1] * n[2];
+    for (int d = 0; d < 3; ++d) b[r * 3 + d] += l * n[d] * cs->dis_T[i];
+  }
+  std::vector<int> lv(cs->nL);
+  std::vector<double> rhs_a(cs->nL), invd_a(cs->nL);
+  for (int i = 0; i < cs->nL; ++i) {
+    int v = L.vinv[cs->ids_L[i]];
+    int r = row(v);
+    if (cline[r] >= 0) return fail(c, HEXOPT_EUNSUPPORTED, "vertex with two feature-line constraints");
+    cline[r] = i;
+    lv[i] = v;
+    double s = 0, tt = 0;
+    for (int d = 0; d < 3; ++d) {
+      S[r * 6 + d] += cs->lamda_L;
+      b[r * 3 + d] += cs->lamda_L * cs->origin_L[i * 3 + d];
+      s += -cs->Axa_L[i * 3 + d] * cs->origin_L[i * 3 + d];
+      tt += cs->Axa_L[i * 3 + d] * cs->Axa_L[i * 3 + d];
+    }
+    rhs_a[i] = cs->lamda_L * s;
+    double dg = cs->lamda_L * tt;
+    invd_a[i] = dg != 0 ? 1.0 / dg : 1.0;
+  }
+  // projection rows: kept separate (enabled per solve) as extra rows appended after the
+  // feature rows; b_ids/b_pos feed the energy, S/b of these rows are added when enabled.
+  std::vector<int> bids(cs->nb), cids(cs->nC), tids(cs->nT);
+  for (int i = 0; i < cs->nb; ++i) bids[i] = L.vinv[cs->b[i]];
+  for (int i = 0; i < cs->nC; ++i) cids[i] = L.vinv[cs->ids_C[i]];
+  for (int i = 0; i < cs->nT; ++i) tids[i] = L.vinv[cs->ids_T[i]];
+  auto vec = [](const double* p, size_t n) { return std::vector<double>(p ? p : (const double*)nullptr, p ? p + n : nullptr); };
+  c->nL = cs->nL;
+  int rc = HEXOPT_OK;
+  rc |= upload(c, c->d_cidx, cidx);
+  rc |= upload(c, c->d_S, S);
+  rc |= upload(c, c->d_b, b);
+  rc |= upload(c, c->d_cline, cline);
+  rc |= upload(c, c->d_bids, bids);
+  rc |= upload(c, c->d_bpos, vec(cs->bc, (size_t)cs->nb * 3));
+  rc |= upload(c, c->d_cids, cids);
+  rc |= upload(c, c->d_cpos, vec(cs->C, (size_t)cs->nC * 3));
+  rc |= upload(c, c->d_tids, tids);
+  rc |= upload(c, c->d_tn, vec(cs->normal_T, (size_t)cs->nT * 3));
+  rc |= upload(c, c->d_td, vec(cs->dis_T, (size_t)cs->nT));
+  rc |= upload(c, c->d_lv, lv);
+  rc |= upload(c, c->d_lt, vec(cs->Axa_L, (size_t)cs->nL * 3));
+  rc |= upload(c, c->d_lo, vec(cs->origin_L, (size_t)cs->nL * 3));
+  rc |= upload(c, c->d_rhs_a, rhs_a);
+  rc |= upload(c, c->d_invd_a, invd_a);
+  HIPCHK(c, c->d_aL.ensure(((size_t)cs->nL + 1) * sizeof(double)));
+  if (rc != HEXOPT_OK) return HEXOPT_EHIP;
+  SoftDev& s = c->sd;
+  s.cidx = c->d_cidx.as<int>(); s.S = c->d_S.as<double>(); s.b = c->d_b.as<double>(); s.cline = c->d_cline.as<int>();
+  s.nb = cs->nb; s.nC = cs->nC; s.nT = cs->nT; s.nL = cs->nL;
+  s.b_ids = c->d_bids.as<int>(); s.b_pos = c->d_bpos.as<double>();
+  s.c_ids = c->d_cids.as<int>(); s.c_pos = c->d_cpos.as<double>();
+  s.t_ids = c->d_tids.as<int>(); s.t_n = c->d_tn.as<double>(); s.t_d = c->d_td.as<double>();
+  s.lv = c->d_lv.as<int>(); s.lt = c->d_lt.as<double>(); s.lo = c->d_lo.as<double>();
+  s.lamP = 0; s.lamC = cs->lamda_C; s.lamT = cs->lamda_T; s.lamL = cs->lamda_L;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HEXOPT_OK;
+}
+
+// Adds/removes the projection rows (lamda_projection |x_b - bc|^2) to the folded S/b rows.
+static int fold_projection(hexopt_ctx* c, const hexopt_params* prm, std::vector<int>& saved_cidx_dummy) {
+  (void)saved_cidx_dummy;
+  if (!prm->projection || c->sd.nb == 0) return HEXOPT_OK;
+  return fail(c, HEXOPT_EUNSUPPORTED, "projection constraints (ts.projection) are not supported by this build");
+}
+
+int hexopt_slim_solve(hexopt_ctx* c, const hexopt_params* prm, int iters, hexopt_iter_log* log, double* E, double* Eq,
+                      double* Es, double* aL_out) {
+  if (!c || !prm || iters < 0) return HEXOPT_EINVAL;
+  if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");
+  if (prm->energy < 0 || prm->energy > 5) return fail(c, HEXOPT_EINVAL, "unknown energy");
+  std::vector<int> dummy;
+  int rc = fold_projection(c, prm, dummy);
+  if (rc != HEXOPT_OK) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = ensure_state(c)) != HEXOPT_OK) return rc;
+  const int nV = c->L.nV, nH = c->L.nH, nL = c->nL;
+  const int n = 3 * nV + nL;
+  const double M = prm->weight_opt, prox = 0.00001 * prm->weight_opt, mesh_area = M * 8.0 * nH;
+  const double exp_f = prm->exp_factor;
+  hipStream_t st = c->stream;
+  SoftDev sd = soft_with(c, prm);
+  RefOp ro = refop(c);
+  double* aL = c->d_aL.as<double>();
+  if (nL > 0) HIPCHK(c, hipMemsetAsync(aL, 0, nL * sizeof(double), st));
+  // slim_precompute: energy at the start point with a_L = 0
+  double eq = 0, es = 0;
+  if ((rc = energy_now(c, prm, nullptr, 0.0, aL, &eq, &es)) != HEXOPT_OK) return rc;
+  double energy = (eq + es) / mesh_area;
+  hexopt_params pl = *prm;
+  (void)pl;
+  const int nb = nblocks(n);
+  double* x = c->d_x.as<double>();
+  double* cur = c->d_cur.as<double>();
+  for (int it = 0; it < iters; ++it) {
+    // local step: K, rhs, Jacobi diagonal
+    launch_local_step(c->tm, ro, cur, c->d_K.as<double>(), prm->energy, M, exp_f, prox, sd, c->d_rhs.as<double>(),
+                      c->d_invd.as<double>(), c->d_halo.as<double>(), st);
+    if (nL > 0) {
+      HIPCHK(c, hipMemcpyAsync(c->d_rhs.as<double>() + 3 * (size_t)nV, c->d_rhs_a.p, nL * sizeof(double),
+                               hipMemcpyDeviceToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(c->d_invd.as<double>() + 3 * (size_t)nV, c->d_invd_a.p, nL * sizeof(double),
+                               hipMemcpyDeviceToDevice, st));
+      HIPCHK(c, hipMemsetAsync(x + 3 * (size_t)nV, 0, nL * sizeof(double), st));
+    }
+    // CG warm start at the current positions (a_L guess 0)
+    HIPCHK(c, hipMemcpyAsync(x, cur, 3 * (size_t)nV * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double* pt = c->d_part[2].as<double>();
+    double* pf = c->d_part[3].as<double>();
+    double* pl2 = c->d_part[4].as<double>();
+    const int nfix = (c->L.nShared + 255) / 256, nline = (nL + 255) / 256;
+    launch_apply(c->tm, ro, c->d_K.as<double>(), x, c->d_tmp.as<double>(), prox, sd, c->d_halo.as<double>(), pt, pf,
+                 pl2, st);
+    launch_cg_init(n, c->d_rhs.as<double>(), c->d_tmp.as<double>(), c->d_r.as<double>(), c->d_part[0].as<double>(),
+                   c->d_part[1].as<double>(), nb, c->d_cg.p, c->d_invd.as<double>(), c->d_p.as<double>(),
+                   c->d_part[5].as<double>(), 1e-8, st);
+    for (int k = 0; k < 20; ++k) {
+      launch_apply(c->tm, ro, c->d_K.as<double>(), c->d_p.as<double>(), c->d_tmp.as<double>(), prox, sd,
+                   c->d_halo.as<double>(), pt, pf, pl2, st);
+      launch_cg_alpha(c->d_cg.p, pt, c->L.nTiles, pf, nfix, pl2, nline, st);
+      launch_cg_update(n, c->d_cg.p, x, c->d_r.as<double>(), c->d_z.as<double>(), c->d_p.as<double>(),
+                       c->d_tmp.as<double>(), c->d_invd.as<double>(), c->d_part[6].as<double>(),
+                       c->d_part[7].as<double>(), nb, c->d_p.as<double>(), st);
+    }
+    launch_cg_zero_if(n, c->d_cg.p, x, nb, st);
+    if (nL > 0 && !prm->projection)
+      HIPCHK(c, hipMemcpyAsync(aL, x + 3 * (size_t)nV, nL * sizeof(double), hipMemcpyDeviceToDevice, st));
+    // flip-avoiding step bound
+    double* scal = c->d_scal.as<double>();
+    launch_max_step(cur, x, nV, c->tm.conn, nH, c->d_part[0].as<double>(), scal + 4, st);
+    std::vector<unsigned char> cgs(cg_state_bytes());
+    double tmin = 0;
+    HIPCHK(c, hipMemcpyAsync(&tmin, scal + 4, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(cgs.data(), c->d_cg.p, cgs.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(st));
+    int cg_iters = 0, cg_done = 0;
+    double cg_err = 0;
+    read_cg_state(cgs.data(), &cg_iters, &cg_err, &cg_done);
+    double step = std::min(1., tmin * 0.8);
+    const double bound = step;
+    // igl::line_search (line_search.cpp:10-56)
+    double q0, s0;
+    if ((rc = energy_now(c, prm, nullptr, 0.0, aL, &q0, &s0)) != HEXOPT_OK) return rc;
+    const double old_energy = q0 + s0;
+    double new_energy = old_energy;
+    int cur_iter = 0;
+    while (new_energy >= old_energy) {
+      if (cur_iter > 20) step = 0;
+      double q1, s1;
+      if ((rc = energy_now(c, prm, x, step, aL, &q1, &s1)) != HEXOPT_OK) return rc;
+      const double cur_e = q1 + s1;
+      eq = q1;
+      es = s1;
+      if (std::isnan(cur_e) || std::isinf(cur_e) || cur_e > old_energy) {
+        step /= 2;
+      } else {
+        launch_step(3 * nV, cur, x, step, st);
+        new_energy = cur_e;
+        break;
+      }
+      if (step == 0) break;
+      cur_iter++;
+    }
+    energy = new_energy / mesh_area;
+    if (log) {
+      hexopt_iter_log& lg = log[it];
+      lg.energy = energy; lg.e_quality = eq; lg.e_soft = es; lg.step_bound = bound; lg.step = step;
+      lg.cg_iters = cg_iters; lg.cg_error = cg_err;
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(st));
+  if (E) *E = energy;
+  if (Eq) *Eq = eq;
+  if (Es) *Es = es;
+  if (aL_out && nL > 0) HIPCHK(c, hipMemcpy(aL_out, aL, nL * sizeof(double), hipMemcpyDeviceToHost));
+  return HEXOPT_OK;
+}
+
+int hexopt_energy(hexopt_ctx* c, const hexopt_params* prm, const double* aL, double* Eq, double* Es) {
+  if (!c || !prm) return HEXOPT_EINVAL;
+  if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");
+  int rc = ensure_state(c);
+  if (rc) return rc;
+  double* d_aL = c->d_aL.as<double>();
+  if (c->nL > 0) {
+    if (aL) HIPCHK(c, hipMemcpyAsync(d_aL, aL, c->nL * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(c, hipMemsetAsync(d_aL, 0, c->nL * sizeof(double), c->stream));
+  }
+  return energy_now(c, prm, nullptr, 0.0, d_aL, Eq, Es);
+}
+
+static int adj_common(hexopt_ctx* c, int precision, const void* X, const void* v) {
+  const size_t es = precision == 64 ? 8 : 4;
+  const int nV = c->L.nV;
+  HIPCHK(c, c->d_X.ensure((size_t)3 * nV * es));
+  HIPCHK(c, c->d_G.ensure((size_t)3 * nV * es));
+  HIPCHK(c, c->d_v.ensure((size_t)3 * nV * es));
+  HIPCHK(c, c->d_halo.ensure((size_t)6 * (c->L.nSlots + 1) * sizeof(double)));
+  auto soa = [&](const void* src, DevBuf& dst) -> int {
+    std::vector<unsigned char> tmp((size_t)3 * nV * es);
+    for (int i = 0; i < nV; ++i)
+      for (int d = 0; d < 3; ++d)
+        std::memcpy(&tmp[((size_t)d * nV + i) * es], (const unsigned char*)src + ((size_t)c->L.vperm[i] * 3 + d) * es, es);
+    HIPCHK(c, hipMemcpyAsync(dst.p, tmp.data(), tmp.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return HEXOPT_OK;
+  };
+  int rc = soa(X, c->d_X);
+  if (rc) return rc;
+  if (v) rc = soa(v, c->d_v);
+  return rc;
+}
+
+static int adj_out(hexopt_ctx* c, int precision, void* out) {
+  const size_t es = precision == 64 ? 8 : 4;
+  const int nV = c->L.nV;
+  std::vector<unsigned char> tmp((size_t)3 * nV * es);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), c->d_G.p, tmp.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < nV; ++i)
+    for (int d = 0; d < 3; ++d)
+      std::memcpy((unsigned char*)out + ((size_t)c->L.vperm[i] * 3 + d) * es, &tmp[((size_t)d * nV + i) * es], es);
+  return HEXOPT_OK;
+}
+
+int hexopt_energy_grad(hexopt_ctx* c, const hexopt_params* prm, int precision, const void* X, const double* aL, void* G) {
+  if (!c || !prm || !X || !G || (precision != 32 && precision != 64)) return HEXOPT_EINVAL;
+  if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");
+  if (prm->energy != HEXOPT_SYMMETRIC_DIRICHLET) return fail(c, HEXOPT_EUNSUPPORTED, "adjoint supports SYMMETRIC_DIRICHLET");
+  int rc = adj_common(c, precision, X, nullptr);
+  if (rc) return rc;
+  double* d_aL = c->d_aL.as<double>();
+  if (c->nL > 0) {
+    if (aL) HIPCHK(c, hipMemcpyAsync(d_aL, aL, c->nL * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(c, hipMemsetAsync(d_aL, 0, c->nL * sizeof(double), c->stream));
+  }
+  SoftDev sd = soft_with(c, prm);
+  launch_grad(precision, c->tm, refop(c), c->d_X.p, c->d_G.p, prm->weight_opt, sd, d_aL, c->d_halo.p, c->stream);
+  HIPCHK(c, hipGetLastError());
+  return adj_out(c, precision, G);
+}
+
+int hexopt_hvp(hexopt_ctx* c, const hexopt_params* prm, int precision, const void* X, const void* v, void* Hv) {
+  if (!c || !prm || !X || !v || !Hv || (precision != 32 && precision != 64)) return HEXOPT_EINVAL;
+  if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");
+  if (prm->energy != HEXOPT_SYMMETRIC_DIRICHLET) return fail(c, HEXOPT_EUNSUPPORTED, "adjoint supports SYMMETRIC_DIRICHLET");
+  int rc = adj_common(c, precision, X, v);
+  if (rc) return rc;
+  SoftDev sd = soft_with(c, prm);
+  launch_hvp(precision, c->tm, refop(c), c->d_X.p, c->d_v.p, c->d_G.p, prm->weight_opt, sd, c->d_halo.p, c->stream);
+  HIPCHK(c, hipGetLastError());
+  return adj_out(c, precision, Hv);
+}
+
+int hexopt_scaled_jacobian(hexopt_ctx* c, int which, double* min_j, double* avg_j, double* per_hex) {
+  if (!c || (which != HEXOPT_REST && which != HEXOPT_CURRENT)) return HEXOPT_EINVAL;
+  int rc = ensure_state(c);
+  if (rc) return rc;
+  const int nH = c->L.nH;
+  DevBuf ph;
+  if (per_hex) HIPCHK(c, ph.ensure((size_t)nH * sizeof(double) + 8));
+  double* out = c->d_scal.as<double>();
+  launch_scaled_jacobian((which == HEXOPT_REST ? c->d_rest : c->d_cur).as<double>(), c->L.nV, c->tm.conn, nH,
+                         per_hex ? ph.as<double>() : nullptr, c->d_part[0].as<double>(), c->d_part[1].as<double>(), out,
+                         c->stream);
+  double h[2] = {1, 0};
+  if (nH > 0) HIPCHK(c, hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> tmp(per_hex ? nH : 0);
+  if (per_hex && nH) HIPCHK(c, hipMemcpyAsync(tmp.data(), ph.p, nH * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (min_j) *min_j = nH ? h[0] : 1.0;
+  if (avg_j) *avg_j = nH ? h[1] / nH : 0.0;
+  if (per_hex)
+    for (int i = 0; i < nH; ++i) per_hex[c->L.hperm[i]] = tmp[i];
+  return HEXOPT_OK;
+}
+
+int hexopt_synchronize(hexopt_ctx* c) {
+  if (!c) return HEXOPT_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HEXOPT_OK;
+}
+
+int hexopt_layout_info(hexopt_ctx* c, int32_t* o) {
+  if (!c || !o) return HEXOPT_EINVAL;
+  o[0] = c->L.nTiles; o[1] = c->L.nShared; o[2] = c->L.nSlots; o[3] = c->L.max_colours; o[4] = c->ref_mode;
+  return HEXOPT_OK;
+}
+
+}  // extern "C"

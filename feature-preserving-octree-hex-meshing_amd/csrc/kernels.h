@@ -1,0 +1,81 @@
+// kernels.h -- device data views and launch entry points of kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "layout.h"
+
+namespace hexopt {
+
+// Device view of a MeshLayout.
+struct TileMesh {
+  int nV, nH, nTiles, nShared, nSlots;
+  const int* conn;
+  const int* tile_hex;
+  const uint16_t* tile_col;
+  const int* tile_lv;
+  const int* lv_gid;
+  const int* lv_slot;
+  const uint16_t* lidx;
+  const int* sv_gid;
+  const int* sv_off;
+  const int* sv_slots;
+};
+
+enum RefMode { REF_CUBE = 0, REF_GENERAL = 1 };
+
+// Reference operator per corner tet: invs (cube) or B (general), see kernels.hip.
+struct RefOp {
+  int mode;
+  const double* invs;  // [8][nH]
+  const double* B;     // [8*9][nH]
+};
+
+// Soft constraints (Feature_Constraints + projection terms of Tetralize_Set).
+struct SoftDev {
+  // per-vertex folded form used by the linear operator and the gradient
+  const int* cidx;     // nV: -1 or row in S/b/cline
+  const double* S;     // rows*6: sum of constraint Hessian halves (xx yy zz xy xz yz)
+  const double* b;     // rows*3: constraint rhs
+  const int* cline;    // rows: -1 or feature-line index
+  // entry lists used for the energy (the reference's own expression per entry)
+  int nb, nC, nT, nL;
+  const int* b_ids; const double* b_pos;
+  const int* c_ids; const double* c_pos;
+  const int* t_ids; const double* t_n; const double* t_d;
+  const int* lv; const double* lt; const double* lo;
+  double lamP, lamC, lamT, lamL;
+};
+
+void launch_ref_cube(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube, double* invs,
+                     int* n_general, hipStream_t st);
+void launch_ref_general(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube, double* B,
+                        hipStream_t st);
+void launch_local_step(const TileMesh& tm, const RefOp& ro, const double* X, double* K, int energy, double M,
+                       double exp_f, double prox, const SoftDev& sd, double* rhs, double* invd, double* halo,
+                       hipStream_t st);
+void launch_apply(const TileMesh& tm, const RefOp& ro, const double* K, const double* p, double* out, double prox,
+                  const SoftDev& sd, double* halo, double* part_tile, double* part_fix, double* part_line,
+                  hipStream_t st);
+void launch_cg_init(int n, const double* rhs, const double* tmp, double* r, double* pb, double* pr, int nb,
+                    void* state, const double* invd, double* p, double* pabs, double tol, hipStream_t st);
+void launch_cg_alpha(void* state, const double* p1, int n1, const double* p2, int n2, const double* p3, int n3,
+                     hipStream_t st);
+void launch_cg_update(int n, void* state, double* x, double* r, double* z, const double* p, const double* tmp,
+                      const double* invd, double* prr, double* prz, int nb, double* p_out, hipStream_t st);
+void launch_cg_zero_if(int n, void* state, double* x, int nb, hipStream_t st);
+size_t cg_state_bytes();
+void read_cg_state(const void* host_state, int* iters, double* tol_error, int* done);
+void launch_max_step(const double* cur, const double* dst, int nV, const int* conn, int nH, double* part, double* out,
+                     hipStream_t st);
+void launch_energy(const RefOp& ro, const double* cur, const double* dst, double step, int nV, const int* conn, int nH,
+                   int energy, double M, double exp_f, const SoftDev& sd, const double* aL, double* part_q,
+                   double* part_s, int ns_blocks, double* out2, hipStream_t st);
+void launch_step(int n, double* cur, const double* dst, double step, hipStream_t st);
+void launch_grad(int prec, const TileMesh& tm, const RefOp& ro, const void* X, void* G, double M, const SoftDev& sd,
+                 const double* aL, void* halo, hipStream_t st);
+void launch_hvp(int prec, const TileMesh& tm, const RefOp& ro, const void* X, const void* Vd, void* Hv, double M,
+                const SoftDev& sd, void* halo, hipStream_t st);
+void launch_scaled_jacobian(const double* X, int nV, const int* conn, int nH, double* per_hex, double* pmin,
+                            double* psum, double* out3, hipStream_t st);
+
+}  // namespace hexopt

@@ -1,0 +1,1116 @@
+// kernels.hip -- HIP kernels of the hex-mesh SLIM optimiser for gfx950 (MI355X).
+//
+// One workgroup of 256 lanes (4 wave64s) owns one tile of up to 256 Morton-ordered hexes:
+// lane i computes the 8 corner tets of hex i in registers, the per-corner results are
+// accumulated into an LDS vertex buffer colour by colour (no atomics), then every tile
+// vertex is either written straight to HBM (all its hexes are in this tile) or parked in a
+// halo slot that the fixup kernel folds in a fixed order. Reductions are two-level and in a
+// fixed order, so every kernel is bitwise deterministic.
+//
+// Data layout (structure of arrays, all arrays indexed in the Morton numbering):
+//   vertex fields   x[d*nV + v]           d = 0,1,2 (the reference's flattened unknowns)
+//   connectivity    conn[k*nH + h]         k = hex corner in the reference's Hybrid::vs order
+//   per-tet scalars f[(c*F + j)*nH + h]    c = corner tet, j = field component
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "device_math.h"
+#include "kernels.h"
+
+namespace hexopt {
+
+// ---------------------------------------------------------------------------------------
+// helpers
+
+// Maps a workgroup id to a tile so that consecutive tiles (neighbours along the Morton
+// curve, which share halo vertices) run on the same XCD and hit the same L2: hardware
+// hands workgroup b to XCD b % 8.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n / 8, r = n % 8;
+  const int x = b % 8, k = b / 8;
+  return x * q + (x < r ? x : r) + k;
+}
+
+template <class T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+  // wave64 butterfly, then the 4 wave sums in fixed order
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+
+__device__ __forceinline__ double block_min(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double s = red[0];
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int i = 1; i < nw; ++i) s = fmin(s, red[i]);
+  return s;
+}
+
+// Reference operator of corner tet c: J = [x_n1-x_c, x_n2-x_c, x_n3-x_c] * B.
+template <class T>
+__device__ __forceinline__ void tet_B(const RefOp& ro, int nH, int h, int c, T B[3][3]) {
+  if (ro.mode == REF_CUBE) {
+    const T s = (T)ro.invs[(size_t)c * nH + h];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) B[i][j] = s * (T)kPi[c][i][j];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) B[i][j] = (T)ro.B[((size_t)c * 9 + i * 3 + j) * nH + h];
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void tet_edges(const T p[8][3], int c, T E[3][3]) {
+  const int v0 = kHexTet[c][0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int vk = kHexTet[c][k + 1];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) E[d][k] = p[vk][d] - p[v0][d];
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void matmul3(const T A[3][3], const T B[3][3], T C[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[i][j] = A[i][0] * B[0][j] + A[i][1] * B[1][j] + A[i][2] * B[2][j];
+}
+
+// Adds the vertex forces of one tet from F = dE/dJ-like 3x3 (columns = reference axes):
+// force on edge k = (F B^T)[:,k]; + to n_k, - to the corner.
+template <class T>
+__device__ __forceinline__ void scatter_tet(const T F[3][3], const T B[3][3], int c, T out[8][3]) {
+  const int v0 = kHexTet[c][0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int vk = kHexTet[c][k + 1];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T f = F[d][0] * B[k][0] + F[d][1] * B[k][1] + F[d][2] * B[k][2];
+      out[vk][d] += f;
+      out[v0][d] -= f;
+    }
+  }
+}
+
+template <class P>
+__device__ __forceinline__ void gather8(const P* x, int nV, const int* conn, int nH, int h, int vid[8], double p[8][3]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) vid[k] = conn[(size_t)k * nH + h];
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) p[k][d] = (double)x[(size_t)d * nV + vid[k]];
+}
+
+// ---------------------------------------------------------------------------------------
+// Tiled scatter skeleton. Op provides:
+//   __device__ void hex(int h, T out[8][NC])          per-hex corner contributions
+//   __device__ T finish(int v, T acc[NC])              writes vertex v; returns its share of
+//                                                       the kernel's optional block sum
+template <class T, int NC, class Op>
+__global__ __launch_bounds__(kTileHex) void tile_scatter_kernel(TileMesh tm, Op op, T* halo, T* partial) {
+  __shared__ T acc[NC * kMaxLV];
+  __shared__ T red[16];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int h0 = tm.tile_hex[tile], nh = tm.tile_hex[tile + 1] - h0;
+  const int lv0 = tm.tile_lv[tile], nlv = tm.tile_lv[tile + 1] - lv0;
+  for (int i = tid; i < NC * kMaxLV; i += kTileHex) acc[i] = T(0);
+  T c[8][NC];
+  int li[8];
+  const bool active = tid < nh;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) c[k][j] = T(0);
+  if (active) {
+    op.hex(h0 + tid, c);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) li[k] = tm.lidx[(size_t)k * tm.nH + h0 + tid];
+  }
+  __syncthreads();
+  const uint16_t* off = tm.tile_col + (size_t)tile * (kMaxCol + 1);
+  for (int col = 0; col < kMaxCol; ++col) {
+    const int a = off[col], b = off[col + 1];
+    if (a >= nh) break;
+    if (active && tid >= a && tid < b) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[j * kMaxLV + li[k]] += c[k][j];
+    }
+    __syncthreads();
+  }
+  T part = T(0);
+  for (int lv = tid; lv < nlv; lv += kTileHex) {
+    const int g = tm.lv_gid[lv0 + lv], s = tm.lv_slot[lv0 + lv];
+    T v[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) v[j] = acc[j * kMaxLV + lv];
+    if (s < 0) {
+      part += op.finish(g, v);
+    } else {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) halo[(size_t)j * tm.nSlots + s] = v[j];
+    }
+  }
+  if (partial) {
+    T s = block_sum(part, red);
+    if (tid == 0) partial[tile] = s;
+  }
+}
+
+template <class T, int NC, class Op>
+__global__ __launch_bounds__(256) void fixup_kernel(TileMesh tm, Op op, const T* halo, T* partial) {
+  __shared__ T red[16];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  T part = T(0);
+  if (i < tm.nShared) {
+    const int g = tm.sv_gid[i];
+    T v[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) v[j] = T(0);
+    for (int k = tm.sv_off[i]; k < tm.sv_off[i + 1]; ++k) {
+      const int s = tm.sv_slots[k];
+#pragma unroll
+      for (int j = 0; j < NC; ++j) v[j] += halo[(size_t)j * tm.nSlots + s];
+    }
+    part = op.finish(g, v);
+  }
+  if (partial) {
+    T s = block_sum(part, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+template <class T, int NC, class Op>
+void launch_scatter(const TileMesh& tm, const Op& op, T* halo, T* part_tile, T* part_fix, hipStream_t st) {
+  if (tm.nTiles > 0)
+    hipLaunchKernelGGL((tile_scatter_kernel<T, NC, Op>), dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, op, halo, part_tile);
+  if (tm.nShared > 0)
+    hipLaunchKernelGGL((fixup_kernel<T, NC, Op>), dim3((tm.nShared + 255) / 256), dim3(256), 0, st, tm, op,
+                       (const T*)halo, part_fix);
+}
+
+// ---------------------------------------------------------------------------------------
+// Reference operator: compute_referenceMesh + grad_tet_ref (uniform) per corner tet.
+
+// hex2tet24 (global_functions.cpp:3994-4022)
+__device__ double hex_volume24(const double p[8][3]) {
+  const int F[6][4] = {{0, 1, 2, 3}, {4, 7, 6, 5}, {0, 4, 5, 1}, {0, 3, 7, 4}, {3, 2, 6, 7}, {1, 5, 6, 2}};
+  double hc[3] = {0, 0, 0};
+  for (int i = 0; i < 8; ++i)
+    for (int d = 0; d < 3; ++d) hc[d] += p[i][d];
+  for (int d = 0; d < 3; ++d) hc[d] /= 8;
+  double vol = 0;
+  for (int i = 0; i < 6; ++i) {
+    double fc[3] = {0, 0, 0};
+    for (int j = 0; j < 4; ++j)
+      for (int d = 0; d < 3; ++d) fc[d] += p[F[i][j]][d];
+    for (int d = 0; d < 3; ++d) fc[d] /= 4;
+    for (int j = 0; j < 4; ++j) {
+      double M[3][3];
+      for (int d = 0; d < 3; ++d) {
+        M[d][0] = p[F[i][j]][d] - hc[d];
+        M[d][1] = p[F[i][(j + 1) % 4]][d] - hc[d];
+        M[d][2] = fc[d] - hc[d];
+      }
+      vol += fabs(det3(M) / 6);
+    }
+  }
+  return vol;
+}
+
+// signed tet volume (igl::volume convention) of corner tet c
+__device__ __forceinline__ double corner_volume(const double p[8][3], int c) {
+  const double* a = p[kHexTet[c][0]];
+  const double* b = p[kHexTet[c][1]];
+  const double* cc = p[kHexTet[c][2]];
+  const double* d = p[kHexTet[c][3]];
+  double ad[3] = {a[0] - d[0], a[1] - d[1], a[2] - d[2]};
+  double bd[3] = {b[0] - d[0], b[1] - d[1], b[2] - d[2]};
+  double cd[3] = {cc[0] - d[0], cc[1] - d[1], cc[2] - d[2]};
+  double cr[3] = {bd[1] * cd[2] - bd[2] * cd[1], bd[2] * cd[0] - bd[0] * cd[2], bd[0] * cd[1] - bd[1] * cd[0]};
+  return -(ad[0] * cr[0] + ad[1] * cr[1] + ad[2] * cr[2]) / 6.0;
+}
+
+// grad_tet_ref (grad.cpp:372-478) for a general reference tet r: returns B (J = E B).
+__device__ void general_B(const double x[4][3], const double r[4][3], double vol, double B[3][3]) {
+  (void)x;
+  double e01[3], e02[3], e03[3];
+  for (int d = 0; d < 3; ++d) { e01[d] = r[0][d] - r[1][d]; e02[d] = r[0][d] - r[2][d]; e03[d] = r[0][d] - r[3][d]; }
+  double cr[3] = {e01[1] * e02[2] - e01[2] * e02[1], e01[2] * e02[0] - e01[0] * e02[2], e01[0] * e02[1] - e01[1] * e02[0]};
+  double vref = fabs(cr[0] * e03[0] + cr[1] * e03[1] + cr[2] * e03[2]) / 6;
+  double scale = cbrt(vol / vref);
+  const int faces[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 3, 2}};
+  const int opp[4] = {3, 1, 2, 0};
+  double G[3][4];
+  for (int k = 0; k < 4; ++k) {
+    double a[3], b[3];
+    for (int d = 0; d < 3; ++d) {
+      a[d] = (r[faces[k][0]][d] - r[faces[k][1]][d]) * scale;
+      b[d] = (r[faces[k][0]][d] - r[faces[k][2]][d]) * scale;
+    }
+    double c1[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    double area = sqrt(c1[0] * c1[0] + c1[1] * c1[1] + c1[2] * c1[2]) / 2.0;
+    double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]), nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    for (int d = 0; d < 3; ++d) { a[d] /= na; b[d] /= nb; }
+    double n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    double w = area / (3.0 * vol);
+    for (int d = 0; d < 3; ++d) G[d][opp[k]] = w * (n[d] / nn);
+  }
+  for (int k = 0; k < 3; ++k)
+    for (int c = 0; c < 3; ++c) B[k][c] = G[c][k + 1];
+}
+
+// Pass 1: per hex, cube-reference operator if the hex qualifies (compute_referenceMesh with
+// cube=true, |vol| > 1e-20, H_flag true); counts the hexes that need the general operator.
+__global__ void ref_cube_kernel(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube,
+                                double* invs, int* n_general) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nH) return;
+  int vid[8];
+  double p[8][3];
+  gather8(X, nV, conn, nH, h, vid, p);
+  bool ok = cube && (hflag == nullptr || hflag[h] != 0);
+  if (ok) {
+    double vol = hex_volume24(p);
+    ok = fabs(vol) > 1e-20;
+  }
+  if (!ok) { atomicAdd(n_general, 1); return; }
+  for (int c = 0; c < 8; ++c) invs[(size_t)c * nH + h] = 1.0 / cbrt(6.0 * corner_volume(p, c));
+}
+
+// Pass 2 (only when some hex needs it): the general operator B for every tet.
+__global__ void ref_general_kernel(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube,
+                                   double* Bout) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nH) return;
+  int vid[8];
+  double p[8][3];
+  gather8(X, nV, conn, nH, h, vid, p);
+  double r8[8][3];
+  const bool hf = hflag == nullptr || hflag[h] != 0;
+  if (!hf) {
+    for (int i = 0; i < 8; ++i)
+      for (int d = 0; d < 3; ++d) r8[i][d] = p[i][d];
+  } else {
+    double vol = hex_volume24(p);
+    double e0 = 0, e1 = 0, e2 = 0;
+    auto L = [&](int a, int b) {
+      double q0 = p[a][0] - p[b][0], q1 = p[a][1] - p[b][1], q2 = p[a][2] - p[b][2];
+      return sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+    };
+    if (cube && fabs(vol) > 1e-20) {
+      e0 = e1 = e2 = cbrt(vol);
+    } else {
+      e0 += L(0, 1); e0 += L(3, 2); e0 += L(4, 5); e0 += L(7, 6); e0 /= 4;
+      e1 += L(0, 3); e1 += L(1, 2); e1 += L(4, 7); e1 += L(5, 6); e1 /= 4;
+      e2 += L(0, 4); e2 += L(1, 5); e2 += L(2, 6); e2 += L(3, 7); e2 /= 4;
+      double ratio = cbrt(vol / (e0 * e1 * e2));
+      e0 *= ratio; e1 *= ratio; e2 *= ratio;
+    }
+    const double cc[8][3] = {{0, 0, 0}, {e0, 0, 0}, {e0, e1, 0}, {0, e1, 0},
+                             {0, 0, e2}, {e0, 0, e2}, {e0, e1, e2}, {0, e1, e2}};
+    for (int i = 0; i < 8; ++i)
+      for (int d = 0; d < 3; ++d) r8[i][d] = cc[i][d];
+  }
+  for (int c = 0; c < 8; ++c) {
+    double x[4][3], r[4][3];
+    for (int j = 0; j < 4; ++j)
+      for (int d = 0; d < 3; ++d) { x[j][d] = p[kHexTet[c][j]][d]; r[j][d] = r8[kHexTet[c][j]][d]; }
+    double B[3][3];
+    general_B(x, r, corner_volume(p, c), B);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Bout[((size_t)c * 9 + i * 3 + j) * nH + h] = B[i][j];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Soft-constraint epilogue data (per vertex): S (sym 3x3, xx yy zz xy xz yz), rhs b, line.
+__device__ __forceinline__ void soft_apply(const SoftDev& sd, int g, const double p[3], int nV, const double* pa,
+                                           double out[3]) {
+  const int ci = sd.cidx ? sd.cidx[g] : -1;
+  if (ci < 0) return;
+  const double* S = sd.S + (size_t)ci * 6;
+  out[0] += S[0] * p[0] + S[3] * p[1] + S[4] * p[2];
+  out[1] += S[3] * p[0] + S[1] * p[1] + S[5] * p[2];
+  out[2] += S[4] * p[0] + S[5] * p[1] + S[2] * p[2];
+  const int li = sd.cline[ci];
+  if (li >= 0) {
+    const double a = pa[li];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[d] -= sd.lamL * sd.lt[(size_t)li * 3 + d] * a;
+  }
+  (void)nV;
+}
+
+// ---------------------------------------------------------------------------------------
+// Local step: per tet SVD -> W, R (slim_m.cpp:236-377); stores K = M W^T W (scaled by
+// invs^2 for the cube operator) and scatters the rhs A^T M (W R) (buildRhs, :1044-1092) and
+// the Jacobi diagonal of A^T M A (the DiagonalPreconditioner of the CG).
+struct LocalStepOp {
+  RefOp ro; int nV, nH; const int* conn; const double* X; double* K;
+  int energy; double M, exp_f, prox;
+  SoftDev sd; double* rhs; double* invd;
+  __device__ void hex(int h, double out[8][6]) const {
+    int vid[8];
+    double p[8][3];
+    gather8(X, nV, conn, nH, h, vid, p);
+#pragma unroll 1
+    for (int c = 0; c < 8; ++c) {
+      double B[3][3], E[3][3], J[3][3], W[3][3], R[3][3];
+      tet_B(ro, nH, h, c, B);
+      tet_edges(p, c, E);
+      matmul3(E, B, J);
+      slim_weights(J, energy, exp_f, W, R);
+      double Kt[3][3];  // M W^T W
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Kt[i][j] = M * (W[0][i] * W[0][j] + W[1][i] * W[1][j] + W[2][i] * W[2][j]);
+      double s2 = 1.0;
+      if (ro.mode == REF_CUBE) { double s = ro.invs[(size_t)c * nH + h]; s2 = s * s; }
+      const double ks[6] = {Kt[0][0], Kt[1][1], Kt[2][2], Kt[0][1], Kt[0][2], Kt[1][2]};
+#pragma unroll
+      for (int j = 0; j < 6; ++j) K[((size_t)c * 6 + j) * nH + h] = ks[j] * s2;
+      // rhs: (W^T M W R) scattered through B
+      double Z[3][3];
+      {
+        double F[3][3];
+        matmul3(W, R, F);
+#pragma unroll
+        for (int b2 = 0; b2 < 3; ++b2)
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) Z[b2][cc] = M * (W[0][b2] * F[0][cc] + W[1][b2] * F[1][cc] + W[2][b2] * F[2][cc]);
+      }
+      double tmp[8][3];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) tmp[k][0] = tmp[k][1] = tmp[k][2] = 0;
+      scatter_tet(Z, B, c, tmp);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[k][d] += tmp[k][d];
+      // diagonal: Kt_bb * |G[:,j]|^2; G[:,k] = B[k-1,:], G[:,0] = -sum_k B[k,:]
+      double g0[3] = {-(B[0][0] + B[1][0] + B[2][0]), -(B[0][1] + B[1][1] + B[2][1]), -(B[0][2] + B[1][2] + B[2][2])};
+      double n0 = g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) out[kHexTet[c][0]][3 + d] += Kt[d][d] * n0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double nk = B[k][0] * B[k][0] + B[k][1] * B[k][1] + B[k][2] * B[k][2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[kHexTet[c][k + 1]][3 + d] += Kt[d][d] * nk;
+      }
+    }
+  }
+  __device__ double finish(int g, double acc[6]) const {
+    double dg[3] = {acc[3] + prox, acc[4] + prox, acc[5] + prox};
+    double r[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r[d] = acc[d] + prox * X[(size_t)d * nV + g];
+    const int ci = sd.cidx ? sd.cidx[g] : -1;
+    if (ci >= 0) {
+      const double* S = sd.S + (size_t)ci * 6;
+      const double* b = sd.b + (size_t)ci * 3;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) { dg[d] += S[d]; r[d] += b[d]; }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      rhs[(size_t)d * nV + g] = r[d];
+      invd[(size_t)d * nV + g] = dg[d] != 0.0 ? 1.0 / dg[d] : 1.0;
+    }
+    return 0.0;
+  }
+};
+
+// CG operator L p = A^T M A p + prox p + soft terms; finish() also returns p.(Lp).
+template <class KT>
+struct ApplyOp {
+  RefOp ro; int nV, nH; const int* conn; const KT* K; const double* p; double* out; double prox;
+  SoftDev sd; const double* pa; int nL; const int* done;
+  __device__ void hex(int h, double o[8][3]) const {
+    int vid[8];
+    double q[8][3];
+    gather8(p, nV, conn, nH, h, vid, q);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      double E[3][3], B[3][3], Y[3][3];
+      tet_edges(q, c, E);
+      const KT* kk = K + (size_t)c * 6 * nH + h;
+      const double k00 = kk[0], k11 = kk[(size_t)nH], k22 = kk[2 * (size_t)nH], k01 = kk[3 * (size_t)nH],
+                   k02 = kk[4 * (size_t)nH], k12 = kk[5 * (size_t)nH];
+      if (ro.mode == REF_CUBE) {
+        // invs^2 is folded into K: forces on the tet edges are K E (columns = edges)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double f0 = k00 * E[0][k] + k01 * E[1][k] + k02 * E[2][k];
+          const double f1 = k01 * E[0][k] + k11 * E[1][k] + k12 * E[2][k];
+          const double f2 = k02 * E[0][k] + k12 * E[1][k] + k22 * E[2][k];
+          const int vk = kHexTet[c][k + 1], v0 = kHexTet[c][0];
+          o[vk][0] += f0; o[vk][1] += f1; o[vk][2] += f2;
+          o[v0][0] -= f0; o[v0][1] -= f1; o[v0][2] -= f2;
+        }
+      } else {
+        tet_B(ro, nH, h, c, B);
+        double J[3][3];
+        matmul3(E, B, J);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          Y[0][j] = k00 * J[0][j] + k01 * J[1][j] + k02 * J[2][j];
+          Y[1][j] = k01 * J[0][j] + k11 * J[1][j] + k12 * J[2][j];
+          Y[2][j] = k02 * J[0][j] + k12 * J[1][j] + k22 * J[2][j];
+        }
+        scatter_tet(Y, B, c, o);
+      }
+    }
+  }
+  __device__ double finish(int g, double acc[3]) const {
+    double pv[3] = {p[g], p[(size_t)nV + g], p[2 * (size_t)nV + g]};
+    double r[3] = {acc[0] + prox * pv[0], acc[1] + prox * pv[1], acc[2] + prox * pv[2]};
+    soft_apply(sd, g, pv, nV, pa, r);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[(size_t)d * nV + g] = r[d];
+    return pv[0] * r[0] + pv[1] * r[1] + pv[2] * r[2];
+  }
+};
+
+// Rows of the feature-line unknowns a_L: (L p)_a = lamL (|t|^2 pa - t . p_v).
+__global__ void line_apply_kernel(SoftDev sd, int nV, const double* p, const double* pa, double* outa, int nL,
+                                  double* partial) {
+  __shared__ double red[16];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double part = 0;
+  if (i < nL) {
+    const int v = sd.lv[i];
+    const double* t = sd.lt + (size_t)i * 3;
+    double tp = t[0] * p[v] + t[1] * p[(size_t)nV + v] + t[2] * p[2 * (size_t)nV + v];
+    double tt = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+    double r = sd.lamL * (tt * pa[i] - tp);
+    outa[i] = r;
+    part = pa[i] * r;
+  }
+  double s = block_sum(part, red);
+  if (threadIdx.x == 0 && partial) partial[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// CG vector kernels. The flattened unknown vector has n = 3 nV + nL entries.
+struct CGState {
+  double rhsNorm2, threshold, absNew, alpha, beta, rn2, tol_error;
+  int done, iters, zero_rhs, pad;
+};
+
+// r = rhs - tmp, partial |rhs|^2, |r|^2
+__global__ void cg_init_kernel(int n, const double* rhs, const double* tmp, double* r, double* part_b, double* part_r) {
+  __shared__ double red[16];
+  double sb = 0, sr = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double v = rhs[i] - tmp[i];
+    r[i] = v;
+    sb += rhs[i] * rhs[i];
+    sr += v * v;
+  }
+  double a = block_sum(sb, red);
+  double b = block_sum(sr, red);
+  if (threadIdx.x == 0) { part_b[blockIdx.x] = a; part_r[blockIdx.x] = b; }
+}
+
+__device__ double sum_parts(const double* p, int n, double* red) {
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+  return block_sum(s, red);
+}
+
+// Eigen ConjugateGradient.h:44-62 (zero rhs, early exit) and the first direction.
+__global__ void cg_init_fin_kernel(CGState* st, const double* part_b, int nb, const double* part_r, double tol) {
+  __shared__ double red[16];
+  double b2 = sum_parts(part_b, nb, red);
+  double r2 = sum_parts(part_r, nb, red);
+  if (threadIdx.x == 0) {
+    st->rhsNorm2 = b2;
+    st->iters = 0;
+    st->done = 0;
+    st->zero_rhs = 0;
+    if (b2 == 0) { st->done = 1; st->zero_rhs = 1; st->tol_error = 0; return; }
+    st->threshold = tol * tol * b2;
+    st->rn2 = r2;
+    if (r2 < st->threshold) { st->done = 1; st->tol_error = sqrt(r2 / b2); }
+  }
+}
+
+// p = invd r, partial r.p
+__global__ void cg_first_dir_kernel(int n, const CGState* st, const double* invd, const double* r, double* p,
+                                    double* part) {
+  __shared__ double red[16];
+  double s = 0;
+  if (!st->done)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      double v = invd[i] * r[i];
+      p[i] = v;
+      s += r[i] * v;
+    }
+  double a = block_sum(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+__global__ void cg_absnew_fin_kernel(CGState* st, const double* part, int nb) {
+  __shared__ double red[16];
+  double a = sum_parts(part, nb, red);
+  if (threadIdx.x == 0 && !st->done) st->absNew = a;
+}
+
+// alpha = absNew / p.(Lp) ; the dot product is split over three partial arrays
+__global__ void cg_alpha_kernel(CGState* st, const double* p1, int n1, const double* p2, int n2, const double* p3, int n3) {
+  __shared__ double red[16];
+  double a = sum_parts(p1, n1, red);
+  double b = sum_parts(p2, n2, red);
+  double c = sum_parts(p3, n3, red);
+  if (threadIdx.x == 0 && !st->done) st->alpha = st->absNew / (a + b + c);
+}
+
+// x += alpha p ; r -= alpha tmp ; z = invd r ; partial |r|^2, r.z
+__global__ void cg_update_kernel(int n, const CGState* st, double* x, double* r, double* z, const double* p,
+                                 const double* tmp, const double* invd, double* part_rr, double* part_rz) {
+  __shared__ double red[16];
+  double srr = 0, srz = 0;
+  if (!st->done) {
+    const double alpha = st->alpha;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      x[i] += alpha * p[i];
+      double rv = r[i] - alpha * tmp[i];
+      r[i] = rv;
+      double zv = invd[i] * rv;
+      z[i] = zv;
+      srr += rv * rv;
+      srz += rv * zv;
+    }
+  }
+  double a = block_sum(srr, red);
+  double b = block_sum(srz, red);
+  if (threadIdx.x == 0) { part_rr[blockIdx.x] = a; part_rz[blockIdx.x] = b; }
+}
+
+// convergence test and beta (ConjugateGradient.h:73-85)
+__global__ void cg_beta_kernel(CGState* st, const double* part_rr, const double* part_rz, int nb) {
+  __shared__ double red[16];
+  double rr = sum_parts(part_rr, nb, red);
+  double rz = sum_parts(part_rz, nb, red);
+  if (threadIdx.x == 0 && !st->done) {
+    st->rn2 = rr;
+    if (rr < st->threshold) {
+      st->done = 1;
+    } else {
+      double absOld = st->absNew;
+      st->absNew = rz;
+      st->beta = rz / absOld;
+      st->iters += 1;
+    }
+    st->tol_error = sqrt(rr / st->rhsNorm2);
+  }
+}
+
+__global__ void cg_pdir_kernel(int n, const CGState* st, double* p, const double* z) {
+  if (st->done) return;
+  const double beta = st->beta;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+}
+
+__global__ void cg_zero_if_kernel(int n, const CGState* st, double* x) {
+  if (!st->zero_rhs) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Flip-avoiding step bound (flip_avoiding_line_search.cpp:24-298), FP64.
+__device__ int solve_p3(double x[3], double a, double b, double c) {
+  double a2 = a * a;
+  double q = (a2 - 3 * b) / 9;
+  double r = (a * (2 * a2 - 9 * b) + 27 * c) / 54;
+  double r2 = r * r, q3 = q * q * q;
+  if (r2 < q3) {
+    double t = r / sqrt(q3);
+    if (t < -1) t = -1;
+    if (t > 1) t = 1;
+    t = acos(t);
+    a /= 3;
+    q = -2 * sqrt(q);
+    x[0] = q * cos(t / 3) - a;
+    x[1] = q * cos((t + (2 * M_PI)) / 3) - a;
+    x[2] = q * cos((t - (2 * M_PI)) / 3) - a;
+    return 3;
+  }
+  double A = -pow(fabs(r) + sqrt(r2 - q3), 1. / 3);
+  if (r < 0) A = -A;
+  double B = (A == 0) ? 0 : q / A;
+  a /= 3;
+  x[0] = (A + B) - a;
+  x[1] = -0.5 * (A + B) - a;
+  x[2] = 0.5 * sqrt(3.) * (A - B);
+  if (fabs(x[2]) < 1e-14) { x[2] = x[1]; return 2; }
+  return 1;
+}
+
+__device__ double smallest_pos_quad_zero(double a, double b, double c) {
+  double t1, t2;
+  if (fabs(a) > 1.0e-10) {
+    double delta_in = b * b - 4 * a * c;
+    if (delta_in <= 0) return INFINITY;
+    double delta = sqrt(delta_in);
+    if (b >= 0) {
+      double bd = -b - delta;
+      t1 = 2 * c / bd;
+      t2 = bd / (2 * a);
+    } else {
+      double bd = -b + delta;
+      t1 = bd / (2 * a);
+      t2 = (2 * c) / bd;
+    }
+    if (a < 0) { double t = t1; t1 = t2; t2 = t; }
+    if (t1 > 0) return t2 > 0 ? t2 : t1;
+    return INFINITY;
+  }
+  if (b == 0) return INFINITY;
+  t1 = -c / b;
+  return t1 > 0 ? t1 : INFINITY;
+}
+
+__device__ __forceinline__ double dd3(const double a[3], const double b[3], const double c[3]) {
+  return a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) + a[2] * (b[0] * c[1] - b[1] * c[0]);
+}
+
+// smallest positive root of the corner tet's volume along x + t d; the coefficients are
+// those of det(E + tD) = det E + t(...) + t^2(...) + t^3 det D, negated to match the
+// reference's 4x4 volume determinant (the sign does not move the roots but enters the
+// |a| <= 1e-10 test only through its magnitude).
+__device__ double tet_min_root(const double x[4][3], const double d[4][3]) {
+  double e0[3], e1[3], e2[3], f0[3], f1[3], f2[3];
+  for (int k = 0; k < 3; ++k) {
+    e0[k] = x[1][k] - x[0][k]; e1[k] = x[2][k] - x[0][k]; e2[k] = x[3][k] - x[0][k];
+    f0[k] = d[1][k] - d[0][k]; f1[k] = d[2][k] - d[0][k]; f2[k] = d[3][k] - d[0][k];
+  }
+  double c3 = dd3(f0, f1, f2);
+  double c2 = dd3(e0, f1, f2) + dd3(f0, e1, f2) + dd3(f0, f1, e2);
+  double c1 = dd3(f0, e1, e2) + dd3(e0, f1, e2) + dd3(e0, e1, f2);
+  double c0 = dd3(e0, e1, e2);
+  double a = -c3, b = -c2, c = -c1, dd = -c0;
+  if (fabs(a) <= 1.e-10) return smallest_pos_quad_zero(b, c, dd);
+  b /= a; c /= a; dd /= a;
+  double res[3];
+  int n = solve_p3(res, b, c, dd);
+  if (n == 1) return (res[0] >= 0) ? res[0] : INFINITY;
+  if (n == 2) {
+    double mx = fmax(res[0], res[1]), mn = fmin(res[0], res[1]);
+    if (mn > 0) return mn;
+    if (mx > 0) return mx;
+    return INFINITY;
+  }
+  // sort 3
+  double r0 = res[0], r1 = res[1], r2 = res[2], t;
+  if (r1 < r0) { t = r0; r0 = r1; r1 = t; }
+  if (r2 < r1) { t = r1; r1 = r2; r2 = t; }
+  if (r1 < r0) { t = r0; r0 = r1; r1 = t; }
+  if (r0 > 0) return r0;
+  if (r1 > 0) return r1;
+  if (r2 > 0) return r2;
+  return INFINITY;
+}
+
+// direction d = dst - cur, taken per vertex on the fly
+__global__ void max_step_kernel(const double* cur, const double* dst, int nV, const int* conn, int nH, double* part) {
+  __shared__ double red[16];
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  double m = INFINITY;
+  if (h < nH) {
+    int vid[8];
+    double p[8][3], q[8][3];
+    gather8(cur, nV, conn, nH, h, vid, p);
+    gather8(dst, nV, conn, nH, h, vid, q);
+    for (int k = 0; k < 8; ++k)
+      for (int d = 0; d < 3; ++d) q[k][d] -= p[k][d];
+#pragma unroll 1
+    for (int c = 0; c < 8; ++c) {
+      double x[4][3], dd[4][3];
+      for (int j = 0; j < 4; ++j)
+        for (int d = 0; d < 3; ++d) { x[j][d] = p[kHexTet[c][j]][d]; dd[j][d] = q[kHexTet[c][j]][d]; }
+      m = fmin(m, tet_min_root(x, dd));
+    }
+  }
+  double s = block_min(m, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void min_fin_kernel(const double* part, int n, double* out) {
+  __shared__ double red[16];
+  double m = INFINITY;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmin(m, part[i]);
+  m = block_min(m, red);
+  if (threadIdx.x == 0) *out = m;
+}
+
+// ---------------------------------------------------------------------------------------
+// Energy of X = cur + step (dst - cur) (compute_energy_with_jacobians); per-block partials.
+template <class T>
+__global__ void energy_kernel(RefOp ro, const double* cur, const double* dst, double step, int nV, const int* conn,
+                              int nH, int energy, double M, double exp_f, double* part) {
+  __shared__ double red[16];
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0;
+  if (h < nH) {
+    int vid[8];
+    double p[8][3];
+    gather8(cur, nV, conn, nH, h, vid, p);
+    if (dst) {
+      double q[8][3];
+      gather8(dst, nV, conn, nH, h, vid, q);
+      for (int k = 0; k < 8; ++k)
+        for (int d = 0; d < 3; ++d) p[k][d] = p[k][d] + step * (q[k][d] - p[k][d]);
+    }
+    T pt[8][3];
+    for (int k = 0; k < 8; ++k)
+      for (int d = 0; d < 3; ++d) pt[k][d] = (T)p[k][d];
+#pragma unroll 1
+    for (int c = 0; c < 8; ++c) {
+      T B[3][3], E[3][3], J[3][3];
+      tet_B(ro, nH, h, c, B);
+      tet_edges(pt, c, E);
+      matmul3(E, B, J);
+      e += M * (double)tet_energy(J, energy, (T)exp_f);
+    }
+  }
+  double s = block_sum(e, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// compute_soft_const_energy (slim_m.cpp:735-790) at X = cur + step (dst - cur)
+__global__ void soft_energy_kernel(SoftDev sd, const double* cur, const double* dst, double step, int nV,
+                                   const double* aL, double* part) {
+  __shared__ double red[16];
+  const int tot = sd.nb + sd.nC + sd.nT + sd.nL;
+  double e = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+    int j = i, v;
+    int kind;
+    if (j < sd.nb) { kind = 0; v = sd.b_ids[j]; }
+    else if ((j -= sd.nb) < sd.nC) { kind = 1; v = sd.c_ids[j]; }
+    else if ((j -= sd.nC) < sd.nT) { kind = 2; v = sd.t_ids[j]; }
+    else { j -= sd.nT; kind = 3; v = sd.lv[j]; }
+    double x[3];
+    for (int d = 0; d < 3; ++d) {
+      double a = cur[(size_t)d * nV + v];
+      x[d] = dst ? a + step * (dst[(size_t)d * nV + v] - a) : a;
+    }
+    if (kind == 0) {
+      double q0 = sd.b_pos[j * 3] - x[0], q1 = sd.b_pos[j * 3 + 1] - x[1], q2 = sd.b_pos[j * 3 + 2] - x[2];
+      e += sd.lamP * (q0 * q0 + q1 * q1 + q2 * q2);
+    } else if (kind == 1) {
+      double q0 = sd.c_pos[j * 3] - x[0], q1 = sd.c_pos[j * 3 + 1] - x[1], q2 = sd.c_pos[j * 3 + 2] - x[2];
+      e += sd.lamC * (q0 * q0 + q1 * q1 + q2 * q2);
+    } else if (kind == 2) {
+      double a = sd.t_n[j * 3] * x[0] + sd.t_n[j * 3 + 1] * x[1] + sd.t_n[j * 3 + 2] * x[2] - sd.t_d[j];
+      e += sd.lamT * a * a;
+    } else {
+      double s = 0;
+      for (int d = 0; d < 3; ++d) {
+        double q = x[d] - aL[j] * sd.lt[(size_t)j * 3 + d] - sd.lo[(size_t)j * 3 + d];
+        s += q * q;
+      }
+      e += sd.lamL * s;
+    }
+  }
+  double s = block_sum(e, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// two partial arrays -> (quality, soft) on device
+__global__ void energy_fin_kernel(const double* pq, int nq, const double* ps, int ns, double* out2) {
+  __shared__ double red[16];
+  double a = sum_parts(pq, nq, red);
+  double b = sum_parts(ps, ns, red);
+  if (threadIdx.x == 0) { out2[0] = a; out2[1] = b; }
+}
+
+// cur += step (dst - cur)
+__global__ void step_kernel(int n, double* cur, const double* dst, double step) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double a = cur[i];
+    cur[i] = a + step * (dst[i] - a);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Adjoint: symmetric-Dirichlet energy, dE/dx (all tets + soft terms) and Hessian-vector
+// products; corner Jacobians are recomputed from x, nothing is stored between calls.
+template <class T, class P>
+struct GradOp {
+  RefOp ro; int nV, nH; const int* conn; const P* X; P* G; double M;
+  SoftDev sd; const double* aL; double* epart_unused;
+  __device__ void hex(int h, T o[8][3]) const {
+    int vid[8];
+    double pd[8][3];
+    gather8(X, nV, conn, nH, h, vid, pd);
+    T p[8][3];
+    for (int k = 0; k < 8; ++k)
+      for (int d = 0; d < 3; ++d) p[k][d] = (T)pd[k][d];
+#pragma unroll 1
+    for (int c = 0; c < 8; ++c) {
+      T B[3][3], E[3][3], J[3][3], Pm[3][3];
+      tet_B(ro, nH, h, c, B);
+      tet_edges(p, c, E);
+      matmul3(E, B, J);
+      sd_energy_grad(J, Pm);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Pm[i][j] *= (T)M;
+      scatter_tet(Pm, B, c, o);
+    }
+  }
+  // soft gradient 2 (S x - b - lamL a t) for constrained vertices
+  __device__ T finish(int g, T acc[3]) const {
+    double x[3] = {(double)X[g], (double)X[(size_t)nV + g], (double)X[2 * (size_t)nV + g]};
+    double r[3] = {(double)acc[0], (double)acc[1], (double)acc[2]};
+    const int ci = sd.cidx ? sd.cidx[g] : -1;
+    if (ci >= 0) {
+      const double* S = sd.S + (size_t)ci * 6;
+      const double* b = sd.b + (size_t)ci * 3;
+      double sx[3] = {S[0] * x[0] + S[3] * x[1] + S[4] * x[2] - b[0], S[3] * x[0] + S[1] * x[1] + S[5] * x[2] - b[1],
+                      S[4] * x[0] + S[5] * x[1] + S[2] * x[2] - b[2]};
+      const int li = sd.cline[ci];
+      if (li >= 0)
+        for (int d = 0; d < 3; ++d) sx[d] -= sd.lamL * aL[li] * sd.lt[(size_t)li * 3 + d];
+      for (int d = 0; d < 3; ++d) r[d] += 2.0 * sx[d];
+    }
+    for (int d = 0; d < 3; ++d) G[(size_t)d * nV + g] = (P)r[d];
+    return T(0);
+  }
+};
+
+template <class T, class P>
+struct HvpOp {
+  RefOp ro; int nV, nH; const int* conn; const P* X; const P* Vd; P* Hv; double M; SoftDev sd;
+  __device__ void hex(int h, T o[8][3]) const {
+    int vid[8];
+    double pd[8][3], vd[8][3];
+    gather8(X, nV, conn, nH, h, vid, pd);
+    gather8(Vd, nV, conn, nH, h, vid, vd);
+    T p[8][3], v[8][3];
+    for (int k = 0; k < 8; ++k)
+      for (int d = 0; d < 3; ++d) { p[k][d] = (T)pd[k][d]; v[k][d] = (T)vd[k][d]; }
+#pragma unroll 1
+    for (int c = 0; c < 8; ++c) {
+      T B[3][3], E[3][3], J[3][3], Ed[3][3], dJ[3][3], Hd[3][3];
+      tet_B(ro, nH, h, c, B);
+      tet_edges(p, c, E);
+      matmul3(E, B, J);
+      tet_edges(v, c, Ed);
+      matmul3(Ed, B, dJ);
+      sd_hess_dir(J, dJ, Hd);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Hd[i][j] *= (T)M;
+      scatter_tet(Hd, B, c, o);
+    }
+  }
+  __device__ T finish(int g, T acc[3]) const {
+    double vv[3] = {(double)Vd[g], (double)Vd[(size_t)nV + g], (double)Vd[2 * (size_t)nV + g]};
+    double r[3] = {(double)acc[0], (double)acc[1], (double)acc[2]};
+    const int ci = sd.cidx ? sd.cidx[g] : -1;
+    if (ci >= 0) {
+      const double* S = sd.S + (size_t)ci * 6;
+      r[0] += 2.0 * (S[0] * vv[0] + S[3] * vv[1] + S[4] * vv[2]);
+      r[1] += 2.0 * (S[3] * vv[0] + S[1] * vv[1] + S[5] * vv[2]);
+      r[2] += 2.0 * (S[4] * vv[0] + S[5] * vv[1] + S[2] * vv[2]);
+    }
+    for (int d = 0; d < 3; ++d) Hv[(size_t)d * nV + g] = (P)r[d];
+    return T(0);
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// scaled_jacobian (global_functions.cpp:2309-2358, a_jacobian :2422-2442): per-hex min of
+// the 8 corner scaled Jacobians; partial min and sum.
+__global__ void scaled_jacobian_kernel(const double* X, int nV, const int* conn, int nH, double* per_hex,
+                                       double* part_min, double* part_sum) {
+  __shared__ double red[16];
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  double hmin = 1, hs = 0;
+  if (h < nH) {
+    int vid[8];
+    double p[8][3];
+    gather8(X, nV, conn, nH, h, vid, p);
+    for (int c = 0; c < 8; ++c) {
+      const double* a = p[kHexTet[c][0]];
+      double M[3][3];
+      for (int k = 0; k < 3; ++k)
+        for (int d = 0; d < 3; ++d) M[d][k] = (p[kHexTet[c][k + 1]][d] - a[d]) * .5;
+      double n1 = sqrt(M[0][0] * M[0][0] + M[1][0] * M[1][0] + M[2][0] * M[2][0]);
+      double n2 = sqrt(M[0][1] * M[0][1] + M[1][1] * M[1][1] + M[2][1] * M[2][1]);
+      double n3 = sqrt(M[0][2] * M[0][2] + M[1][2] * M[1][2] + M[2][2] * M[2][2]);
+      double sj = det3(M);
+      const double Precision = 1.e-7;
+      if (!(fabs(n1) < Precision || fabs(n2) < Precision || fabs(n3) < Precision)) sj /= n1 * n2 * n3;
+      if (hmin > sj) hmin = sj;
+    }
+    if (per_hex) per_hex[h] = hmin;
+    hs = hmin;
+  }
+  double m = block_min(h < nH ? hmin : INFINITY, red);
+  double s = block_sum(hs, red);
+  if (threadIdx.x == 0) { part_min[blockIdx.x] = m; part_sum[blockIdx.x] = s; }
+}
+
+// ---------------------------------------------------------------------------------------
+// host launch wrappers (kernels.h)
+
+void launch_ref_cube(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube, double* invs,
+                     int* n_general, hipStream_t st) {
+  if (nH == 0) return;
+  hipLaunchKernelGGL(ref_cube_kernel, dim3((nH + 255) / 256), dim3(256), 0, st, X, nV, conn, nH, hflag, cube, invs,
+                     n_general);
+}
+
+void launch_ref_general(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube, double* B,
+                        hipStream_t st) {
+  if (nH == 0) return;
+  hipLaunchKernelGGL(ref_general_kernel, dim3((nH + 255) / 256), dim3(256), 0, st, X, nV, conn, nH, hflag, cube, B);
+}
+
+void launch_local_step(const TileMesh& tm, const RefOp& ro, const double* X, double* K, int energy, double M,
+                       double exp_f, double prox, const SoftDev& sd, double* rhs, double* invd, double* halo,
+                       hipStream_t st) {
+  LocalStepOp op{ro, tm.nV, tm.nH, tm.conn, X, K, energy, M, exp_f, prox, sd, rhs, invd};
+  launch_scatter<double, 6>(tm, op, halo, nullptr, nullptr, st);
+}
+
+void launch_apply(const TileMesh& tm, const RefOp& ro, const double* K, const double* p, double* out, double prox,
+                  const SoftDev& sd, double* halo, double* part_tile, double* part_fix, double* part_line,
+                  hipStream_t st) {
+  const double* pa = p + 3 * (size_t)tm.nV;
+  ApplyOp<double> op{ro, tm.nV, tm.nH, tm.conn, K, p, out, prox, sd, pa, sd.nL, nullptr};
+  launch_scatter<double, 3>(tm, op, halo, part_tile, part_fix, st);
+  if (sd.nL > 0)
+    hipLaunchKernelGGL(line_apply_kernel, dim3((sd.nL + 255) / 256), dim3(256), 0, st, sd, tm.nV, p, pa,
+                       out + 3 * (size_t)tm.nV, sd.nL, part_line);
+}
+
+void launch_cg_init(int n, const double* rhs, const double* tmp, double* r, double* pb, double* pr, int nb,
+                    void* state, const double* invd, double* p, double* pabs, double tol, hipStream_t st) {
+  CGState* S = (CGState*)state;
+  hipLaunchKernelGGL(cg_init_kernel, dim3(nb), dim3(256), 0, st, n, rhs, tmp, r, pb, pr);
+  hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)pb, nb, (const double*)pr, tol);
+  hipLaunchKernelGGL(cg_first_dir_kernel, dim3(nb), dim3(256), 0, st, n, (const CGState*)S, invd, (const double*)r, p,
+                     pabs);
+  hipLaunchKernelGGL(cg_absnew_fin_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)pabs, nb);
+}
+
+void launch_cg_alpha(void* state, const double* p1, int n1, const double* p2, int n2, const double* p3, int n3,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(1024), 0, st, (CGState*)state, p1, n1, p2, n2, p3, n3);
+}
+
+void launch_cg_update(int n, void* state, double* x, double* r, double* z, const double* p, const double* tmp,
+                      const double* invd, double* prr, double* prz, int nb, double* p_out, hipStream_t st) {
+  CGState* S = (CGState*)state;
+  hipLaunchKernelGGL(cg_update_kernel, dim3(nb), dim3(256), 0, st, n, (const CGState*)S, x, r, z, p, tmp, invd, prr, prz);
+  hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)prr, (const double*)prz, nb);
+  hipLaunchKernelGGL(cg_pdir_kernel, dim3(nb), dim3(256), 0, st, n, (const CGState*)S, p_out, (const double*)z);
+}
+
+void launch_cg_zero_if(int n, void* state, double* x, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(cg_zero_if_kernel, dim3(nb), dim3(256), 0, st, n, (const CGState*)state, x);
+}
+
+size_t cg_state_bytes() { return sizeof(CGState); }
+
+void read_cg_state(const void* host_state, int* iters, double* tol_error, int* done) {
+  const CGState* S = (const CGState*)host_state;
+  *iters = S->iters;
+  *tol_error = S->tol_error;
+  *done = S->done;
+}
+
+void launch_max_step(const double* cur, const double* dst, int nV, const int* conn, int nH, double* part,
+                     double* out, hipStream_t st) {
+  const int nb = (nH + 255) / 256;
+  if (nb == 0) return;
+  hipLaunchKernelGGL(max_step_kernel, dim3(nb), dim3(256), 0, st, cur, dst, nV, conn, nH, part);
+  hipLaunchKernelGGL(min_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part, nb, out);
+}
+
+void launch_energy(const RefOp& ro, const double* cur, const double* dst, double step, int nV, const int* conn, int nH,
+                   int energy, double M, double exp_f, const SoftDev& sd, const double* aL, double* part_q,
+                   double* part_s, int ns_blocks, double* out2, hipStream_t st) {
+  const int nb = (nH + 255) / 256;
+  if (nb > 0)
+    hipLaunchKernelGGL(energy_kernel<double>, dim3(nb), dim3(256), 0, st, ro, cur, dst, step, nV, conn, nH, energy, M,
+                       exp_f, part_q);
+  hipLaunchKernelGGL(soft_energy_kernel, dim3(ns_blocks), dim3(256), 0, st, sd, cur, dst, step, nV, aL, part_s);
+  hipLaunchKernelGGL(energy_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part_q, nb, (const double*)part_s,
+                     ns_blocks, out2);
+}
+
+void launch_step(int n, double* cur, const double* dst, double step, hipStream_t st) {
+  hipLaunchKernelGGL(step_kernel, dim3((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096), dim3(256), 0, st, n, cur, dst, step);
+}
+
+template <class T, class P>
+void launch_grad_t(const TileMesh& tm, const RefOp& ro, const P* X, P* G, double M, const SoftDev& sd, const double* aL,
+                   T* halo, hipStream_t st) {
+  GradOp<T, P> op{ro, tm.nV, tm.nH, tm.conn, X, G, M, sd, aL, nullptr};
+  launch_scatter<T, 3>(tm, op, halo, (T*)nullptr, (T*)nullptr, st);
+}
+
+template <class T, class P>
+void launch_hvp_t(const TileMesh& tm, const RefOp& ro, const P* X, const P* Vd, P* Hv, double M, const SoftDev& sd,
+                  T* halo, hipStream_t st) {
+  HvpOp<T, P> op{ro, tm.nV, tm.nH, tm.conn, X, Vd, Hv, M, sd};
+  launch_scatter<T, 3>(tm, op, halo, (T*)nullptr, (T*)nullptr, st);
+}
+
+void launch_grad(int prec, const TileMesh& tm, const RefOp& ro, const void* X, void* G, double M, const SoftDev& sd,
+                 const double* aL, void* halo, hipStream_t st) {
+  if (prec == 64)
+    launch_grad_t<double, double>(tm, ro, (const double*)X, (double*)G, M, sd, aL, (double*)halo, st);
+  else
+    launch_grad_t<float, float>(tm, ro, (const float*)X, (float*)G, M, sd, aL, (float*)halo, st);
+}
+
+void launch_hvp(int prec, const TileMesh& tm, const RefOp& ro, const void* X, const void* Vd, void* Hv, double M,
+                const SoftDev& sd, void* halo, hipStream_t st) {
+  if (prec == 64)
+    launch_hvp_t<double, double>(tm, ro, (const double*)X, (const double*)Vd, (double*)Hv, M, sd, (double*)halo, st);
+  else
+    launch_hvp_t<float, float>(tm, ro, (const float*)X, (const float*)Vd, (float*)Hv, M, sd, (float*)halo, st);
+}
+
+void launch_scaled_jacobian(const double* X, int nV, const int* conn, int nH, double* per_hex, double* pmin,
+                            double* psum, double* out2, hipStream_t st) {
+  const int nb = (nH + 255) / 256;
+  if (nb == 0) return;
+  hipLaunchKernelGGL(scaled_jacobian_kernel, dim3(nb), dim3(256), 0, st, X, nV, conn, nH, per_hex, pmin, psum);
+  hipLaunchKernelGGL(min_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)pmin, nb, out2);
+  hipLaunchKernelGGL(energy_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)psum, nb, (const double*)psum, 0,
+                     out2 + 1);
+}
+
+}  // namespace hexopt

@@ -1,0 +1,120 @@
+/* hexopt.h -- C ABI of the MI355X hex-mesh optimisation path (libhexopt.so).
+ *
+ * Drop-in boundary for the reference's optimiser inner loop. Plain pointers and sizes only;
+ * every array is host memory in the caller's numbering unless a function says otherwise.
+ * Vertex arrays are nV x 3 row-major doubles (one row per vertex, like Tetralize_Set::V);
+ * hexes are nH x 8 int32 in the reference's Hybrid::vs corner order; the 8 corner tets of
+ * hex h are tets 8h..8h+7 in global_types.h:163-173 order (exactly how deformation(),
+ * smoothing() and feature_alignment() build Tetralize_Set::T).
+ *
+ * Errors: every int-returning function returns HEXOPT_OK (0) or a negative code and leaves a
+ * message readable with hexopt_last_error(). Nothing throws across the ABI. A context owns
+ * one HIP stream on one device; calls on one context must not overlap (not thread-safe per
+ * context, like the reference's single-threaded optimiser).
+ */
+#ifndef HEXOPT_H
+#define HEXOPT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HEXOPT_OK 0
+#define HEXOPT_EINVAL (-1)
+#define HEXOPT_EHIP (-2)
+#define HEXOPT_EUNSUPPORTED (-3)
+
+typedef struct hexopt_ctx hexopt_ctx;
+
+/* SLIM_ENERGY values (global_types.h:56-64) */
+enum { HEXOPT_ARAP = 0, HEXOPT_LOG_ARAP = 1, HEXOPT_SYMMETRIC_DIRICHLET = 2, HEXOPT_CONFORMAL = 3,
+       HEXOPT_EXP_CONFORMAL = 4, HEXOPT_EXP_SYMMETRIC_DIRICHLET = 5 };
+
+/* Which position buffer: REST = Tetralize_Set::V (defines the reference operator),
+ * CURRENT = Tetralize_Set::UV (the iterate, SLIMData::V_o). */
+enum { HEXOPT_REST = 0, HEXOPT_CURRENT = 1 };
+
+typedef struct {
+  int energy;          /* Tetralize_Set::energy_type */
+  double weight_opt;   /* optimization::weight_opt: M = weight_opt, proximal = 1e-5 * weight_opt
+                          (slim_m.cpp:1118-1122) */
+  double exp_factor;   /* exponential energies; slim_precompute forces 1.0 (slim_m.cpp:1125) */
+  int projection;      /* Tetralize_Set::projection: adds lamda_projection * |x_b - bc|^2 */
+  double lamda_projection;
+} hexopt_params;
+
+/* Soft constraints of Tetralize_Set / Feature_Constraints (global_types.h:583-650).
+ * Vertex ids in the caller's numbering; positions/normals/axes are n x 3 row-major. */
+typedef struct {
+  int nb; const int32_t* b; const double* bc;                       /* ts.b, ts.bc */
+  int nC; const int32_t* ids_C; const double* C; double lamda_C;     /* corners */
+  int nT; const int32_t* ids_T; const double* normal_T; const double* dis_T; double lamda_T; /* planes */
+  int nL; const int32_t* ids_L; const double* Axa_L; const double* origin_L; double lamda_L; /* lines */
+} hexopt_constraints;
+
+/* Per-iteration record of one SLIM iteration (slim_solve2 fixed branch, slim_m.cpp:1269-1288) */
+typedef struct {
+  double energy;        /* SLIMData::energy (normalised by mesh_area) */
+  double e_quality;     /* SLIMData::engery_quality (unnormalised) */
+  double e_soft;        /* SLIMData::engery_soft */
+  double step_bound;    /* min(1, 0.8 t*) of flip_avoiding_line_search */
+  double step;          /* accepted step of igl::line_search */
+  int cg_iters;         /* iterations of the Eigen-style CG */
+  double cg_error;      /* its relative residual estimate */
+} hexopt_iter_log;
+
+/* Creates a context on HIP device `device` for a hex mesh. V gives the positions used to
+ * order vertices and hexes along a Morton curve (any representative positions will do).
+ * Replaces the set-up part of slim_precompute (slim_m.cpp:1095-1132). */
+hexopt_ctx* hexopt_create(int nV, const double* V, int nH, const int32_t* hexes, int device);
+const char* hexopt_create_error(void);
+void hexopt_destroy(hexopt_ctx* ctx);
+const char* hexopt_last_error(hexopt_ctx* ctx);
+
+int hexopt_set_positions(hexopt_ctx* ctx, int which, const double* V);
+int hexopt_get_positions(hexopt_ctx* ctx, int which, double* V);
+/* Device-resident positions in the library's own layout (SoA x[d*nV+v], Morton numbering);
+ * for callers that keep the iterate in HBM. */
+double* hexopt_device_positions(hexopt_ctx* ctx, int which);
+
+/* compute_referenceMesh (global_functions.cpp:3735-3806) on the REST positions followed by
+ * igl::grad_ref(..., uniform=true) (grad.cpp:372-478). hflag: nullable nH bytes
+ * (compute_referenceMesh's H_flag); cube: its `cube` argument. */
+int hexopt_reference(hexopt_ctx* ctx, const uint8_t* hflag, int cube);
+
+/* Soft constraints used by the next hexopt_slim_solve / energy / gradient calls. */
+int hexopt_set_constraints(hexopt_ctx* ctx, const hexopt_constraints* cs);
+
+/* optimization::slim_m_opt(ts, iters, -1) (optimization.cpp:883-900) minus the set-up:
+ * slim_precompute's energy followed by `iters` SLIM iterations starting from CURRENT; the
+ * result replaces CURRENT. log: nullable, `iters` entries. E/Eq/Es: nullable outputs
+ * (optimization::energy, engery_quality, engery_soft). aL: nullable, nL entries (a_L). */
+int hexopt_slim_solve(hexopt_ctx* ctx, const hexopt_params* prm, int iters, hexopt_iter_log* log, double* E,
+                      double* Eq, double* Es, double* aL);
+
+/* Unnormalised SLIM energy of CURRENT: quality part and soft part (compute_energy_total,
+ * slim_m.cpp:716-733, with a_L = aL or 0 when aL is NULL). */
+int hexopt_energy(hexopt_ctx* ctx, const hexopt_params* prm, const double* aL, double* Eq, double* Es);
+
+/* Adjoint of the symmetric-Dirichlet SLIM energy (no counterpart in the reference, which
+ * has no gradient): G = dE/dx at X, Hv = d^2E/dx^2 v. precision 64: X, v, G, Hv are host
+ * double nV x 3; precision 32: host float nV x 3 with FP32 arithmetic. */
+int hexopt_energy_grad(hexopt_ctx* ctx, const hexopt_params* prm, int precision, const void* X, const double* aL,
+                       void* G);
+int hexopt_hvp(hexopt_ctx* ctx, const hexopt_params* prm, int precision, const void* X, const void* v, void* Hv);
+
+/* scaled_jacobian (global_functions.cpp:2309-2358) of the `which` positions: min and mean
+ * of the per-hex minimum corner scaled Jacobian; per_hex nullable (caller order). */
+int hexopt_scaled_jacobian(hexopt_ctx* ctx, int which, double* min_j, double* avg_j, double* per_hex);
+
+/* Blocks until the context's stream is idle. */
+int hexopt_synchronize(hexopt_ctx* ctx);
+
+/* Layout facts for tests/benchmarks: [nTiles, nShared, nSlots, max_colours, ref_mode]. */
+int hexopt_layout_info(hexopt_ctx* ctx, int32_t* out5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
