@@ -483,6 +483,49 @@ int hexopt_slim_solve(hexopt_ctx* c, const hexopt_params* prm, int iters, hexopt
   return HEXOPT_OK;
 }
 
+int hexopt_debug_system(hexopt_ctx* c, const hexopt_params* prm, const double* p_in, double* rhs_out,
+                        double* invd_out, double* Lp_out) {
+  if (!c || !prm || !p_in) return HEXOPT_EINVAL;
+  if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");
+  int rc = ensure_state(c);
+  if (rc) return rc;
+  const int nV = c->L.nV, nL = c->nL;
+  const size_t n = (size_t)3 * nV + nL;
+  const double M = prm->weight_opt, prox = 0.00001 * prm->weight_opt;
+  hipStream_t st = c->stream;
+  SoftDev sd = soft_with(c, prm);
+  launch_local_step(c->tm, refop(c), c->d_cur.as<double>(), c->d_K.as<double>(), prm->energy, M, prm->exp_factor, prox,
+                    sd, c->d_rhs.as<double>(), c->d_invd.as<double>(), c->d_halo.as<double>(), st);
+  if (nL > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->d_rhs.as<double>() + 3 * (size_t)nV, c->d_rhs_a.p, nL * sizeof(double),
+                             hipMemcpyDeviceToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->d_invd.as<double>() + 3 * (size_t)nV, c->d_invd_a.p, nL * sizeof(double),
+                             hipMemcpyDeviceToDevice, st));
+  }
+  // p_in: caller numbering, flattened [x(0..nV) y z a_L]
+  std::vector<double> h(n), o(n);
+  for (int i = 0; i < nV; ++i)
+    for (int d = 0; d < 3; ++d) h[(size_t)d * nV + i] = p_in[(size_t)d * nV + c->L.vperm[i]];
+  for (int i = 0; i < nL; ++i) h[3 * (size_t)nV + i] = p_in[3 * (size_t)nV + i];
+  HIPCHK(c, hipMemcpyAsync(c->d_p.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+  launch_apply(c->tm, refop(c), c->d_K.as<double>(), c->d_p.as<double>(), c->d_tmp.as<double>(), prox, sd,
+               c->d_halo.as<double>(), c->d_part[2].as<double>(), c->d_part[3].as<double>(), c->d_part[4].as<double>(),
+               st);
+  auto back = [&](DevBuf& b, double* out) -> int {
+    if (!out) return HEXOPT_OK;
+    HIPCHK(c, hipMemcpyAsync(o.data(), b.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    for (int i = 0; i < nV; ++i)
+      for (int d = 0; d < 3; ++d) out[(size_t)d * nV + c->L.vperm[i]] = o[(size_t)d * nV + i];
+    for (int i = 0; i < nL; ++i) out[3 * (size_t)nV + i] = o[3 * (size_t)nV + i];
+    return HEXOPT_OK;
+  };
+  HIPCHK(c, hipGetLastError());
+  if ((rc = back(c->d_rhs, rhs_out))) return rc;
+  if ((rc = back(c->d_invd, invd_out))) return rc;
+  return back(c->d_tmp, Lp_out);
+}
+
 int hexopt_energy(hexopt_ctx* c, const hexopt_params* prm, const double* aL, double* Eq, double* Es) {
   if (!c || !prm) return HEXOPT_EINVAL;
   if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");

@@ -16,7 +16,7 @@ REST, CURRENT = 0, 1
 EXPORTS = ["hexopt_create", "hexopt_create_error", "hexopt_destroy", "hexopt_last_error", "hexopt_set_positions",
            "hexopt_get_positions", "hexopt_device_positions", "hexopt_reference", "hexopt_set_constraints",
            "hexopt_slim_solve", "hexopt_energy", "hexopt_energy_grad", "hexopt_hvp", "hexopt_scaled_jacobian",
-           "hexopt_synchronize", "hexopt_layout_info"]
+           "hexopt_synchronize", "hexopt_layout_info", "hexopt_debug_system"]
 
 _LIB = None
 
@@ -56,7 +56,7 @@ def lib():
         L.hexopt_device_positions.argtypes = [C.c_void_p, C.c_int]
         for name in ["hexopt_set_positions", "hexopt_get_positions", "hexopt_reference", "hexopt_set_constraints",
                      "hexopt_slim_solve", "hexopt_energy", "hexopt_energy_grad", "hexopt_hvp",
-                     "hexopt_scaled_jacobian", "hexopt_synchronize", "hexopt_layout_info"]:
+                     "hexopt_scaled_jacobian", "hexopt_synchronize", "hexopt_layout_info", "hexopt_debug_system"]:
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -183,6 +183,16 @@ class HexOpt:
         self._chk(lib().hexopt_scaled_jacobian(C.c_void_p(self.h), which, C.byref(mn), C.byref(avg), _ptr(ph)),
                   "scaled_jacobian")
         return (mn.value, avg.value, ph) if per_hex else (mn.value, avg.value)
+
+    def debug_system(self, p, energy=SYMMETRIC_DIRICHLET, weight_opt=1.0):
+        """(rhs, inverse Jacobi diagonal, L p) of the next SLIM iteration at CURRENT, flattened
+        [x y z a_L] in caller numbering."""
+        prm = self.params(energy, weight_opt)
+        p = _d(p)
+        rhs, invd, Lp = np.zeros_like(p), np.zeros_like(p), np.zeros_like(p)
+        self._chk(lib().hexopt_debug_system(C.c_void_p(self.h), C.byref(prm), _ptr(p), _ptr(rhs), _ptr(invd),
+                                            _ptr(Lp)), "debug_system")
+        return rhs, invd, Lp
 
     def synchronize(self):
         self._chk(lib().hexopt_synchronize(C.c_void_p(self.h)), "synchronize")

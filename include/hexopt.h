@@ -108,6 +108,13 @@ int hexopt_hvp(hexopt_ctx* ctx, const hexopt_params* prm, int precision, const v
  * of the per-hex minimum corner scaled Jacobian; per_hex nullable (caller order). */
 int hexopt_scaled_jacobian(hexopt_ctx* ctx, int which, double* min_j, double* avg_j, double* per_hex);
 
+/* Diagnostic view of the global system of the next SLIM iteration at CURRENT (no reference
+ * counterpart; used by the parity tests): runs the local step, then returns rhs, the inverse
+ * Jacobi diagonal and L p_in. Vectors are flattened like the reference's unknowns
+ * [x(0..nV) y(0..nV) z(0..nV) a_L(0..nL)] in the caller's numbering; outputs nullable. */
+int hexopt_debug_system(hexopt_ctx* ctx, const hexopt_params* prm, const double* p_in, double* rhs, double* inv_diag,
+                        double* Lp);
+
 /* Blocks until the context's stream is idle. */
 int hexopt_synchronize(hexopt_ctx* ctx);
 

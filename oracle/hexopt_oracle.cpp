@@ -826,6 +826,54 @@ int orc_slim_m_opt(int nV, const double* V, int nH, const int* H, const double* 
   return 0;
 }
 
+// The global system of the first SLIM iteration at UV (build_linear_system,
+// slim_m.cpp:568-714) applied column by column: L (n x n dense, small meshes only) and rhs.
+int orc_linear_system(int nV, int nH, const int* H, const double* G, const double* UV, const orc_slim_params* prm,
+                      const orc_slim_constraints* cs, double* Ldense, double* rhs_out, const double* p_in,
+                      double* Lp_out, double* diag_out) {
+  Slim s;
+  s.nV = nV; s.nT = nH * 8; s.energy = prm->energy;
+  s.T.resize((size_t)s.nT * 4);
+  for (int h = 0; h < nH; ++h)
+    for (int c = 0; c < 8; ++c)
+      for (int j = 0; j < 4; ++j) s.T[((size_t)h * 8 + c) * 4 + j] = H[h * 8 + kHexTet[c][j]];
+  s.G.assign(G, G + (size_t)s.nT * 12);
+  s.Vo.assign(UV, UV + (size_t)nV * 3);
+  s.M = prm->weight_opt; s.prox = 0.00001 * prm->weight_opt;
+  s.projection = prm->projection != 0; s.glue = prm->glue != 0;
+  s.lam_proj = prm->lam_proj; s.lam_glue = prm->lam_glue; s.lam_C = prm->lam_C; s.lam_T = prm->lam_T; s.lam_L = prm->lam_L;
+  s.b.assign(cs->b, cs->b + cs->nb); s.bc.assign(cs->bc, cs->bc + (size_t)cs->nb * 3);
+  s.ranges.assign(cs->ranges, cs->ranges + cs->nranges);
+  s.idsC.assign(cs->idsC, cs->idsC + cs->nC); s.C.assign(cs->C, cs->C + (size_t)cs->nC * 3);
+  s.idsT.assign(cs->idsT, cs->idsT + cs->nT); s.nT_.assign(cs->normalT, cs->normalT + (size_t)cs->nT * 3);
+  s.dT.assign(cs->disT, cs->disT + cs->nT);
+  s.idsL.assign(cs->idsL, cs->idsL + cs->nL); s.axa.assign(cs->axaL, cs->axaL + (size_t)cs->nL * 3);
+  s.org.assign(cs->originL, cs->originL + (size_t)cs->nL * 3);
+  s.aL.assign(cs->nL, 0.0);
+  s.local_step(UV);
+  const int n = s.nvar();
+  if (p_in) {
+    std::vector<double> pv(p_in, p_in + n), out;
+    s.apply(pv, out);
+    std::memcpy(Lp_out, out.data(), sizeof(double) * n);
+    std::vector<double> dg;
+    s.diagonal(dg);
+    for (int i = 0; i < n; ++i) diag_out[i] = dg[i];
+  }
+  std::vector<double> e(n, 0.0), col;
+  if (Ldense)
+    for (int j = 0; j < n; ++j) {
+      e[j] = 1.0;
+      s.apply(e, col);
+      for (int i = 0; i < n; ++i) Ldense[(size_t)i * n + j] = col[i];
+      e[j] = 0.0;
+    }
+  std::vector<double> r;
+  s.rhs(r);
+  std::memcpy(rhs_out, r.data(), sizeof(double) * n);
+  return n;
+}
+
 // Per-tet SLIM energy density, local-step W and R at positions X (for kernel-level tests).
 void orc_local_step(int nV, int nH, const int* H, const double* G, const double* X, int energy,
                     double* W, double* R, double* E) {
@@ -841,6 +889,68 @@ void orc_local_step(int nV, int nH, const int* H, const double* G, const double*
   if (R) std::memcpy(R, s.R.data(), sizeof(double) * s.R.size());
   if (E)
     for (int t = 0; t < s.nT; ++t) { M3 J; s.jacobian(X, t, J); E[t] = s.tet_energy(J); }
+}
+
+// Gradient of the symmetric-Dirichlet SLIM energy E = M sum_t (sum_i s_i^2 + s_i^-2) + soft
+// with respect to the positions, written through the SVD: dE/dJ = U diag(2 s - 2 s^-3) V^T
+// (an expression independent of the adjugate form the kernels use). soft: tangent-plane,
+// corner and line terms with a_L = aL. X, G: nV*3 row-major.
+void orc_sd_grad(int nV, int nH, const int* H, const double* Gop, const double* X, double M,
+                 const orc_slim_constraints* cs, double lamC, double lamT, double lamL, const double* aL,
+                 double* Gout, double* Eout) {
+  std::memset(Gout, 0, sizeof(double) * (size_t)nV * 3);
+  double e = 0;
+  for (int h = 0; h < nH; ++h)
+    for (int c = 0; c < 8; ++c) {
+      const double* g = &Gop[((size_t)h * 8 + c) * 12];
+      int v[4];
+      for (int j = 0; j < 4; ++j) v[j] = H[h * 8 + kHexTet[c][j]];
+      M3 J;
+      for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc) {
+          double sum = 0;
+          for (int j = 0; j < 4; ++j) sum += g[cc * 4 + j] * X[(size_t)v[j] * 3 + r];
+          J[r][cc] = sum;
+        }
+      M3 U, V;
+      double sv[3];
+      svd3(J, U, sv, V);
+      // signed singular values keep J = U diag(s) V^T exact for inverted tets as well
+      double dv[3];
+      for (int k = 0; k < 3; ++k) {
+        dv[k] = 2 * sv[k] - 2 / (sv[k] * sv[k] * sv[k]);
+        e += M * (sv[k] * sv[k] + 1 / (sv[k] * sv[k]));
+      }
+      M3 P;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P[i][j] = M * (U[i][0] * dv[0] * V[j][0] + U[i][1] * dv[1] * V[j][1] + U[i][2] * dv[2] * V[j][2]);
+      for (int j = 0; j < 4; ++j)
+        for (int r = 0; r < 3; ++r) {
+          double sum = 0;
+          for (int cc = 0; cc < 3; ++cc) sum += P[r][cc] * g[cc * 4 + j];
+          Gout[(size_t)v[j] * 3 + r] += sum;
+        }
+    }
+  for (int i = 0; i < cs->nC; ++i)
+    for (int d = 0; d < 3; ++d) {
+      double q = X[(size_t)cs->idsC[i] * 3 + d] - cs->C[i * 3 + d];
+      Gout[(size_t)cs->idsC[i] * 3 + d] += 2 * lamC * q;
+      e += lamC * q * q;
+    }
+  for (int i = 0; i < cs->nT; ++i) {
+    const double* n = &cs->normalT[i * 3];
+    const double* x = &X[(size_t)cs->idsT[i] * 3];
+    double a = n[0] * x[0] + n[1] * x[1] + n[2] * x[2] - cs->disT[i];
+    for (int d = 0; d < 3; ++d) Gout[(size_t)cs->idsT[i] * 3 + d] += 2 * lamT * a * n[d];
+    e += lamT * a * a;
+  }
+  for (int i = 0; i < cs->nL; ++i)
+    for (int d = 0; d < 3; ++d) {
+      double q = X[(size_t)cs->idsL[i] * 3 + d] - aL[i] * cs->axaL[i * 3 + d] - cs->originL[i * 3 + d];
+      Gout[(size_t)cs->idsL[i] * 3 + d] += 2 * lamL * q;
+      e += lamL * q * q;
+    }
+  if (Eout) *Eout = e;
 }
 
 // 3x3 SVD (for closed-form tests): A row-major; U, V row-major; s descending.
@@ -933,15 +1043,18 @@ struct Tree {
     double med;
     if (A.size() % 2 == 1) med = A[n];
     else {
-      int an = A[n];
+      // both elements are read after the second selection, as AABB.cpp:161-170 does
       std::nth_element(A.begin(), A.begin() + n - 1, A.end());
-      med = 0.5 * (an + A[n - 1]);
+      med = 0.5 * (A[n] + A[n - 1]);
     }
     std::vector<int> L, R;
     for (size_t i = 0; i < I.size(); ++i) (S[i] <= med ? L : R).push_back(I[i]);
+    // the reference sizes its halves as (n+1)/2 and n/2; ranks are unique, so a split of
+    // any other shape would be a mismatch of this restatement
+    if (L.size() != (I.size() + 1) / 2) return -1;
     nodes[id] = nd;
-    if (!L.empty()) { int l = build(L); nodes[id].left = l; }
-    if (!R.empty()) { int r = build(R); nodes[id].right = r; }
+    if (!L.empty()) { int l = build(L); if (l < 0) return -1; nodes[id].left = l; }
+    if (!R.empty()) { int r = build(R); if (r < 0) return -1; nodes[id].right = r; }
     return id;
   }
   bool contains(const Node& n, const double* p) const {
@@ -1059,7 +1172,10 @@ void orc_signed_distance_pseudonormal(int nq, const double* P, int nv, const dou
   }
   std::vector<int> all(nf);
   std::iota(all.begin(), all.end(), 0);
-  if (nf > 0) tr.build(all);
+  if (nf > 0 && tr.build(all) < 0) {
+    for (int q = 0; q < nq; ++q) if (I) I[q] = -2;
+    return;
+  }
   for (int q = 0; q < nq; ++q) {
     const double* p = &P[(size_t)q * 3];
     int f = -1;

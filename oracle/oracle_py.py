@@ -153,3 +153,55 @@ def voxel_parity(V, F, nx, ny, nz, origin, spacing):
     lib().orc_voxel_parity(V.shape[0], _p(V, C.c_double), F.shape[0], _p(F, C.c_int), nx, ny, nz, _p(o, C.c_double),
                            C.c_double(spacing), _p(out, C.c_uint8))
     return out.reshape(nz, ny, nx)
+
+
+def linear_system(V, H, G, UV, energy=2, weight_opt=1.0, lam_C=0.0, lam_T=0.0, lam_L=0.0, idsC=None, Cpos=None,
+                  idsT=None, normalT=None, disT=None, idsL=None, axaL=None, originL=None, dense=True, p=None):
+    """Global system of the first SLIM iteration at UV: (L dense or None, rhs), plus (L p, diag)
+    when p is given."""
+    H, G, UV = _i(H), _d(G), _d(UV)
+    e3 = np.zeros((0, 3)); e1 = np.zeros(0); ei = np.zeros(0, dtype=np.int32)
+    idsC = _i(ei if idsC is None else idsC); Cpos = _d(e3 if Cpos is None else Cpos)
+    idsT = _i(ei if idsT is None else idsT); normalT = _d(e3 if normalT is None else normalT)
+    disT = _d(e1 if disT is None else disT)
+    idsL = _i(ei if idsL is None else idsL); axaL = _d(e3 if axaL is None else axaL)
+    originL = _d(e3 if originL is None else originL)
+    prm = SlimParams(energy, 0, 0, 0, weight_opt, 1e5, 0.0, lam_C, lam_T, lam_L)
+    cs = SlimConstraints(0, _p(ei, C.c_int), _p(e3, C.c_double), 0, _p(ei, C.c_int),
+                         idsC.size, _p(idsC, C.c_int), _p(Cpos, C.c_double),
+                         idsT.size, _p(idsT, C.c_int), _p(normalT, C.c_double), _p(disT, C.c_double),
+                         idsL.size, _p(idsL, C.c_int), _p(axaL, C.c_double), _p(originL, C.c_double))
+    n = 3 * UV.shape[0] + idsL.size
+    L = np.zeros((n, n)) if dense else None
+    rhs = np.zeros(n)
+    pv = None if p is None else _d(p)
+    Lp = np.zeros(n); dg = np.zeros(n)
+    lib().orc_linear_system(UV.shape[0], H.shape[0], _p(H, C.c_int), _p(G, C.c_double), _p(UV, C.c_double),
+                            C.byref(prm), C.byref(cs), None if L is None else _p(L, C.c_double), _p(rhs, C.c_double),
+                            None if pv is None else _p(pv, C.c_double), _p(Lp, C.c_double), _p(dg, C.c_double))
+    if p is not None:
+        return L, rhs, Lp, dg
+    return L, rhs
+
+
+def sd_grad(H, G, X, M=1.0, lam_C=0.0, lam_T=0.0, lam_L=0.0, idsC=None, Cpos=None, idsT=None, normalT=None,
+            disT=None, idsL=None, axaL=None, originL=None, aL=None):
+    """Energy and gradient of the symmetric-Dirichlet SLIM energy (SVD form)."""
+    H, G, X = _i(H), _d(G), _d(X)
+    e3 = np.zeros((0, 3)); e1 = np.zeros(0); ei = np.zeros(0, dtype=np.int32)
+    idsC = _i(ei if idsC is None else idsC); Cpos = _d(e3 if Cpos is None else Cpos)
+    idsT = _i(ei if idsT is None else idsT); normalT = _d(e3 if normalT is None else normalT)
+    disT = _d(e1 if disT is None else disT)
+    idsL = _i(ei if idsL is None else idsL); axaL = _d(e3 if axaL is None else axaL)
+    originL = _d(e3 if originL is None else originL)
+    aL = _d(np.zeros(idsL.size) if aL is None else aL)
+    cs = SlimConstraints(0, _p(ei, C.c_int), _p(e3, C.c_double), 0, _p(ei, C.c_int),
+                         idsC.size, _p(idsC, C.c_int), _p(Cpos, C.c_double),
+                         idsT.size, _p(idsT, C.c_int), _p(normalT, C.c_double), _p(disT, C.c_double),
+                         idsL.size, _p(idsL, C.c_int), _p(axaL, C.c_double), _p(originL, C.c_double))
+    out = np.zeros_like(X)
+    E = C.c_double()
+    lib().orc_sd_grad(X.shape[0], H.shape[0], _p(H, C.c_int), _p(G, C.c_double), _p(X, C.c_double), C.c_double(M),
+                      C.byref(cs), C.c_double(lam_C), C.c_double(lam_T), C.c_double(lam_L), _p(aL, C.c_double),
+                      _p(out, C.c_double), C.byref(E))
+    return E.value, out

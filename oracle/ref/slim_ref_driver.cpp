@@ -27,6 +27,7 @@
 void update_weights_and_closest_rotations(SLIMData& s, const Eigen::MatrixXd& V, const Eigen::MatrixXi& F,
                                           Eigen::MatrixXd& uv);
 double compute_energy_total(SLIMData& s, Eigen::MatrixXd& V_new);
+void build_linear_system(SLIMData& s, Eigen::SparseMatrix<double>& L);
 
 namespace {
 struct In {
@@ -191,7 +192,7 @@ int main(int argc, char** argv) {
 
   std::vector<double> log;
   std::vector<double> Vtrace;
-  if (mode == 0) {
+  if (mode == 0 || mode == 3) {
     compute_referenceMesh(ts.V, m.Hs, H_flag, Hids, ts.RT, cube != 0);
     SLIMData data;
     data.stop_threshold = 1.e-7;
@@ -228,6 +229,19 @@ int main(int argc, char** argv) {
       }
       out.putd("W0", W);
       out.putd("R0", R);
+      if (mode == 3) {
+        // the assembled global system of the first iteration (small meshes only)
+        Eigen::SparseMatrix<double> Lm;
+        build_linear_system(d2, Lm);
+        std::vector<double> trip;
+        for (int k = 0; k < Lm.outerSize(); ++k)
+          for (Eigen::SparseMatrix<double>::InnerIterator it(Lm, k); it; ++it) {
+            trip.push_back(it.row()); trip.push_back(it.col()); trip.push_back(it.value());
+          }
+        out.putd("L_triplets", trip);
+        std::vector<double> rh(d2.rhs.data(), d2.rhs.data() + d2.rhs.size());
+        out.putd("rhs", rh);
+      }
     }
     for (int it = 0; it < n_inner; ++it) {
       slim_solve2(data, 1, -1, false);

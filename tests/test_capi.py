@@ -1,0 +1,34 @@
+"""The C ABI library loads and exports every function include/hexopt.h declares (CPU only:
+no compute call is made)."""
+import ctypes
+import os
+import re
+
+from pyhexopt import capi
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    src = open(os.path.join(REPO, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(hexopt_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    names = declared("hexopt.h")
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(names) == sorted(capi.EXPORTS)
+
+
+def test_create_fails_cleanly_on_bad_mesh():
+    lib = capi.lib()
+    import numpy as np
+    V = np.zeros((4, 3))
+    H = np.full((1, 8), 7, dtype=np.int32)   # corner index out of range
+    h = lib.hexopt_create(4, V.ctypes.data, 1, H.ctypes.data, 0)
+    assert not h
+    assert lib.hexopt_create_error().decode() != ""

@@ -1,0 +1,146 @@
+"""The CPU oracle against golden vectors written by the compiled reference
+(tests/golden/make_golden.py): this pins the oracle that every GPU parity test uses."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from helpers import aabb_cases, constraint_kwargs, load, rel, slim_cases
+
+
+@pytest.mark.parametrize("name", slim_cases())
+def test_grad_ref_matches_reference(name):
+    c = load(name)
+    hflag = c.get("hflag")
+    G = O.grad_ref(c["V"], c["H"], hflag=hflag, cube=True)
+    Gr = c["ref_G"].reshape(G.shape)
+    assert rel(G, Gr) < 1e-12
+
+
+@pytest.mark.parametrize("name", slim_cases())
+def test_local_step_matches_reference(name):
+    c = load(name)
+    G = O.grad_ref(c["V"], c["H"], hflag=c.get("hflag"))
+    W, R, _ = O.local_step(c["H"], G, c["V"], int(c["energy"]))
+    assert rel(W.reshape(-1), c["ref_W0"]) < 1e-9
+    assert rel(R.reshape(-1), c["ref_R0"]) < 1e-9
+
+
+# EXP_SYMMETRIC_DIRICHLET is used by no call site of the reference; its exponential weights
+# (~1e13 here) leave the 20-step CG so ill-conditioned that the trajectory depends on the
+# summation order, so it is pinned by its start energy and local step only.
+TRAJECTORY_CASES = [n for n in slim_cases() if not n.endswith("expsd")]
+
+
+@pytest.mark.parametrize("name", slim_cases())
+def test_start_energy_matches_reference(name):
+    c = load(name)
+    G = O.grad_ref(c["V"], c["H"], hflag=c.get("hflag"))
+    _, E, Eq, Es, _, _ = O.slim_m_opt(c["V"], c["H"], G, c["V"], 0, energy=int(c["energy"]),
+                                      weight_opt=float(c["weight_opt"]), **constraint_kwargs(c))
+    assert rel([E, Eq], c["ref_E0"][:2]) < 1e-12
+    assert abs(Es - c["ref_E0"][2]) <= 1e-12 * abs(Eq)
+
+
+def traj_tol(c):
+    """Relative tolerance of per-iteration energies and positions: 1e-10, or 20x the measured
+    order sensitivity of the reference's own 20-step CG on this input (the relative change of
+    its solve when its assembled system is merely permuted, recorded by make_golden.py) when
+    that is larger; no restatement with another summation order can be held closer."""
+    return max(1e-10, 20.0 * float(c["ref_cg_order_sensitivity"]))
+
+
+@pytest.mark.parametrize("name", TRAJECTORY_CASES)
+def test_slim_iterations_match_reference(name):
+    c = load(name)
+    G = O.grad_ref(c["V"], c["H"], hflag=c.get("hflag"))
+    iters = int(c["iters"])
+    UV, E, Eq, Es, log, aL = O.slim_m_opt(c["V"], c["H"], G, c["V"], iters, energy=int(c["energy"]),
+                                          weight_opt=float(c["weight_opt"]), **constraint_kwargs(c))
+    ref_log = c["ref_log"].reshape(iters, 3)
+    # energy, quality and soft energy per iteration (soft energy is a small difference of
+    # large terms on the constrained cases; its tolerance is relative to the total energy)
+    tol = traj_tol(c)
+    assert rel(log[:, 0], ref_log[:, 0]) < tol
+    assert rel(log[:, 1], ref_log[:, 1]) < tol
+    assert np.abs(log[:, 2] - ref_log[:, 2]).max() <= tol * np.abs(ref_log[:, 1]).max()
+    assert rel(UV.reshape(-1), c["ref_UV"]) < tol
+    mn, avg, per = O.scaled_jacobian(UV, c["H"])
+    assert abs(mn - c["ref_SJ"][0]) < 10 * tol and abs(avg - c["ref_SJ"][1]) < 10 * tol
+    if aL.size:
+        assert rel(aL, c["ref_aL"]) < max(1e-8, tol)
+
+
+@pytest.mark.parametrize("name", aabb_cases())
+def test_closest_point_matches_reference(name):
+    c = load(name)
+    nf = c["SF"].shape[0]
+    S, I, C, N, _ = O.signed_distance_pseudonormal(c["P"], c["SV"], c["SF"], c["ref_FN"].reshape(nf, 3),
+                                                  c["ref_VN"].reshape(-1, 3), c["ref_FEN"].reshape(nf, 9))
+    # Face ids agree except on exact ties (query equidistant from faces sharing the closest
+    # edge/vertex): igl's 3-element dot products use an alignment-dependent summation order,
+    # so which tied face wins is implementation-defined. Ties resolve to the same point and
+    # the same pseudonormal.
+    mism = np.nonzero(I != c["ref_I"])[0]
+    assert mism.size <= max(2, I.size // 200)
+    for q in mism:
+        f, g = c["SF"][I[q]], c["SF"][c["ref_I"][q]]
+        assert len(set(f.tolist()) & set(g.tolist())) >= 1
+    assert np.abs(C.reshape(-1) - c["ref_C"]).max() < 1e-14
+    assert np.abs(N.reshape(-1) - c["ref_N"]).max() < 1e-14
+    assert np.abs(S - c["ref_S"]).max() < 1e-14
+
+
+def _dense(trip, n):
+    t = trip.reshape(-1, 3)
+    L = np.zeros((n, n))
+    np.add.at(L, (t[:, 0].astype(int), t[:, 1].astype(int)), t[:, 2])
+    return L
+
+
+def _eigen_cg(A, b, x, iters=20, tol=1e-8):
+    """Eigen 3.2.9 ConjugateGradient with DiagonalPreconditioner (ConjugateGradient.h:27-90)."""
+    d = np.diag(A).copy()
+    inv = np.where(d != 0, 1.0 / np.where(d != 0, d, 1.0), 1.0)
+    r = b - A @ x
+    thr = tol * tol * (b @ b)
+    p = inv * r
+    absNew = r @ p
+    for _ in range(iters):
+        t = A @ p
+        a = absNew / (p @ t)
+        x = x + a * p
+        r = r - a * t
+        if r @ r < thr:
+            break
+        z = inv * r
+        old = absNew
+        absNew = r @ z
+        p = z + (absNew / old) * p
+    return x
+
+
+def test_oracle_operator_matches_reference_assembly():
+    """The oracle's matrix-free global system equals the reference's assembled L and rhs."""
+    c = load("slim_grid4_hflag")
+    G = O.grad_ref(c["V"], c["H"], hflag=c["hflag"])
+    L, rhs = O.linear_system(c["V"], c["H"], G, c["V"], **constraint_kwargs(c))
+    Lr = _dense(c["ref_L_triplets"], L.shape[0])
+    assert rel(L, Lr) < 1e-13
+    assert rel(rhs, c["ref_rhs"]) < 1e-13
+
+
+def test_reference_cg_is_order_sensitive():
+    """Permuting the unknowns of the reference's own system moves its 20-step CG result by
+    far more than 1e-10: the tolerance of the H_flag trajectory is set by this, not by the
+    restatement."""
+    c = load("slim_grid4_hflag")
+    n = 3 * c["V"].shape[0]
+    Lr = _dense(c["ref_L_triplets"], n)
+    b = c["ref_rhs"]
+    x0 = c["V"].T.reshape(-1).copy()
+    perm = np.arange(n)[::-1]
+    xa = _eigen_cg(Lr[np.ix_(perm, perm)], b[perm], x0[perm].copy())[np.argsort(perm)]
+    xb = _eigen_cg(Lr, b, x0.copy())
+    d = rel(xa, xb)
+    assert d > 1e-8
+    assert abs(d - float(c["ref_cg_order_sensitivity"])) <= 1e-3 * d
