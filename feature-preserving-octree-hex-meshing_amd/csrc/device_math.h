@@ -192,6 +192,27 @@ __device__ void slim_weights(const T J[3][3], int energy, T exp_f, T W[3][3], T 
     }
 }
 
+// Symmetric-Dirichlet SLIM weight matrix for the FP32 path: W = U diag(m) U^T with
+// m^2 = (s - s^-3)/(s - 1) evaluated as (s + 1)(s^2 + 1)/s^3 (the same function without the
+// cancellation at s -> 1 that FP32 cannot afford), and m = 1 where |s - 1| < 1e-8 as in
+// slim_m.cpp:351-353.
+template <class T>
+__device__ void slim_weights_sd_stable(const T J[3][3], T W[3][3]) {
+  T U[3][3], s[3], V[3][3];
+  svd3(J, U, s, V);
+  T m[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const T si = s[i];
+    const T is = T(1) / si;
+    m[i] = fabs(si - T(1)) < T(1e-8) ? T(1) : dsqrt((si + T(1)) * (si * si + T(1)) * is * is * is);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) W[i][j] = U[i][0] * m[0] * U[j][0] + U[i][1] * m[1] * U[j][1] + U[i][2] * m[2] * U[j][2];
+}
+
 // Per-tet energy density (slim_m.cpp:855-913). The symmetric Dirichlet and conformal
 // energies are evaluated in closed form from invariants (sum s_i^2 = |J|_F^2,
 // sum s_i^-2 = |adj J|_F^2 / det^2, s1 s2 s3 = |det J|), the rest from the singular values.

@@ -67,6 +67,8 @@ struct hexopt_ctx {
   std::vector<double> h_lt;  // nL*3 (caller copy for rhs/diag of the a_L rows)
   // adjoint scratch
   DevBuf d_X, d_G, d_v;
+  // FP32 optimiser state
+  DevBuf f_K, f_rhs, f_invd, f_d, f_r, f_z, f_p, f_tmp, f_e, f_halo, d_Ld64, d_halo64b, d_ninv;
 };
 
 namespace {
@@ -226,6 +228,16 @@ int hexopt_get_positions(hexopt_ctx* c, int which, double* V) {
   HIPCHK(c, hipMemcpyAsync(soa.data(), b.p, soa.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   from_soa(c->L, soa.data(), V);
+  return HEXOPT_OK;
+}
+
+int hexopt_copy_positions(hexopt_ctx* c, int from, int to) {
+  if (!c || (from != HEXOPT_REST && from != HEXOPT_CURRENT) || (to != HEXOPT_REST && to != HEXOPT_CURRENT))
+    return HEXOPT_EINVAL;
+  if (from == to) return HEXOPT_OK;
+  DevBuf& a = from == HEXOPT_REST ? c->d_rest : c->d_cur;
+  DevBuf& b = to == HEXOPT_REST ? c->d_rest : c->d_cur;
+  HIPCHK(c, hipMemcpyAsync(b.p, a.p, (size_t)3 * c->L.nV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   return HEXOPT_OK;
 }
 
@@ -524,6 +536,215 @@ int hexopt_debug_system(hexopt_ctx* c, const hexopt_params* prm, const double* p
   if ((rc = back(c->d_rhs, rhs_out))) return rc;
   if ((rc = back(c->d_invd, invd_out))) return rc;
   return back(c->d_tmp, Lp_out);
+}
+
+static int ensure_state32(hexopt_ctx* c) {
+  const int nH = c->L.nH;
+  const size_t n = (size_t)3 * c->L.nV + c->nL;
+  HIPCHK(c, c->f_K.ensure((size_t)48 * nH * sizeof(float)));
+  for (DevBuf* b : {&c->f_rhs, &c->f_invd, &c->f_d, &c->f_r, &c->f_z, &c->f_p, &c->f_tmp, &c->f_e})
+    HIPCHK(c, b->ensure(n * sizeof(float)));
+  HIPCHK(c, c->f_halo.ensure((size_t)6 * (c->L.nSlots + 1) * sizeof(float)));
+  HIPCHK(c, c->d_Ld64.ensure(n * sizeof(double)));
+  HIPCHK(c, c->d_ninv.ensure(sizeof(int)));
+  return HEXOPT_OK;
+}
+
+// FP32 CG on L x = b from x = 0 (Eigen's algorithm, FP64 dot products), `iters` iterations.
+static int cg32(hexopt_ctx* c, const SoftDev& sd, double prox, const float* b, float* x, int iters, int* it_out,
+                double* err_out) {
+  const int nV = c->L.nV, nL = c->nL;
+  const int n = 3 * nV + nL;
+  const int nb = nblocks(n);
+  hipStream_t st = c->stream;
+  RefOp ro = refop(c);
+  HIPCHK(c, hipMemsetAsync(x, 0, (size_t)n * sizeof(float), st));
+  HIPCHK(c, hipMemsetAsync(c->f_tmp.p, 0, (size_t)n * sizeof(float), st));
+  launch_cg_init32(n, b, c->f_tmp.as<float>(), c->f_r.as<float>(), c->d_part[0].as<double>(), c->d_part[1].as<double>(),
+                   nb, c->d_cg.p, c->f_invd.as<float>(), c->f_p.as<float>(), c->d_part[5].as<double>(), 1e-8, st);
+  const int nfix = (c->L.nShared + 255) / 256, nline = (nL + 255) / 256;
+  double* pt = c->d_part[2].as<double>();
+  double* pf = c->d_part[3].as<double>();
+  double* pl = c->d_part[4].as<double>();
+  for (int k = 0; k < iters; ++k) {
+    launch_apply32(c->tm, ro, c->f_K.as<float>(), c->f_p.as<float>(), c->f_tmp.as<float>(), prox, sd,
+                   c->f_halo.as<float>(), pt, pf, pl, st);
+    launch_cg_alpha(c->d_cg.p, pt, c->L.nTiles, pf, nfix, pl, nline, st);
+    launch_cg_update32(n, c->d_cg.p, x, c->f_r.as<float>(), c->f_z.as<float>(), c->f_p.as<float>(),
+                       c->f_tmp.as<float>(), c->f_invd.as<float>(), c->d_part[6].as<double>(),
+                       c->d_part[7].as<double>(), nb, c->f_p.as<float>(), st);
+  }
+  launch_cg_zero_if32(n, c->d_cg.p, x, nb, st);
+  if (it_out || err_out) {
+    std::vector<unsigned char> cgs(cg_state_bytes());
+    HIPCHK(c, hipMemcpyAsync(cgs.data(), c->d_cg.p, cgs.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    int done = 0, its = 0;
+    double err = 0;
+    read_cg_state(cgs.data(), &its, &err, &done);
+    if (it_out) *it_out = its;
+    if (err_out) *err_out = err;
+  }
+  return HEXOPT_OK;
+}
+
+int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt32_options* opt, int iters,
+                       hexopt_opt32_log* log, double* E_out) {
+  if (!c || !prm || !opt || iters < 0) return HEXOPT_EINVAL;
+  if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");
+  if (prm->energy != HEXOPT_SYMMETRIC_DIRICHLET)
+    return fail(c, HEXOPT_EUNSUPPORTED, "the FP32 adjoint optimiser supports SYMMETRIC_DIRICHLET");
+  if (prm->projection && c->sd.nb > 0) return fail(c, HEXOPT_EUNSUPPORTED, "projection constraints");
+  if (opt->cg_iters < 1 || opt->refine_iters < 0) return fail(c, HEXOPT_EINVAL, "bad iteration counts");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = ensure_state(c);
+  if (rc) return rc;
+  if ((rc = ensure_state32(c))) return rc;
+  const int nV = c->L.nV, nH = c->L.nH, nL = c->nL;
+  const int n = 3 * nV + nL;
+  const double M = prm->weight_opt, prox = 0.00001 * prm->weight_opt, mesh_area = M * 8.0 * nH;
+  hipStream_t st = c->stream;
+  SoftDev sd = soft_with(c, prm);
+  RefOp ro = refop(c);
+  double* cur = c->d_cur.as<double>();
+  double* scal = c->d_scal.as<double>();
+  const int nsb = nblocks(c->sd.nb + c->sd.nC + c->sd.nT + c->sd.nL);
+  double energy = 0;
+  for (int it = 0; it < iters; ++it) {
+    // one pass over the elements: energy, adjoint rhs -dE/dx/2, K, Jacobi diagonal
+    HIPCHK(c, hipMemsetAsync(c->d_ninv.p, 0, sizeof(int), st));
+    launch_adj_local32(c->tm, ro, cur, c->f_K.as<float>(), M, prox, sd, c->f_rhs.as<float>(), c->f_invd.as<float>(),
+                       c->f_halo.as<float>(), c->d_part[0].as<double>(), c->d_ninv.as<int>(), st);
+    // direction: L d = -dE/dx / 2 by FP32 CG, then one FP64-residual refinement solve
+    int cg_it = 0;
+    double cg_err = 0;
+    float* d = c->f_d.as<float>();
+    if ((rc = cg32(c, sd, prox, c->f_rhs.as<float>(), d, opt->cg_iters, &cg_it, &cg_err))) return rc;
+    launch_apply32_f64(c->tm, ro, c->f_K.as<float>(), d, c->d_Ld64.as<double>(), prox, sd, c->d_halo.as<double>(), st);
+    const int nb = nblocks(n);
+    launch_residual64(n, c->f_rhs.as<float>(), c->d_Ld64.as<double>(), c->f_r.as<float>(), c->d_part[1].as<double>(),
+                      nb, scal + 6, st);
+    launch_sum_partials(c->d_part[0].as<double>(), c->L.nTiles, scal + 8, st);
+    double h3[3];
+    HIPCHK(c, hipMemcpyAsync(h3, scal + 6, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    double res64 = h3[0];
+    if (opt->refine_iters > 0) {
+      // residual r (FP32 copy of the FP64 residual) -> correction e, d += e
+      HIPCHK(c, hipMemcpyAsync(c->f_z.p, c->f_r.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      float* e = c->f_e.as<float>();
+      // cg32 overwrites f_r/f_p/f_tmp; the right-hand side lives in f_z only until cg_init copies it
+      HIPCHK(c, hipMemcpyAsync(c->d_Ld64.p, c->f_z.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      if ((rc = cg32(c, sd, prox, (const float*)c->d_Ld64.p, e, opt->refine_iters, nullptr, nullptr))) return rc;
+      launch_add_inplace(n, d, e, nb, st);
+    }
+    // a_L is the line unknowns' solution (as slim_m.cpp:501-503), used by the soft energy
+    double* aL = c->d_aL.as<double>();
+    if (nL > 0) {
+      std::vector<float> ha(nL);
+      std::vector<double> hd(nL);
+      HIPCHK(c, hipMemcpyAsync(ha.data(), d + 3 * (size_t)nV, nL * sizeof(float), hipMemcpyDeviceToHost, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+      for (int i = 0; i < nL; ++i) hd[i] = ha[i];
+      HIPCHK(c, hipMemcpyAsync(aL, hd.data(), nL * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    // FP64 flip-free bound, then the reference's bisection on FP32 energies summed in FP64
+    launch_max_step_dir32(cur, d, nV, c->tm.conn, nH, c->d_part[0].as<double>(), scal + 4, st);
+    double tmin = 0;
+    int ninv = 0;
+    HIPCHK(c, hipMemcpyAsync(&tmin, scal + 4, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(&ninv, c->d_ninv.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    launch_energy_dir32(ro, cur, d, 0.0, nV, c->tm.conn, nH, M, sd, aL, c->d_part[0].as<double>(),
+                        c->d_part[1].as<double>(), nsb, scal, st);
+    double h2[2];
+    HIPCHK(c, hipMemcpyAsync(h2, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(st));
+    const double old_energy = h2[0] + h2[1];
+    double step = std::min(1., tmin * 0.8);
+    const double bound = step;
+    double new_energy = old_energy, eq = h2[0], es = h2[1];
+    int cur_iter = 0;
+    while (new_energy >= old_energy) {
+      if (cur_iter > 20) step = 0;
+      launch_energy_dir32(ro, cur, d, step, nV, c->tm.conn, nH, M, sd, aL, c->d_part[0].as<double>(),
+                          c->d_part[1].as<double>(), nsb, scal, st);
+      HIPCHK(c, hipMemcpyAsync(h2, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+      const double ce = h2[0] + h2[1];
+      if (std::isnan(ce) || std::isinf(ce) || ce > old_energy) {
+        step /= 2;
+      } else {
+        launch_step_dir32(3 * nV, cur, d, step, st);
+        new_energy = ce;
+        eq = h2[0];
+        es = h2[1];
+        break;
+      }
+      if (step == 0) break;
+      cur_iter++;
+    }
+    energy = new_energy / mesh_area;
+    if (log) {
+      hexopt_opt32_log& lg = log[it];
+      lg.energy = energy; lg.e_quality = eq; lg.e_soft = es; lg.step_bound = bound; lg.step = step;
+      lg.cg_iters = cg_it; lg.cg_error = cg_err; lg.residual64 = h3[2] > 0 ? std::sqrt(res64) : std::sqrt(res64);
+      lg.inverted_tets = ninv;
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(st));
+  if (E_out) *E_out = energy;
+  return HEXOPT_OK;
+}
+
+int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch) {
+  if (!c || !ms_per_launch || reps < 1) return HEXOPT_EINVAL;
+  if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");
+  int rc = ensure_state(c);
+  if (rc) return rc;
+  if ((rc = ensure_state32(c))) return rc;
+  hipStream_t st = c->stream;
+  RefOp ro = refop(c);
+  SoftDev sd = c->sd;
+  TileMesh tm = c->tm;
+  tm.nShared = 0;  // the tile kernel alone (the fixup pass is timed by the profiler)
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  auto one = [&]() {
+    switch (which) {
+      case 0:
+        launch_apply(tm, ro, c->d_K.as<double>(), c->d_p.as<double>(), c->d_tmp.as<double>(), 1e-5, sd,
+                     c->d_halo.as<double>(), c->d_part[2].as<double>(), nullptr, nullptr, st);
+        break;
+      case 1:
+        launch_apply32(tm, ro, c->f_K.as<float>(), c->f_p.as<float>(), c->f_tmp.as<float>(), 1e-5, sd,
+                       c->f_halo.as<float>(), c->d_part[2].as<double>(), nullptr, nullptr, st);
+        break;
+      case 2:
+        launch_local_step(tm, ro, c->d_cur.as<double>(), c->d_K.as<double>(), HEXOPT_SYMMETRIC_DIRICHLET, 1.0, 1.0,
+                          1e-5, sd, c->d_rhs.as<double>(), c->d_invd.as<double>(), c->d_halo.as<double>(), st);
+        break;
+      default:
+        launch_adj_local32(tm, ro, c->d_cur.as<double>(), c->f_K.as<float>(), 1.0, 1e-5, sd, c->f_rhs.as<float>(),
+                           c->f_invd.as<float>(), c->f_halo.as<float>(), c->d_part[0].as<double>(),
+                           c->d_ninv.as<int>(), st);
+    }
+  };
+  sd.nL = 0;
+  one();  // warm
+  HIPCHK(c, hipEventRecord(e0, st));
+  for (int i = 0; i < reps; ++i) one();
+  HIPCHK(c, hipEventRecord(e1, st));
+  HIPCHK(c, hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per_launch = ms / reps;
+  HIPCHK(c, hipGetLastError());
+  return HEXOPT_OK;
 }
 
 int hexopt_energy(hexopt_ctx* c, const hexopt_params* prm, const double* aL, double* Eq, double* Es) {

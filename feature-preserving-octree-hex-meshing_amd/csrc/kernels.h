@@ -75,6 +75,31 @@ void launch_grad(int prec, const TileMesh& tm, const RefOp& ro, const void* X, v
                  const double* aL, void* halo, hipStream_t st);
 void launch_hvp(int prec, const TileMesh& tm, const RefOp& ro, const void* X, const void* Vd, void* Hv, double M,
                 const SoftDev& sd, void* halo, hipStream_t st);
+// FP32 optimiser (adjoint formulation of the SLIM step)
+void launch_adj_local32(const TileMesh& tm, const RefOp& ro, const double* X, float* K, double M, double prox,
+                        const SoftDev& sd, float* rhs, float* invd, float* halo, double* hex_part, int* n_inverted,
+                        hipStream_t st);
+void launch_apply32(const TileMesh& tm, const RefOp& ro, const float* K, const float* p, float* out, double prox,
+                    const SoftDev& sd, float* halo, double* part_tile, double* part_fix, double* part_line,
+                    hipStream_t st);
+void launch_apply32_f64(const TileMesh& tm, const RefOp& ro, const float* K, const float* p, double* out, double prox,
+                        const SoftDev& sd, double* halo, hipStream_t st);
+void launch_cg_init32(int n, const float* rhs, const float* tmp, float* r, double* pb, double* pr, int nb, void* state,
+                      const float* invd, float* p, double* pabs, double tol, hipStream_t st);
+void launch_cg_update32(int n, void* state, float* x, float* r, float* z, const float* p, const float* tmp,
+                        const float* invd, double* prr, double* prz, int nb, float* p_out, hipStream_t st);
+void launch_cg_zero_if32(int n, void* state, float* x, int nb, hipStream_t st);
+void launch_residual64(int n, const float* b, const double* Ld, float* r32, double* part, int nb, double* out,
+                       hipStream_t st);
+void launch_add_inplace(int n, float* d, const float* e, int nb, hipStream_t st);
+void launch_max_step_dir32(const double* cur, const float* d, int nV, const int* conn, int nH, double* part,
+                           double* out, hipStream_t st);
+void launch_energy_dir32(const RefOp& ro, const double* cur, const float* d, double step, int nV, const int* conn,
+                         int nH, double M, const SoftDev& sd, const double* aL, double* part_q, double* part_s,
+                         int ns_blocks, double* out2, hipStream_t st);
+void launch_step_dir32(int n, double* cur, const float* d, double step, hipStream_t st);
+void launch_sum_partials(const double* part, int n, double* out, hipStream_t st);
+
 void launch_scaled_jacobian(const double* X, int nV, const int* conn, int nH, double* per_hex, double* pmin,
                             double* psum, double* out3, hipStream_t st);
 

@@ -123,13 +123,15 @@ __device__ __forceinline__ void gather8(const P* x, int nV, const int* conn, int
 
 // ---------------------------------------------------------------------------------------
 // Tiled scatter skeleton. Op provides:
-//   __device__ void hex(int h, T out[8][NC])          per-hex corner contributions
-//   __device__ T finish(int v, T acc[NC])              writes vertex v; returns its share of
-//                                                       the kernel's optional block sum
+//   __device__ double hex(int h, T out[8][NC])        per-hex corner contributions; returns
+//                                                       the hex's share of an optional sum
+//   __device__ double finish(int v, T acc[NC])        writes vertex v; returns its share of
+//                                                       the kernel's optional vertex sum
 template <class T, int NC, class Op>
-__global__ __launch_bounds__(kTileHex) void tile_scatter_kernel(TileMesh tm, Op op, T* halo, T* partial) {
+__global__ __launch_bounds__(kTileHex) void tile_scatter_kernel(TileMesh tm, Op op, T* halo, double* partial,
+                                                                 double* hex_partial) {
   __shared__ T acc[NC * kMaxLV];
-  __shared__ T red[16];
+  __shared__ double red[16];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int h0 = tm.tile_hex[tile], nh = tm.tile_hex[tile + 1] - h0;
@@ -138,14 +140,19 @@ __global__ __launch_bounds__(kTileHex) void tile_scatter_kernel(TileMesh tm, Op 
   T c[8][NC];
   int li[8];
   const bool active = tid < nh;
+  double hv = 0.0;
 #pragma unroll
   for (int k = 0; k < 8; ++k)
 #pragma unroll
     for (int j = 0; j < NC; ++j) c[k][j] = T(0);
   if (active) {
-    op.hex(h0 + tid, c);
+    hv = op.hex(h0 + tid, c);
 #pragma unroll
     for (int k = 0; k < 8; ++k) li[k] = tm.lidx[(size_t)k * tm.nH + h0 + tid];
+  }
+  if (hex_partial) {
+    double sh = block_sum(hv, red);
+    if (tid == 0) hex_partial[tile] = sh;
   }
   __syncthreads();
   const uint16_t* off = tm.tile_col + (size_t)tile * (kMaxCol + 1);
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(kTileHex) void tile_scatter_kernel(TileMesh tm, Op 
     }
     __syncthreads();
   }
-  T part = T(0);
+  double part = 0.0;
   for (int lv = tid; lv < nlv; lv += kTileHex) {
     const int g = tm.lv_gid[lv0 + lv], s = tm.lv_slot[lv0 + lv];
     T v[NC];
@@ -174,16 +181,16 @@ __global__ __launch_bounds__(kTileHex) void tile_scatter_kernel(TileMesh tm, Op 
     }
   }
   if (partial) {
-    T s = block_sum(part, red);
+    double s = block_sum(part, red);
     if (tid == 0) partial[tile] = s;
   }
 }
 
 template <class T, int NC, class Op>
-__global__ __launch_bounds__(256) void fixup_kernel(TileMesh tm, Op op, const T* halo, T* partial) {
-  __shared__ T red[16];
+__global__ __launch_bounds__(256) void fixup_kernel(TileMesh tm, Op op, const T* halo, double* partial) {
+  __shared__ double red[16];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  T part = T(0);
+  double part = 0.0;
   if (i < tm.nShared) {
     const int g = tm.sv_gid[i];
     T v[NC];
@@ -197,15 +204,17 @@ __global__ __launch_bounds__(256) void fixup_kernel(TileMesh tm, Op op, const T*
     part = op.finish(g, v);
   }
   if (partial) {
-    T s = block_sum(part, red);
+    double s = block_sum(part, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
   }
 }
 
 template <class T, int NC, class Op>
-void launch_scatter(const TileMesh& tm, const Op& op, T* halo, T* part_tile, T* part_fix, hipStream_t st) {
+void launch_scatter(const TileMesh& tm, const Op& op, T* halo, double* part_tile, double* part_fix, hipStream_t st,
+                    double* hex_part = nullptr) {
   if (tm.nTiles > 0)
-    hipLaunchKernelGGL((tile_scatter_kernel<T, NC, Op>), dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, op, halo, part_tile);
+    hipLaunchKernelGGL((tile_scatter_kernel<T, NC, Op>), dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, op, halo, part_tile,
+                       hex_part);
   if (tm.nShared > 0)
     hipLaunchKernelGGL((fixup_kernel<T, NC, Op>), dim3((tm.nShared + 255) / 256), dim3(256), 0, st, tm, op,
                        (const T*)halo, part_fix);
@@ -373,7 +382,7 @@ struct LocalStepOp {
   RefOp ro; int nV, nH; const int* conn; const double* X; double* K;
   int energy; double M, exp_f, prox;
   SoftDev sd; double* rhs; double* invd;
-  __device__ void hex(int h, double out[8][6]) const {
+  __device__ double hex(int h, double out[8][6]) const {
     int vid[8];
     double p[8][3];
     gather8(X, nV, conn, nH, h, vid, p);
@@ -424,6 +433,7 @@ struct LocalStepOp {
         for (int d = 0; d < 3; ++d) out[kHexTet[c][k + 1]][3 + d] += Kt[d][d] * nk;
       }
     }
+    return 0.0;
   }
   __device__ double finish(int g, double acc[6]) const {
     double dg[3] = {acc[3] + prox, acc[4] + prox, acc[5] + prox};
@@ -447,35 +457,40 @@ struct LocalStepOp {
 };
 
 // CG operator L p = A^T M A p + prox p + soft terms; finish() also returns p.(Lp).
-template <class KT>
+// KT: stored K precision, PT: vector precision, CT: arithmetic, OT: output precision.
+template <class KT, class PT, class CT, class OT>
 struct ApplyOp {
-  RefOp ro; int nV, nH; const int* conn; const KT* K; const double* p; double* out; double prox;
-  SoftDev sd; const double* pa; int nL; const int* done;
-  __device__ void hex(int h, double o[8][3]) const {
-    int vid[8];
-    double q[8][3];
-    gather8(p, nV, conn, nH, h, vid, q);
+  RefOp ro; int nV, nH; const int* conn; const KT* K; const PT* p; OT* out; double prox;
+  SoftDev sd; const PT* pa; int nL;
+  __device__ double hex(int h, CT o[8][3]) const {
+    CT q[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int v = conn[(size_t)k * nH + h];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q[k][d] = (CT)p[(size_t)d * nV + v];
+    }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      double E[3][3], B[3][3], Y[3][3];
+      CT E[3][3], B[3][3], Y[3][3];
       tet_edges(q, c, E);
       const KT* kk = K + (size_t)c * 6 * nH + h;
-      const double k00 = kk[0], k11 = kk[(size_t)nH], k22 = kk[2 * (size_t)nH], k01 = kk[3 * (size_t)nH],
-                   k02 = kk[4 * (size_t)nH], k12 = kk[5 * (size_t)nH];
+      const CT k00 = (CT)kk[0], k11 = (CT)kk[(size_t)nH], k22 = (CT)kk[2 * (size_t)nH], k01 = (CT)kk[3 * (size_t)nH],
+               k02 = (CT)kk[4 * (size_t)nH], k12 = (CT)kk[5 * (size_t)nH];
       if (ro.mode == REF_CUBE) {
         // invs^2 is folded into K: forces on the tet edges are K E (columns = edges)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const double f0 = k00 * E[0][k] + k01 * E[1][k] + k02 * E[2][k];
-          const double f1 = k01 * E[0][k] + k11 * E[1][k] + k12 * E[2][k];
-          const double f2 = k02 * E[0][k] + k12 * E[1][k] + k22 * E[2][k];
+          const CT f0 = k00 * E[0][k] + k01 * E[1][k] + k02 * E[2][k];
+          const CT f1 = k01 * E[0][k] + k11 * E[1][k] + k12 * E[2][k];
+          const CT f2 = k02 * E[0][k] + k12 * E[1][k] + k22 * E[2][k];
           const int vk = kHexTet[c][k + 1], v0 = kHexTet[c][0];
           o[vk][0] += f0; o[vk][1] += f1; o[vk][2] += f2;
           o[v0][0] -= f0; o[v0][1] -= f1; o[v0][2] -= f2;
         }
       } else {
         tet_B(ro, nH, h, c, B);
-        double J[3][3];
+        CT J[3][3];
         matmul3(E, B, J);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -486,52 +501,177 @@ struct ApplyOp {
         scatter_tet(Y, B, c, o);
       }
     }
+    return 0.0;
   }
-  __device__ double finish(int g, double acc[3]) const {
-    double pv[3] = {p[g], p[(size_t)nV + g], p[2 * (size_t)nV + g]};
-    double r[3] = {acc[0] + prox * pv[0], acc[1] + prox * pv[1], acc[2] + prox * pv[2]};
-    soft_apply(sd, g, pv, nV, pa, r);
+  __device__ double finish(int g, CT acc[3]) const {
+    double pv[3] = {(double)p[g], (double)p[(size_t)nV + g], (double)p[2 * (size_t)nV + g]};
+    double r[3] = {(double)acc[0] + prox * pv[0], (double)acc[1] + prox * pv[1], (double)acc[2] + prox * pv[2]};
+    const int ci = sd.cidx ? sd.cidx[g] : -1;
+    if (ci >= 0) {
+      const double* S = sd.S + (size_t)ci * 6;
+      r[0] += S[0] * pv[0] + S[3] * pv[1] + S[4] * pv[2];
+      r[1] += S[3] * pv[0] + S[1] * pv[1] + S[5] * pv[2];
+      r[2] += S[4] * pv[0] + S[5] * pv[1] + S[2] * pv[2];
+      const int li = sd.cline[ci];
+      if (li >= 0) {
+        const double a = (double)pa[li];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) out[(size_t)d * nV + g] = r[d];
+        for (int d = 0; d < 3; ++d) r[d] -= sd.lamL * sd.lt[(size_t)li * 3 + d] * a;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[(size_t)d * nV + g] = (OT)r[d];
     return pv[0] * r[0] + pv[1] * r[1] + pv[2] * r[2];
   }
 };
 
+// ---------------------------------------------------------------------------------------
+// FP32 adjoint + local step of the symmetric-Dirichlet energy (the FP32 optimiser's only
+// pass over the elements per iteration): per corner tet in FP32 arithmetic the Jacobian,
+// its SVD, the SLIM weights K = M W^T W (stored FP32), the energy, and dE/dJ; scatters the
+// direction rhs -dE/dx / 2 and the Jacobi diagonal. The corner-determinant sign is tested
+// in FP64 (inverted-tet count); the energy is summed in FP64.
+struct AdjLocalOp32 {
+  RefOp ro; int nV, nH; const int* conn; const double* X; float* K; double M, prox;
+  SoftDev sd; float* rhs; float* invd; int* n_inverted;
+  __device__ double hex(int h, float out[8][6]) const {
+    int vid[8];
+    double pd[8][3];
+    gather8(X, nV, conn, nH, h, vid, pd);
+    float p[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) p[k][d] = (float)pd[k][d];
+    double e = 0.0;
+    int inv = 0;
+#pragma unroll 1
+    for (int c = 0; c < 8; ++c) {
+      {  // FP64 orientation test of the corner tet: sign of det J = det(E B)
+        double E64[3][3], B64[3][3], J64[3][3];
+        tet_edges(pd, c, E64);
+        tet_B(ro, nH, h, c, B64);
+        matmul3(E64, B64, J64);
+        inv += det3(J64) < 0.0;
+      }
+      float B[3][3], E[3][3], J[3][3], W[3][3], R[3][3], P[3][3];
+      tet_B(ro, nH, h, c, B);
+      tet_edges(p, c, E);
+      matmul3(E, B, J);
+      slim_weights_sd_stable(J, W);
+      e += M * (double)sd_energy_grad(J, P);
+      float Kt[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Kt[i][j] = (float)M * (W[0][i] * W[0][j] + W[1][i] * W[1][j] + W[2][i] * W[2][j]);
+      float s2 = 1.0f;
+      if (ro.mode == REF_CUBE) { float s = (float)ro.invs[(size_t)c * nH + h]; s2 = s * s; }
+      const float ks[6] = {Kt[0][0], Kt[1][1], Kt[2][2], Kt[0][1], Kt[0][2], Kt[1][2]};
+#pragma unroll
+      for (int j = 0; j < 6; ++j) K[((size_t)c * 6 + j) * nH + h] = ks[j] * s2;
+      const float hm = -0.5f * (float)M;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P[i][j] *= hm;
+      float tmp[8][3];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) tmp[k][0] = tmp[k][1] = tmp[k][2] = 0.0f;
+      scatter_tet(P, B, c, tmp);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[k][d] += tmp[k][d];
+      float g0[3] = {-(B[0][0] + B[1][0] + B[2][0]), -(B[0][1] + B[1][1] + B[2][1]), -(B[0][2] + B[1][2] + B[2][2])};
+      float n0 = g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) out[kHexTet[c][0]][3 + d] += Kt[d][d] * n0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float nk = B[k][0] * B[k][0] + B[k][1] * B[k][1] + B[k][2] * B[k][2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[kHexTet[c][k + 1]][3 + d] += Kt[d][d] * nk;
+      }
+      (void)R;
+    }
+    if (inv) atomicAdd(n_inverted, inv);
+    return e;
+  }
+  __device__ double finish(int g, float acc[6]) const {
+    double dg[3] = {(double)acc[3] + prox, (double)acc[4] + prox, (double)acc[5] + prox};
+    double r[3] = {(double)acc[0], (double)acc[1], (double)acc[2]};
+    const int ci = sd.cidx ? sd.cidx[g] : -1;
+    if (ci >= 0) {
+      const double* S = sd.S + (size_t)ci * 6;
+      const double* b = sd.b + (size_t)ci * 3;
+      const double x[3] = {X[g], X[(size_t)nV + g], X[2 * (size_t)nV + g]};
+      r[0] -= S[0] * x[0] + S[3] * x[1] + S[4] * x[2] - b[0];
+      r[1] -= S[3] * x[0] + S[1] * x[1] + S[5] * x[2] - b[1];
+      r[2] -= S[4] * x[0] + S[5] * x[1] + S[2] * x[2] - b[2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dg[d] += S[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      rhs[(size_t)d * nV + g] = (float)r[d];
+      invd[(size_t)d * nV + g] = dg[d] != 0.0 ? (float)(1.0 / dg[d]) : 1.0f;
+    }
+    return 0.0;
+  }
+};
+
+// rhs of the a_L rows of the direction system at a_L = 0: lamL t . (x_v - o)
+__global__ void line_rhs32_kernel(SoftDev sd, int nV, const double* X, float* rhs_a, float* invd_a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= sd.nL) return;
+  const int v = sd.lv[i];
+  const double* t = sd.lt + (size_t)i * 3;
+  const double* o = sd.lo + (size_t)i * 3;
+  double s = 0, tt = 0;
+  for (int d = 0; d < 3; ++d) { s += t[d] * (X[(size_t)d * nV + v] - o[d]); tt += t[d] * t[d]; }
+  rhs_a[i] = (float)(sd.lamL * s);
+  double dg = sd.lamL * tt;
+  invd_a[i] = dg != 0 ? (float)(1.0 / dg) : 1.0f;
+}
+
 // Rows of the feature-line unknowns a_L: (L p)_a = lamL (|t|^2 pa - t . p_v).
-__global__ void line_apply_kernel(SoftDev sd, int nV, const double* p, const double* pa, double* outa, int nL,
-                                  double* partial) {
+template <class PT, class OT>
+__global__ void line_apply_kernel(SoftDev sd, int nV, const PT* p, const PT* pa, OT* outa, int nL, double* partial) {
   __shared__ double red[16];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double part = 0;
   if (i < nL) {
     const int v = sd.lv[i];
     const double* t = sd.lt + (size_t)i * 3;
-    double tp = t[0] * p[v] + t[1] * p[(size_t)nV + v] + t[2] * p[2 * (size_t)nV + v];
+    double tp = t[0] * (double)p[v] + t[1] * (double)p[(size_t)nV + v] + t[2] * (double)p[2 * (size_t)nV + v];
     double tt = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
-    double r = sd.lamL * (tt * pa[i] - tp);
-    outa[i] = r;
-    part = pa[i] * r;
+    double r = sd.lamL * (tt * (double)pa[i] - tp);
+    outa[i] = (OT)r;
+    part = (double)pa[i] * r;
   }
   double s = block_sum(part, red);
   if (threadIdx.x == 0 && partial) partial[blockIdx.x] = s;
 }
 
 // ---------------------------------------------------------------------------------------
-// CG vector kernels. The flattened unknown vector has n = 3 nV + nL entries.
+// CG vector kernels (ConjugateGradient.h:27-90). The flattened unknown vector has
+// n = 3 nV + nL entries; T is the vector precision, every dot product is summed in FP64.
 struct CGState {
   double rhsNorm2, threshold, absNew, alpha, beta, rn2, tol_error;
   int done, iters, zero_rhs, pad;
 };
 
 // r = rhs - tmp, partial |rhs|^2, |r|^2
-__global__ void cg_init_kernel(int n, const double* rhs, const double* tmp, double* r, double* part_b, double* part_r) {
+template <class T>
+__global__ void cg_init_kernel(int n, const T* rhs, const T* tmp, T* r, double* part_b, double* part_r) {
   __shared__ double red[16];
   double sb = 0, sr = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double v = rhs[i] - tmp[i];
+    T v = rhs[i] - tmp[i];
     r[i] = v;
-    sb += rhs[i] * rhs[i];
-    sr += v * v;
+    sb += (double)rhs[i] * (double)rhs[i];
+    sr += (double)v * (double)v;
   }
   double a = block_sum(sb, red);
   double b = block_sum(sr, red);
@@ -562,15 +702,15 @@ __global__ void cg_init_fin_kernel(CGState* st, const double* part_b, int nb, co
 }
 
 // p = invd r, partial r.p
-__global__ void cg_first_dir_kernel(int n, const CGState* st, const double* invd, const double* r, double* p,
-                                    double* part) {
+template <class T>
+__global__ void cg_first_dir_kernel(int n, const CGState* st, const T* invd, const T* r, T* p, double* part) {
   __shared__ double red[16];
   double s = 0;
   if (!st->done)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-      double v = invd[i] * r[i];
+      T v = invd[i] * r[i];
       p[i] = v;
-      s += r[i] * v;
+      s += (double)r[i] * (double)v;
     }
   double a = block_sum(s, red);
   if (threadIdx.x == 0) part[blockIdx.x] = a;
@@ -592,20 +732,21 @@ __global__ void cg_alpha_kernel(CGState* st, const double* p1, int n1, const dou
 }
 
 // x += alpha p ; r -= alpha tmp ; z = invd r ; partial |r|^2, r.z
-__global__ void cg_update_kernel(int n, const CGState* st, double* x, double* r, double* z, const double* p,
-                                 const double* tmp, const double* invd, double* part_rr, double* part_rz) {
+template <class T>
+__global__ void cg_update_kernel(int n, const CGState* st, T* x, T* r, T* z, const T* p, const T* tmp, const T* invd,
+                                 double* part_rr, double* part_rz) {
   __shared__ double red[16];
   double srr = 0, srz = 0;
   if (!st->done) {
-    const double alpha = st->alpha;
+    const T alpha = (T)st->alpha;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
       x[i] += alpha * p[i];
-      double rv = r[i] - alpha * tmp[i];
+      T rv = r[i] - alpha * tmp[i];
       r[i] = rv;
-      double zv = invd[i] * rv;
+      T zv = invd[i] * rv;
       z[i] = zv;
-      srr += rv * rv;
-      srz += rv * zv;
+      srr += (double)rv * (double)rv;
+      srz += (double)rv * (double)zv;
     }
   }
   double a = block_sum(srr, red);
@@ -632,15 +773,40 @@ __global__ void cg_beta_kernel(CGState* st, const double* part_rr, const double*
   }
 }
 
-__global__ void cg_pdir_kernel(int n, const CGState* st, double* p, const double* z) {
+template <class T>
+__global__ void cg_pdir_kernel(int n, const CGState* st, T* p, const T* z) {
   if (st->done) return;
-  const double beta = st->beta;
+  const T beta = (T)st->beta;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
 }
 
-__global__ void cg_zero_if_kernel(int n, const CGState* st, double* x) {
+template <class T>
+__global__ void cg_zero_if_kernel(int n, const CGState* st, T* x) {
   if (!st->zero_rhs) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = 0;
+}
+
+// residual of the direction system in FP64: r64 = b - L d (L d given in FP64), partial |r64|^2
+__global__ void residual64_kernel(int n, const float* b, const double* Ld, float* r32, double* part) {
+  __shared__ double red[16];
+  double s = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double r = (double)b[i] - Ld[i];
+    r32[i] = (float)r;
+    s += r * r;
+  }
+  double a = block_sum(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+__global__ void add_inplace_kernel(int n, float* d, const float* e) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] += e[i];
+}
+
+__global__ void sum_fin_kernel(const double* part, int n, double* out) {
+  __shared__ double red[16];
+  double a = sum_parts(part, n, red);
+  if (threadIdx.x == 0) *out = a;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -738,8 +904,22 @@ __device__ double tet_min_root(const double x[4][3], const double d[4][3]) {
   return INFINITY;
 }
 
+// Point on the search line. DIFF: dst is the global-step solution, the direction is
+// dst - cur (flip_avoiding_line_search.cpp:307); otherwise dst already is the direction.
+template <bool DIFF, class DT>
+__device__ __forceinline__ double line_point(double a, DT q, double t) {
+  if (DIFF) return a + t * ((double)q - a);
+  return a + t * (double)q;
+}
+template <bool DIFF, class DT>
+__device__ __forceinline__ double line_dir(double a, DT q) {
+  if (DIFF) return (double)q - a;
+  return (double)q;
+}
+
 // direction d = dst - cur, taken per vertex on the fly
-__global__ void max_step_kernel(const double* cur, const double* dst, int nV, const int* conn, int nH, double* part) {
+template <bool DIFF, class DT>
+__global__ void max_step_kernel(const double* cur, const DT* dst, int nV, const int* conn, int nH, double* part) {
   __shared__ double red[16];
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   double m = INFINITY;
@@ -747,9 +927,8 @@ __global__ void max_step_kernel(const double* cur, const double* dst, int nV, co
     int vid[8];
     double p[8][3], q[8][3];
     gather8(cur, nV, conn, nH, h, vid, p);
-    gather8(dst, nV, conn, nH, h, vid, q);
     for (int k = 0; k < 8; ++k)
-      for (int d = 0; d < 3; ++d) q[k][d] -= p[k][d];
+      for (int d = 0; d < 3; ++d) q[k][d] = line_dir<DIFF>(p[k][d], dst[(size_t)d * nV + vid[k]]);
 #pragma unroll 1
     for (int c = 0; c < 8; ++c) {
       double x[4][3], dd[4][3];
@@ -772,8 +951,8 @@ __global__ void min_fin_kernel(const double* part, int n, double* out) {
 
 // ---------------------------------------------------------------------------------------
 // Energy of X = cur + step (dst - cur) (compute_energy_with_jacobians); per-block partials.
-template <class T>
-__global__ void energy_kernel(RefOp ro, const double* cur, const double* dst, double step, int nV, const int* conn,
+template <class T, bool DIFF, class DT>
+__global__ void energy_kernel(RefOp ro, const double* cur, const DT* dst, double step, int nV, const int* conn,
                               int nH, int energy, double M, double exp_f, double* part) {
   __shared__ double red[16];
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -783,10 +962,8 @@ __global__ void energy_kernel(RefOp ro, const double* cur, const double* dst, do
     double p[8][3];
     gather8(cur, nV, conn, nH, h, vid, p);
     if (dst) {
-      double q[8][3];
-      gather8(dst, nV, conn, nH, h, vid, q);
       for (int k = 0; k < 8; ++k)
-        for (int d = 0; d < 3; ++d) p[k][d] = p[k][d] + step * (q[k][d] - p[k][d]);
+        for (int d = 0; d < 3; ++d) p[k][d] = line_point<DIFF>(p[k][d], dst[(size_t)d * nV + vid[k]], step);
     }
     T pt[8][3];
     for (int k = 0; k < 8; ++k)
@@ -805,7 +982,8 @@ __global__ void energy_kernel(RefOp ro, const double* cur, const double* dst, do
 }
 
 // compute_soft_const_energy (slim_m.cpp:735-790) at X = cur + step (dst - cur)
-__global__ void soft_energy_kernel(SoftDev sd, const double* cur, const double* dst, double step, int nV,
+template <bool DIFF, class DT>
+__global__ void soft_energy_kernel(SoftDev sd, const double* cur, const DT* dst, double step, int nV,
                                    const double* aL, double* part) {
   __shared__ double red[16];
   const int tot = sd.nb + sd.nC + sd.nT + sd.nL;
@@ -820,7 +998,7 @@ __global__ void soft_energy_kernel(SoftDev sd, const double* cur, const double* 
     double x[3];
     for (int d = 0; d < 3; ++d) {
       double a = cur[(size_t)d * nV + v];
-      x[d] = dst ? a + step * (dst[(size_t)d * nV + v] - a) : a;
+      x[d] = dst ? line_point<DIFF>(a, dst[(size_t)d * nV + v], step) : a;
     }
     if (kind == 0) {
       double q0 = sd.b_pos[j * 3] - x[0], q1 = sd.b_pos[j * 3 + 1] - x[1], q2 = sd.b_pos[j * 3 + 2] - x[2];
@@ -852,12 +1030,11 @@ __global__ void energy_fin_kernel(const double* pq, int nq, const double* ps, in
   if (threadIdx.x == 0) { out2[0] = a; out2[1] = b; }
 }
 
-// cur += step (dst - cur)
-__global__ void step_kernel(int n, double* cur, const double* dst, double step) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double a = cur[i];
-    cur[i] = a + step * (dst[i] - a);
-  }
+// cur += step (dst - cur)  (or cur += step d)
+template <bool DIFF, class DT>
+__global__ void step_kernel(int n, double* cur, const DT* dst, double step) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    cur[i] = line_point<DIFF>(cur[i], dst[i], step);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -867,27 +1044,29 @@ template <class T, class P>
 struct GradOp {
   RefOp ro; int nV, nH; const int* conn; const P* X; P* G; double M;
   SoftDev sd; const double* aL; double* epart_unused;
-  __device__ void hex(int h, T o[8][3]) const {
+  __device__ double hex(int h, T o[8][3]) const {
     int vid[8];
     double pd[8][3];
     gather8(X, nV, conn, nH, h, vid, pd);
     T p[8][3];
     for (int k = 0; k < 8; ++k)
       for (int d = 0; d < 3; ++d) p[k][d] = (T)pd[k][d];
+    double e = 0.0;
 #pragma unroll 1
     for (int c = 0; c < 8; ++c) {
       T B[3][3], E[3][3], J[3][3], Pm[3][3];
       tet_B(ro, nH, h, c, B);
       tet_edges(p, c, E);
       matmul3(E, B, J);
-      sd_energy_grad(J, Pm);
+      e += M * (double)sd_energy_grad(J, Pm);
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) Pm[i][j] *= (T)M;
       scatter_tet(Pm, B, c, o);
     }
+    return e;
   }
   // soft gradient 2 (S x - b - lamL a t) for constrained vertices
-  __device__ T finish(int g, T acc[3]) const {
+  __device__ double finish(int g, T acc[3]) const {
     double x[3] = {(double)X[g], (double)X[(size_t)nV + g], (double)X[2 * (size_t)nV + g]};
     double r[3] = {(double)acc[0], (double)acc[1], (double)acc[2]};
     const int ci = sd.cidx ? sd.cidx[g] : -1;
@@ -902,14 +1081,14 @@ struct GradOp {
       for (int d = 0; d < 3; ++d) r[d] += 2.0 * sx[d];
     }
     for (int d = 0; d < 3; ++d) G[(size_t)d * nV + g] = (P)r[d];
-    return T(0);
+    return 0.0;
   }
 };
 
 template <class T, class P>
 struct HvpOp {
   RefOp ro; int nV, nH; const int* conn; const P* X; const P* Vd; P* Hv; double M; SoftDev sd;
-  __device__ void hex(int h, T o[8][3]) const {
+  __device__ double hex(int h, T o[8][3]) const {
     int vid[8];
     double pd[8][3], vd[8][3];
     gather8(X, nV, conn, nH, h, vid, pd);
@@ -930,8 +1109,9 @@ struct HvpOp {
         for (int j = 0; j < 3; ++j) Hd[i][j] *= (T)M;
       scatter_tet(Hd, B, c, o);
     }
+    return 0.0;
   }
-  __device__ T finish(int g, T acc[3]) const {
+  __device__ double finish(int g, T acc[3]) const {
     double vv[3] = {(double)Vd[g], (double)Vd[(size_t)nV + g], (double)Vd[2 * (size_t)nV + g]};
     double r[3] = {(double)acc[0], (double)acc[1], (double)acc[2]};
     const int ci = sd.cidx ? sd.cidx[g] : -1;
@@ -942,7 +1122,7 @@ struct HvpOp {
       r[2] += 2.0 * (S[4] * vv[0] + S[5] * vv[1] + S[2] * vv[2]);
     }
     for (int d = 0; d < 3; ++d) Hv[(size_t)d * nV + g] = (P)r[d];
-    return T(0);
+    return 0.0;
   }
 };
 
@@ -1002,25 +1182,61 @@ void launch_local_step(const TileMesh& tm, const RefOp& ro, const double* X, dou
   launch_scatter<double, 6>(tm, op, halo, nullptr, nullptr, st);
 }
 
+void launch_adj_local32(const TileMesh& tm, const RefOp& ro, const double* X, float* K, double M, double prox,
+                        const SoftDev& sd, float* rhs, float* invd, float* halo, double* hex_part, int* n_inverted,
+                        hipStream_t st) {
+  AdjLocalOp32 op{ro, tm.nV, tm.nH, tm.conn, X, K, M, prox, sd, rhs, invd, n_inverted};
+  launch_scatter<float, 6>(tm, op, halo, nullptr, nullptr, st, hex_part);
+  if (sd.nL > 0)
+    hipLaunchKernelGGL(line_rhs32_kernel, dim3((sd.nL + 255) / 256), dim3(256), 0, st, sd, tm.nV, X,
+                       rhs + 3 * (size_t)tm.nV, invd + 3 * (size_t)tm.nV);
+}
+
+template <class KT, class PT, class CT, class OT>
+void launch_apply_t(const TileMesh& tm, const RefOp& ro, const KT* K, const PT* p, OT* out, double prox,
+                    const SoftDev& sd, CT* halo, double* part_tile, double* part_fix, double* part_line, hipStream_t st) {
+  const PT* pa = p + 3 * (size_t)tm.nV;
+  ApplyOp<KT, PT, CT, OT> op{ro, tm.nV, tm.nH, tm.conn, K, p, out, prox, sd, pa, sd.nL};
+  launch_scatter<CT, 3>(tm, op, halo, part_tile, part_fix, st);
+  if (sd.nL > 0)
+    hipLaunchKernelGGL((line_apply_kernel<PT, OT>), dim3((sd.nL + 255) / 256), dim3(256), 0, st, sd, tm.nV, p, pa,
+                       out + 3 * (size_t)tm.nV, sd.nL, part_line);
+}
+
 void launch_apply(const TileMesh& tm, const RefOp& ro, const double* K, const double* p, double* out, double prox,
                   const SoftDev& sd, double* halo, double* part_tile, double* part_fix, double* part_line,
                   hipStream_t st) {
-  const double* pa = p + 3 * (size_t)tm.nV;
-  ApplyOp<double> op{ro, tm.nV, tm.nH, tm.conn, K, p, out, prox, sd, pa, sd.nL, nullptr};
-  launch_scatter<double, 3>(tm, op, halo, part_tile, part_fix, st);
-  if (sd.nL > 0)
-    hipLaunchKernelGGL(line_apply_kernel, dim3((sd.nL + 255) / 256), dim3(256), 0, st, sd, tm.nV, p, pa,
-                       out + 3 * (size_t)tm.nV, sd.nL, part_line);
+  launch_apply_t<double, double, double, double>(tm, ro, K, p, out, prox, sd, halo, part_tile, part_fix, part_line, st);
+}
+
+void launch_apply32(const TileMesh& tm, const RefOp& ro, const float* K, const float* p, float* out, double prox,
+                    const SoftDev& sd, float* halo, double* part_tile, double* part_fix, double* part_line,
+                    hipStream_t st) {
+  launch_apply_t<float, float, float, float>(tm, ro, K, p, out, prox, sd, halo, part_tile, part_fix, part_line, st);
+}
+
+void launch_apply32_f64(const TileMesh& tm, const RefOp& ro, const float* K, const float* p, double* out, double prox,
+                        const SoftDev& sd, double* halo, hipStream_t st) {
+  launch_apply_t<float, float, double, double>(tm, ro, K, p, out, prox, sd, halo, nullptr, nullptr, nullptr, st);
+}
+
+template <class T>
+void cg_init_t(int n, const T* rhs, const T* tmp, T* r, double* pb, double* pr, int nb, void* state, const T* invd, T* p,
+               double* pabs, double tol, hipStream_t st) {
+  CGState* S = (CGState*)state;
+  hipLaunchKernelGGL(cg_init_kernel<T>, dim3(nb), dim3(256), 0, st, n, rhs, tmp, r, pb, pr);
+  hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)pb, nb, (const double*)pr, tol);
+  hipLaunchKernelGGL(cg_first_dir_kernel<T>, dim3(nb), dim3(256), 0, st, n, (const CGState*)S, invd, (const T*)r, p, pabs);
+  hipLaunchKernelGGL(cg_absnew_fin_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)pabs, nb);
 }
 
 void launch_cg_init(int n, const double* rhs, const double* tmp, double* r, double* pb, double* pr, int nb,
                     void* state, const double* invd, double* p, double* pabs, double tol, hipStream_t st) {
-  CGState* S = (CGState*)state;
-  hipLaunchKernelGGL(cg_init_kernel, dim3(nb), dim3(256), 0, st, n, rhs, tmp, r, pb, pr);
-  hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)pb, nb, (const double*)pr, tol);
-  hipLaunchKernelGGL(cg_first_dir_kernel, dim3(nb), dim3(256), 0, st, n, (const CGState*)S, invd, (const double*)r, p,
-                     pabs);
-  hipLaunchKernelGGL(cg_absnew_fin_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)pabs, nb);
+  cg_init_t<double>(n, rhs, tmp, r, pb, pr, nb, state, invd, p, pabs, tol, st);
+}
+void launch_cg_init32(int n, const float* rhs, const float* tmp, float* r, double* pb, double* pr, int nb, void* state,
+                      const float* invd, float* p, double* pabs, double tol, hipStream_t st) {
+  cg_init_t<float>(n, rhs, tmp, r, pb, pr, nb, state, invd, p, pabs, tol, st);
 }
 
 void launch_cg_alpha(void* state, const double* p1, int n1, const double* p2, int n2, const double* p3, int n3,
@@ -1028,16 +1244,44 @@ void launch_cg_alpha(void* state, const double* p1, int n1, const double* p2, in
   hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(1024), 0, st, (CGState*)state, p1, n1, p2, n2, p3, n3);
 }
 
+template <class T>
+void cg_update_t(int n, void* state, T* x, T* r, T* z, const T* p, const T* tmp, const T* invd, double* prr,
+                 double* prz, int nb, T* p_out, hipStream_t st) {
+  CGState* S = (CGState*)state;
+  hipLaunchKernelGGL(cg_update_kernel<T>, dim3(nb), dim3(256), 0, st, n, (const CGState*)S, x, r, z, p, tmp, invd, prr,
+                     prz);
+  hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)prr, (const double*)prz, nb);
+  hipLaunchKernelGGL(cg_pdir_kernel<T>, dim3(nb), dim3(256), 0, st, n, (const CGState*)S, p_out, (const T*)z);
+}
+
 void launch_cg_update(int n, void* state, double* x, double* r, double* z, const double* p, const double* tmp,
                       const double* invd, double* prr, double* prz, int nb, double* p_out, hipStream_t st) {
-  CGState* S = (CGState*)state;
-  hipLaunchKernelGGL(cg_update_kernel, dim3(nb), dim3(256), 0, st, n, (const CGState*)S, x, r, z, p, tmp, invd, prr, prz);
-  hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)prr, (const double*)prz, nb);
-  hipLaunchKernelGGL(cg_pdir_kernel, dim3(nb), dim3(256), 0, st, n, (const CGState*)S, p_out, (const double*)z);
+  cg_update_t<double>(n, state, x, r, z, p, tmp, invd, prr, prz, nb, p_out, st);
+}
+void launch_cg_update32(int n, void* state, float* x, float* r, float* z, const float* p, const float* tmp,
+                        const float* invd, double* prr, double* prz, int nb, float* p_out, hipStream_t st) {
+  cg_update_t<float>(n, state, x, r, z, p, tmp, invd, prr, prz, nb, p_out, st);
 }
 
 void launch_cg_zero_if(int n, void* state, double* x, int nb, hipStream_t st) {
-  hipLaunchKernelGGL(cg_zero_if_kernel, dim3(nb), dim3(256), 0, st, n, (const CGState*)state, x);
+  hipLaunchKernelGGL(cg_zero_if_kernel<double>, dim3(nb), dim3(256), 0, st, n, (const CGState*)state, x);
+}
+void launch_cg_zero_if32(int n, void* state, float* x, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(cg_zero_if_kernel<float>, dim3(nb), dim3(256), 0, st, n, (const CGState*)state, x);
+}
+
+void launch_residual64(int n, const float* b, const double* Ld, float* r32, double* part, int nb, double* out,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(residual64_kernel, dim3(nb), dim3(256), 0, st, n, b, Ld, r32, part);
+  hipLaunchKernelGGL(sum_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part, nb, out);
+}
+
+void launch_add_inplace(int n, float* d, const float* e, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(add_inplace_kernel, dim3(nb), dim3(256), 0, st, n, d, e);
+}
+
+void launch_sum_partials(const double* part, int n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(sum_fin_kernel, dim3(1), dim3(1024), 0, st, part, n, out);
 }
 
 size_t cg_state_bytes() { return sizeof(CGState); }
@@ -1053,7 +1297,15 @@ void launch_max_step(const double* cur, const double* dst, int nV, const int* co
                      double* out, hipStream_t st) {
   const int nb = (nH + 255) / 256;
   if (nb == 0) return;
-  hipLaunchKernelGGL(max_step_kernel, dim3(nb), dim3(256), 0, st, cur, dst, nV, conn, nH, part);
+  hipLaunchKernelGGL((max_step_kernel<true, double>), dim3(nb), dim3(256), 0, st, cur, dst, nV, conn, nH, part);
+  hipLaunchKernelGGL(min_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part, nb, out);
+}
+
+void launch_max_step_dir32(const double* cur, const float* d, int nV, const int* conn, int nH, double* part,
+                           double* out, hipStream_t st) {
+  const int nb = (nH + 255) / 256;
+  if (nb == 0) return;
+  hipLaunchKernelGGL((max_step_kernel<false, float>), dim3(nb), dim3(256), 0, st, cur, d, nV, conn, nH, part);
   hipLaunchKernelGGL(min_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part, nb, out);
 }
 
@@ -1062,29 +1314,48 @@ void launch_energy(const RefOp& ro, const double* cur, const double* dst, double
                    double* part_s, int ns_blocks, double* out2, hipStream_t st) {
   const int nb = (nH + 255) / 256;
   if (nb > 0)
-    hipLaunchKernelGGL(energy_kernel<double>, dim3(nb), dim3(256), 0, st, ro, cur, dst, step, nV, conn, nH, energy, M,
-                       exp_f, part_q);
-  hipLaunchKernelGGL(soft_energy_kernel, dim3(ns_blocks), dim3(256), 0, st, sd, cur, dst, step, nV, aL, part_s);
+    hipLaunchKernelGGL((energy_kernel<double, true, double>), dim3(nb), dim3(256), 0, st, ro, cur, dst, step, nV, conn,
+                       nH, energy, M, exp_f, part_q);
+  hipLaunchKernelGGL((soft_energy_kernel<true, double>), dim3(ns_blocks), dim3(256), 0, st, sd, cur, dst, step, nV, aL,
+                     part_s);
+  hipLaunchKernelGGL(energy_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part_q, nb, (const double*)part_s,
+                     ns_blocks, out2);
+}
+
+void launch_energy_dir32(const RefOp& ro, const double* cur, const float* d, double step, int nV, const int* conn,
+                         int nH, double M, const SoftDev& sd, const double* aL, double* part_q, double* part_s,
+                         int ns_blocks, double* out2, hipStream_t st) {
+  const int nb = (nH + 255) / 256;
+  if (nb > 0)
+    hipLaunchKernelGGL((energy_kernel<float, false, float>), dim3(nb), dim3(256), 0, st, ro, cur, d, step, nV, conn, nH,
+                       (int)E_SYMMETRIC_DIRICHLET, M, 1.0, part_q);
+  hipLaunchKernelGGL((soft_energy_kernel<false, float>), dim3(ns_blocks), dim3(256), 0, st, sd, cur, d, step, nV, aL,
+                     part_s);
   hipLaunchKernelGGL(energy_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part_q, nb, (const double*)part_s,
                      ns_blocks, out2);
 }
 
 void launch_step(int n, double* cur, const double* dst, double step, hipStream_t st) {
-  hipLaunchKernelGGL(step_kernel, dim3((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096), dim3(256), 0, st, n, cur, dst, step);
+  hipLaunchKernelGGL((step_kernel<true, double>), dim3((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096), dim3(256), 0,
+                     st, n, cur, dst, step);
+}
+void launch_step_dir32(int n, double* cur, const float* d, double step, hipStream_t st) {
+  hipLaunchKernelGGL((step_kernel<false, float>), dim3((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096), dim3(256), 0,
+                     st, n, cur, d, step);
 }
 
 template <class T, class P>
 void launch_grad_t(const TileMesh& tm, const RefOp& ro, const P* X, P* G, double M, const SoftDev& sd, const double* aL,
                    T* halo, hipStream_t st) {
   GradOp<T, P> op{ro, tm.nV, tm.nH, tm.conn, X, G, M, sd, aL, nullptr};
-  launch_scatter<T, 3>(tm, op, halo, (T*)nullptr, (T*)nullptr, st);
+  launch_scatter<T, 3>(tm, op, halo, nullptr, nullptr, st);
 }
 
 template <class T, class P>
 void launch_hvp_t(const TileMesh& tm, const RefOp& ro, const P* X, const P* Vd, P* Hv, double M, const SoftDev& sd,
                   T* halo, hipStream_t st) {
   HvpOp<T, P> op{ro, tm.nV, tm.nH, tm.conn, X, Vd, Hv, M, sd};
-  launch_scatter<T, 3>(tm, op, halo, (T*)nullptr, (T*)nullptr, st);
+  launch_scatter<T, 3>(tm, op, halo, nullptr, nullptr, st);
 }
 
 void launch_grad(int prec, const TileMesh& tm, const RefOp& ro, const void* X, void* G, double M, const SoftDev& sd,

@@ -16,7 +16,8 @@ REST, CURRENT = 0, 1
 EXPORTS = ["hexopt_create", "hexopt_create_error", "hexopt_destroy", "hexopt_last_error", "hexopt_set_positions",
            "hexopt_get_positions", "hexopt_device_positions", "hexopt_reference", "hexopt_set_constraints",
            "hexopt_slim_solve", "hexopt_energy", "hexopt_energy_grad", "hexopt_hvp", "hexopt_scaled_jacobian",
-           "hexopt_synchronize", "hexopt_layout_info", "hexopt_debug_system"]
+           "hexopt_synchronize", "hexopt_layout_info", "hexopt_debug_system", "hexopt_opt32_solve",
+           "hexopt_time_kernel", "hexopt_copy_positions"]
 
 _LIB = None
 
@@ -40,6 +41,16 @@ class IterLog(C.Structure):
                 ("step_bound", C.c_double), ("step", C.c_double), ("cg_iters", C.c_int), ("cg_error", C.c_double)]
 
 
+class Opt32Options(C.Structure):
+    _fields_ = [("cg_iters", C.c_int), ("refine_iters", C.c_int)]
+
+
+class Opt32Log(C.Structure):
+    _fields_ = [("energy", C.c_double), ("e_quality", C.c_double), ("e_soft", C.c_double),
+                ("step_bound", C.c_double), ("step", C.c_double), ("cg_iters", C.c_int), ("cg_error", C.c_double),
+                ("residual64", C.c_double), ("inverted_tets", C.c_int)]
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -56,7 +67,8 @@ def lib():
         L.hexopt_device_positions.argtypes = [C.c_void_p, C.c_int]
         for name in ["hexopt_set_positions", "hexopt_get_positions", "hexopt_reference", "hexopt_set_constraints",
                      "hexopt_slim_solve", "hexopt_energy", "hexopt_energy_grad", "hexopt_hvp",
-                     "hexopt_scaled_jacobian", "hexopt_synchronize", "hexopt_layout_info", "hexopt_debug_system"]:
+                     "hexopt_scaled_jacobian", "hexopt_synchronize", "hexopt_layout_info", "hexopt_debug_system",
+                     "hexopt_opt32_solve", "hexopt_time_kernel", "hexopt_copy_positions"]:
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -110,6 +122,9 @@ class HexOpt:
         out = np.zeros((self.nV, 3))
         self._chk(lib().hexopt_get_positions(C.c_void_p(self.h), which, _ptr(out)), "get_positions")
         return out
+
+    def rest_from_current(self):
+        self._chk(lib().hexopt_copy_positions(C.c_void_p(self.h), CURRENT, REST), "copy_positions")
 
     def reference(self, hflag=None, cube=True):
         hf = None if hflag is None else np.ascontiguousarray(hflag, dtype=np.uint8)
@@ -193,6 +208,24 @@ class HexOpt:
         self._chk(lib().hexopt_debug_system(C.c_void_p(self.h), C.byref(prm), _ptr(p), _ptr(rhs), _ptr(invd),
                                             _ptr(Lp)), "debug_system")
         return rhs, invd, Lp
+
+    def opt32_solve(self, iters, cg_iters=20, refine_iters=0, weight_opt=1.0):
+        """FP32 + adjoint optimiser iterations from CURRENT (include/hexopt.h)."""
+        prm = self.params(SYMMETRIC_DIRICHLET, weight_opt)
+        opt = Opt32Options(cg_iters, refine_iters)
+        logs = (Opt32Log * max(iters, 1))()
+        E = C.c_double()
+        self._chk(lib().hexopt_opt32_solve(C.c_void_p(self.h), C.byref(prm), C.byref(opt), iters, logs, C.byref(E)),
+                  "opt32_solve")
+        cols = ["energy", "e_quality", "e_soft", "step_bound", "step", "cg_iters", "cg_error", "residual64",
+                "inverted_tets"]
+        log = np.array([[getattr(l, k) for k in cols] for l in logs[:iters]]).reshape(iters, len(cols))
+        return E.value, log
+
+    def time_kernel(self, which, reps=20):
+        ms = C.c_double()
+        self._chk(lib().hexopt_time_kernel(C.c_void_p(self.h), which, reps, C.byref(ms)), "time_kernel")
+        return ms.value
 
     def synchronize(self):
         self._chk(lib().hexopt_synchronize(C.c_void_p(self.h)), "synchronize")

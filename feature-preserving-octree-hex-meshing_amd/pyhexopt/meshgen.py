@@ -169,3 +169,60 @@ def boundary_vertices(H, nV):
     mask = np.zeros(nV, dtype=bool)
     mask[bq.ravel()] = True
     return mask
+
+
+def clipped_torus_inside(P, R=1.0, r=0.35, zcut=0.2):
+    rho = np.sqrt(P[:, 0] ** 2 + P[:, 1] ** 2)
+    return ((rho - R) ** 2 + P[:, 2] ** 2 <= r * r) & (np.abs(P[:, 2]) <= zcut)
+
+
+def clipped_torus_closest(P, R=1.0, r=0.35, zcut=0.2):
+    """Closest point and outward unit normal on the surface of the torus (R, r) cut by the
+    planes z = +-zcut: the tube band, the two flat annuli and their four rim circles."""
+    rho = np.maximum(np.sqrt(P[:, 0] ** 2 + P[:, 1] ** 2), 1e-300)
+    ex, ey = P[:, 0] / rho, P[:, 1] / rho
+    q = np.stack([rho - R, P[:, 2]], axis=1)
+    qn = np.maximum(np.linalg.norm(q, axis=1), 1e-300)
+    cands = []
+    # tube band
+    tq = q / qn[:, None] * r
+    ok = np.abs(tq[:, 1]) <= zcut
+    cands.append((tq, np.where(ok, np.abs(qn - r), np.inf), q / qn[:, None]))
+    # annuli z = +-zcut, radial extent |rho - R| <= sqrt(r^2 - zcut^2)
+    w = np.sqrt(r * r - zcut * zcut)
+    for s in (1.0, -1.0):
+        rr = np.clip(q[:, 0], -w, w)
+        c = np.stack([rr, np.full_like(rr, s * zcut)], axis=1)
+        d = np.linalg.norm(q - c, axis=1)
+        n = np.tile([0.0, s], (q.shape[0], 1))
+        inside_ann = np.abs(q[:, 0]) <= w
+        d = np.where(inside_ann, np.abs(q[:, 1] - s * zcut), d)
+        # off the annulus the nearest point is a rim circle; its normal is the direction to p
+        rim = ~inside_ann
+        dn = q - c
+        dn = dn / np.maximum(np.linalg.norm(dn, axis=1), 1e-300)[:, None]
+        n = np.where(rim[:, None], dn, n)
+        cands.append((c, d, n))
+    best = np.argmin(np.stack([cd[1] for cd in cands], axis=1), axis=1)
+    C2 = np.zeros_like(q)
+    N2 = np.zeros_like(q)
+    for k, (c, d, n) in enumerate(cands):
+        m = best == k
+        C2[m] = c[m]
+        N2[m] = n[m]
+    C = np.stack([(R + C2[:, 0]) * ex, (R + C2[:, 0]) * ey, C2[:, 1]], axis=1)
+    N = np.stack([N2[:, 0] * ex, N2[:, 0] * ey, N2[:, 1]], axis=1)
+    N /= np.maximum(np.linalg.norm(N, axis=1), 1e-300)[:, None]
+    return C, N
+
+
+def torus_hex_mesh(target_hexes=1_000_000, R=1.0, r=0.35, zcut=0.2):
+    """Uniform-level octree hex mesh of the clipped torus: cells of a cubic grid whose centre
+    lies inside (cell size chosen for ~target_hexes). Returns V, H, boundary vertex ids."""
+    vol = 2 * np.pi * R * (np.pi * r * r) * 0.684   # clipped-torus volume fraction, calibrated
+    h = (vol / target_hexes) ** (1.0 / 3.0)
+    ext = R + r + h
+    n = int(np.ceil(2 * ext / h))
+    V, H = voxel_hex_mesh(lambda P: clipped_torus_inside(P, R, r, zcut), -ext, -ext + n * h, n)
+    bnd = np.nonzero(boundary_vertices(H, V.shape[0]))[0].astype(np.int32)
+    return V, H, bnd

@@ -74,6 +74,9 @@ const char* hexopt_last_error(hexopt_ctx* ctx);
 
 int hexopt_set_positions(hexopt_ctx* ctx, int which, const double* V);
 int hexopt_get_positions(hexopt_ctx* ctx, int which, double* V);
+/* Device-side copy between the two position buffers (ts.V = ts.UV between slim_m_opt calls,
+ * grid_hex_meshing.cpp:2956). */
+int hexopt_copy_positions(hexopt_ctx* ctx, int from, int to);
 /* Device-resident positions in the library's own layout (SoA x[d*nV+v], Morton numbering);
  * for callers that keep the iterate in HBM. */
 double* hexopt_device_positions(hexopt_ctx* ctx, int which);
@@ -92,6 +95,35 @@ int hexopt_set_constraints(hexopt_ctx* ctx, const hexopt_constraints* cs);
  * (optimization::energy, engery_quality, engery_soft). aL: nullable, nL entries (a_L). */
 int hexopt_slim_solve(hexopt_ctx* ctx, const hexopt_params* prm, int iters, hexopt_iter_log* log, double* E,
                       double* Eq, double* Es, double* aL);
+
+/* FP32 + adjoint optimiser (no single counterpart in the reference; it performs the same
+ * SLIM iteration as hexopt_slim_solve in its adjoint form). Per iteration: one FP32 pass
+ * over the elements gives the energy, -dE/dx / 2 (the hand-written adjoint), the SLIM
+ * weights K and the Jacobi diagonal; the direction solves L d = -dE/dx / 2 (equal to the
+ * SLIM global step taken from CURRENT) by FP32 CG with FP64 dot products, whose residual is
+ * recomputed in FP64 and corrected by a second FP32 solve; the step bound (FP64 cubic), the
+ * corner-determinant sign test and the energy sums are FP64; the line search is the
+ * reference's bisection. Symmetric Dirichlet only. */
+typedef struct {
+  int cg_iters;      /* FP32 CG iterations of the direction solve (the reference uses 20) */
+  int refine_iters;  /* FP32 CG iterations of the FP64-residual correction (0: none) */
+} hexopt_opt32_options;
+
+typedef struct {
+  double energy, e_quality, e_soft, step_bound, step;
+  int cg_iters;
+  double cg_error;     /* relative residual estimate of the FP32 solve */
+  double residual64;   /* |b - L d| computed in FP64 before the correction */
+  int inverted_tets;   /* corner tets with negative determinant at the iterate (FP64 test) */
+} hexopt_opt32_log;
+
+int hexopt_opt32_solve(hexopt_ctx* ctx, const hexopt_params* prm, const hexopt_opt32_options* opt, int iters,
+                       hexopt_opt32_log* log, double* E);
+
+/* Average duration (ms, HIP events on the context's stream) of one launch of a kernel:
+ * 0 FP64 CG operator tile pass, 1 FP32 CG operator tile pass, 2 FP64 local step tile pass,
+ * 3 FP32 adjoint/local tile pass. Needs a prior solve (operands resident). */
+int hexopt_time_kernel(hexopt_ctx* ctx, int which, int reps, double* ms_per_launch);
 
 /* Unnormalised SLIM energy of CURRENT: quality part and soft part (compute_energy_total,
  * slim_m.cpp:716-733, with a_L = aL or 0 when aL is NULL). */
