@@ -1,0 +1,48 @@
+// bvh.h -- closest point + pseudonormal on a triangle surface (igl::AABB +
+// igl::signed_distance_pseudonormal semantics) for the tangent-plane projection.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <vector>
+
+namespace hexopt {
+
+// Tree in igl::AABB's shape, flattened: node i has box[6*i..] = (min xyz, max xyz),
+// children left/right (-1 if absent) and prim (face id at a leaf, else -1).
+struct SurfaceTree {
+  std::vector<double> box;
+  std::vector<int> left, right, prim;
+  int depth = 0;
+};
+
+bool build_surface_tree(int nv, const double* V, int nf, const int* F, SurfaceTree& T, const char** err);
+void surface_normals(int nv, const double* V, int nf, const int* F, std::vector<double>& FN, std::vector<double>& VN,
+                     std::vector<double>& FEN);
+
+struct SurfaceDev {
+  int nv, nf, nnodes, depth;
+  const double* V;    // nv*3 row-major
+  const int* F;       // nf*3
+  const double* box;  // nnodes*6
+  const int* left;
+  const int* right;
+  const int* prim;
+  const double* FN;   // nf*3
+  const double* VN;   // nv*3
+  const double* FEN;  // nf*9
+};
+
+constexpr int kBvhBlock = 128;
+constexpr int kBvhMaxDepth = 48;
+
+// Queries: P (SoA x[d*stride + i] when soa, else nq*3 rows), optional index list `ids`
+// (query i uses point ids[i]). Outputs (nullable): S, I, C (nq*3), N (nq*3), sqrD. status:
+// set to 1 if any traversal exceeded the stack depth (never for trees of depth <= 48).
+void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int stride, const int* ids, double* S, int* I,
+                    double* C, double* N, double* sqrD, int* status, hipStream_t st);
+
+struct SoftDev;
+void launch_refold_planes(const SoftDev& sd, int nT, const double* Cp, double* t_d, const double* S0, const double* b0,
+                          double* S, double* b, double lam, hipStream_t st);
+
+}  // namespace hexopt

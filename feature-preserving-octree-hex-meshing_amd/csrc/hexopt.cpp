@@ -18,6 +18,8 @@
 #include "../../include/hexopt.h"
 #include "kernels.h"
 #include "layout.h"
+#include "bvh.h"
+#include "occupancy.h"
 
 namespace hexopt {
 namespace {
@@ -67,6 +69,9 @@ struct hexopt_ctx {
   std::vector<double> h_lt;  // nL*3 (caller copy for rhs/diag of the a_L rows)
   // adjoint scratch
   DevBuf d_X, d_G, d_v;
+  // tangent-plane refresh: folded rows without the plane terms
+  DevBuf d_S0, d_b0;
+  bool planes_refreshable = false;
   // FP32 optimiser state
   DevBuf f_K, f_rhs, f_invd, f_d, f_r, f_z, f_p, f_tmp, f_e, f_halo, d_Ld64, d_halo64b, d_ninv;
 };
@@ -308,6 +313,14 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
     int r = row(L.vinv[cs->ids_C[i]]);
     for (int d = 0; d < 3; ++d) { S[r * 6 + d] += cs->lamda_C; b[r * 3 + d] += cs->lamda_C * cs->C[i * 3 + d]; }
   }
+  // rows of the plane vertices exist before the plane terms are added, so S0/b0 (the rows
+  // without the planes) can be kept for the on-device plane refresh
+  for (int i = 0; i < cs->nT; ++i) row(L.vinv[cs->ids_T[i]]);
+  for (int i = 0; i < cs->nL; ++i) row(L.vinv[cs->ids_L[i]]);
+  std::vector<double> S0_pre = S, b0_pre = b;
+  std::vector<int> tcount(nV, 0);
+  bool dupT = false;
+  for (int i = 0; i < cs->nT; ++i) dupT |= (tcount[L.vinv[cs->ids_T[i]]]++ > 0);
   for (int i = 0; i < cs->nT; ++i) {
     int r = row(L.vinv[cs->ids_T[i]]);
     const double* n = &cs->normal_T[i * 3];
@@ -328,6 +341,8 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
     for (int d = 0; d < 3; ++d) {
       S[r * 6 + d] += cs->lamda_L;
       b[r * 3 + d] += cs->lamda_L * cs->origin_L[i * 3 + d];
+      S0_pre[r * 6 + d] += cs->lamda_L;
+      b0_pre[r * 3 + d] += cs->lamda_L * cs->origin_L[i * 3 + d];
       s += -cs->Axa_L[i * 3 + d] * cs->origin_L[i * 3 + d];
       tt += cs->Axa_L[i * 3 + d] * cs->Axa_L[i * 3 + d];
     }
@@ -345,6 +360,9 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
   c->nL = cs->nL;
   int rc = HEXOPT_OK;
   rc |= upload(c, c->d_cidx, cidx);
+  rc |= upload(c, c->d_S0, S0_pre);
+  rc |= upload(c, c->d_b0, b0_pre);
+  c->planes_refreshable = !dupT;
   rc |= upload(c, c->d_S, S);
   rc |= upload(c, c->d_b, b);
   rc |= upload(c, c->d_cline, cline);
@@ -629,6 +647,13 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
     HIPCHK(c, hipMemcpyAsync(h3, scal + 6, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
     double res64 = h3[0];
+    // |b| of the direction system from the CG state's rhs norm (FP64 sum of the FP32 rhs)
+    double rhs_norm = 0;
+    {
+      std::vector<unsigned char> cgs(cg_state_bytes());
+      HIPCHK(c, hipMemcpy(cgs.data(), c->d_cg.p, cgs.size(), hipMemcpyDeviceToHost));
+      rhs_norm = std::sqrt(*(const double*)cgs.data());
+    }
     if (opt->refine_iters > 0) {
       // residual r (FP32 copy of the FP64 residual) -> correction e, d += e
       HIPCHK(c, hipMemcpyAsync(c->f_z.p, c->f_r.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
@@ -688,7 +713,7 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
     if (log) {
       hexopt_opt32_log& lg = log[it];
       lg.energy = energy; lg.e_quality = eq; lg.e_soft = es; lg.step_bound = bound; lg.step = step;
-      lg.cg_iters = cg_it; lg.cg_error = cg_err; lg.residual64 = h3[2] > 0 ? std::sqrt(res64) : std::sqrt(res64);
+      lg.cg_iters = cg_it; lg.cg_error = cg_err; lg.residual64 = rhs_norm > 0 ? std::sqrt(res64) / rhs_norm : 0.0;
       lg.inverted_tets = ninv;
     }
   }
@@ -744,6 +769,275 @@ int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch
   (void)hipEventDestroy(e1);
   *ms_per_launch = ms / reps;
   HIPCHK(c, hipGetLastError());
+  return HEXOPT_OK;
+}
+
+}  // extern "C"
+
+struct hexopt_surface {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  SurfaceTree T;
+  int nv = 0, nf = 0;
+  std::vector<double> FN, VN, FEN;
+  DevBuf dV, dF, dbox, dl, dr, dp, dFN, dVN, dFEN, dstat, dP, dS, dI, dC, dN;
+  SurfaceDev view{};
+  // occupancy: column bins (built on first use), cell and flag staging
+  bool bins_ready = false;
+  ColumnBins bins;
+  DevBuf dboff, dbfid, dbfbox, dcells, dflags;
+  ColumnBinsDev bview{};
+  std::string err;
+};
+
+namespace {
+thread_local std::string g_surface_error;
+template <class T>
+int sup(hexopt_surface* s, DevBuf& b, const T* p, size_t n) {
+  if (b.ensure(n * sizeof(T) + 8) != hipSuccess) return HEXOPT_EHIP;
+  if (n && hipMemcpyAsync(b.p, p, n * sizeof(T), hipMemcpyHostToDevice, s->stream) != hipSuccess) return HEXOPT_EHIP;
+  return HEXOPT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* hexopt_surface_error(void) { return g_surface_error.c_str(); }
+
+hexopt_surface* hexopt_surface_create(int nv, const double* V, int nf, const int32_t* F, const double* FN,
+                                      const double* VN, const double* FEN, int device) {
+  g_surface_error.clear();
+  if (nv <= 0 || nf <= 0 || !V || !F) { g_surface_error = "invalid surface"; return nullptr; }
+  if ((FN || VN || FEN) && !(FN && VN && FEN)) { g_surface_error = "give all of FN, VN, FEN or none"; return nullptr; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) { g_surface_error = "no such HIP device"; return nullptr; }
+  hexopt_surface* s = new hexopt_surface();
+  s->device = device;
+  s->nv = nv;
+  s->nf = nf;
+  const char* e = nullptr;
+  if (!build_surface_tree(nv, V, nf, F, s->T, &e)) { g_surface_error = e; delete s; return nullptr; }
+  if (s->T.depth > kBvhMaxDepth) { g_surface_error = "surface tree deeper than the traversal stack"; delete s; return nullptr; }
+  if (FN) {
+    s->FN.assign(FN, FN + (size_t)nf * 3);
+    s->VN.assign(VN, VN + (size_t)nv * 3);
+    s->FEN.assign(FEN, FEN + (size_t)nf * 9);
+  } else {
+    surface_normals(nv, V, nf, F, s->FN, s->VN, s->FEN);
+  }
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess) {
+    g_surface_error = "hipStreamCreate failed";
+    delete s;
+    return nullptr;
+  }
+  int rc = 0;
+  rc |= sup(s, s->dV, V, (size_t)nv * 3);
+  rc |= sup(s, s->dF, F, (size_t)nf * 3);
+  rc |= sup(s, s->dbox, s->T.box.data(), s->T.box.size());
+  rc |= sup(s, s->dl, s->T.left.data(), s->T.left.size());
+  rc |= sup(s, s->dr, s->T.right.data(), s->T.right.size());
+  rc |= sup(s, s->dp, s->T.prim.data(), s->T.prim.size());
+  rc |= sup(s, s->dFN, s->FN.data(), s->FN.size());
+  rc |= sup(s, s->dVN, s->VN.data(), s->VN.size());
+  rc |= sup(s, s->dFEN, s->FEN.data(), s->FEN.size());
+  if (rc || s->dstat.ensure(sizeof(int)) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) {
+    g_surface_error = "device upload failed";
+    hexopt_surface_destroy(s);
+    return nullptr;
+  }
+  s->view = SurfaceDev{nv, nf, (int)s->T.prim.size(), s->T.depth, s->dV.as<double>(), s->dF.as<int>(),
+                       s->dbox.as<double>(), s->dl.as<int>(), s->dr.as<int>(), s->dp.as<int>(), s->dFN.as<double>(),
+                       s->dVN.as<double>(), s->dFEN.as<double>()};
+  return s;
+}
+
+void hexopt_surface_destroy(hexopt_surface* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  hipStream_t st = s->stream;
+  delete s;
+  if (st) (void)hipStreamDestroy(st);
+}
+
+const char* hexopt_surface_last_error(hexopt_surface* s) { return s ? s->err.c_str() : "null surface"; }
+
+int hexopt_compute_surface_normals(int nv, const double* V, int nf, const int32_t* F, double* FN, double* VN,
+                                   double* FEN) {
+  if (nv <= 0 || nf <= 0 || !V || !F || !FN || !VN || !FEN) return HEXOPT_EINVAL;
+  for (long i = 0; i < 3L * nf; ++i)
+    if (F[i] < 0 || F[i] >= nv) return HEXOPT_EINVAL;
+  std::vector<double> a, b, c;
+  surface_normals(nv, V, nf, F, a, b, c);
+  std::memcpy(FN, a.data(), a.size() * sizeof(double));
+  std::memcpy(VN, b.data(), b.size() * sizeof(double));
+  std::memcpy(FEN, c.data(), c.size() * sizeof(double));
+  return HEXOPT_OK;
+}
+
+int hexopt_surface_normals(hexopt_surface* s, double* FN, double* VN, double* FEN) {
+  if (!s) return HEXOPT_EINVAL;
+  if (FN) std::memcpy(FN, s->FN.data(), s->FN.size() * sizeof(double));
+  if (VN) std::memcpy(VN, s->VN.data(), s->VN.size() * sizeof(double));
+  if (FEN) std::memcpy(FEN, s->FEN.data(), s->FEN.size() * sizeof(double));
+  return HEXOPT_OK;
+}
+
+int hexopt_surface_closest(hexopt_surface* s, int nq, const double* P, double* S, int32_t* I, double* C, double* N) {
+  if (!s || nq < 0 || (nq > 0 && !P)) return HEXOPT_EINVAL;
+  if (nq == 0) return HEXOPT_OK;
+  hexopt_ctx dummy;  // error plumbing for HIPCHK
+  auto fin = [&](int rc) {
+    if (rc) s->err = dummy.err;
+    dummy.stream = nullptr;
+    return rc;
+  };
+  hipStream_t st = s->stream;
+  if (s->dP.ensure((size_t)nq * 3 * 8) || s->dS.ensure((size_t)nq * 8) || s->dI.ensure((size_t)nq * 4) ||
+      s->dC.ensure((size_t)nq * 24) || s->dN.ensure((size_t)nq * 24))
+    return fin(fail(&dummy, HEXOPT_EHIP, "hipMalloc failed"));
+  if (hipMemcpyAsync(s->dP.p, P, (size_t)nq * 24, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemsetAsync(s->dstat.p, 0, sizeof(int), st) != hipSuccess)
+    return fin(fail(&dummy, HEXOPT_EHIP, "copy failed"));
+  launch_closest(s->view, nq, s->dP.as<double>(), 0, 0, nullptr, s->dS.as<double>(), s->dI.as<int>(), s->dC.as<double>(),
+                 s->dN.as<double>(), nullptr, s->dstat.as<int>(), st);
+  int stat = 0;
+  bool ok = hipGetLastError() == hipSuccess;
+  ok = ok && hipMemcpyAsync(&stat, s->dstat.p, sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess;
+  if (S) ok = ok && hipMemcpyAsync(S, s->dS.p, (size_t)nq * 8, hipMemcpyDeviceToHost, st) == hipSuccess;
+  if (I) ok = ok && hipMemcpyAsync(I, s->dI.p, (size_t)nq * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+  if (C) ok = ok && hipMemcpyAsync(C, s->dC.p, (size_t)nq * 24, hipMemcpyDeviceToHost, st) == hipSuccess;
+  if (N) ok = ok && hipMemcpyAsync(N, s->dN.p, (size_t)nq * 24, hipMemcpyDeviceToHost, st) == hipSuccess;
+  ok = ok && hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) return fin(fail(&dummy, HEXOPT_EHIP, "closest-point kernel failed"));
+  if (stat) return fin(fail(&dummy, HEXOPT_EUNSUPPORTED, "traversal stack overflow"));
+  return fin(HEXOPT_OK);
+}
+
+int hexopt_project_planes(hexopt_ctx* c, hexopt_surface* s, double lamda_T) {
+  if (!c || !s) return HEXOPT_EINVAL;
+  if (s->device != c->device) return fail(c, HEXOPT_EINVAL, "surface and context on different devices");
+  if (!c->planes_refreshable) return fail(c, HEXOPT_EUNSUPPORTED, "duplicate plane vertices");
+  const int nT = c->sd.nT;
+  if (nT == 0) return HEXOPT_OK;
+  hipStream_t st = c->stream;
+  HIPCHK(c, s->dC.ensure((size_t)nT * 24));
+  HIPCHK(c, hipMemsetAsync(s->dstat.p, 0, sizeof(int), st));
+  // closest point C and pseudonormal N of every plane vertex at CURRENT (projection_smooth,
+  // grid_hex_meshing.cpp:3760-3781): normal_T = N, dis_T = N . C
+  launch_closest(s->view, nT, c->d_cur.as<double>(), 1, c->L.nV, c->d_tids.as<int>(), nullptr, nullptr,
+                 s->dC.as<double>(), const_cast<double*>(c->sd.t_n), nullptr, s->dstat.as<int>(), st);
+  launch_refold_planes(c->sd, nT, s->dC.as<double>(), const_cast<double*>(c->sd.t_d), c->d_S0.as<double>(),
+                       c->d_b0.as<double>(), const_cast<double*>(c->sd.S), const_cast<double*>(c->sd.b), lamda_T, st);
+  c->sd.lamT = lamda_T;
+  int stat = 0;
+  HIPCHK(c, hipMemcpyAsync(&stat, s->dstat.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(st));
+  if (stat) return fail(c, HEXOPT_EUNSUPPORTED, "traversal stack overflow");
+  return HEXOPT_OK;
+}
+
+static int surface_bins(hexopt_surface* s) {
+  if (s->bins_ready) return HEXOPT_OK;
+  std::vector<double> V((size_t)s->nv * 3);
+  std::vector<int> F((size_t)s->nf * 3);
+  HIPCHK(s, hipMemcpyAsync(V.data(), s->dV.p, V.size() * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipMemcpyAsync(F.data(), s->dF.p, F.size() * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  build_column_bins(s->nv, V.data(), s->nf, F.data(), s->bins);
+  const ColumnBins& B = s->bins;
+  if (sup(s, s->dboff, B.off.data(), B.off.size()) || sup(s, s->dbfid, B.fid.data(), B.fid.size()) ||
+      sup(s, s->dbfbox, B.fbox.data(), B.fbox.size()))
+    return (s->err = "device upload failed", HEXOPT_EHIP);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  s->bview = ColumnBinsDev{{B.lo[0], B.lo[1]}, {B.hi[0], B.hi[1]}, {B.inv[0], B.inv[1]}, {B.nb[0], B.nb[1]},
+                           s->dboff.as<int>(), s->dbfid.as<int>(), s->dbfbox.as<double>()};
+  s->bins_ready = true;
+  return HEXOPT_OK;
+}
+
+static int upload_cells(hexopt_surface* s, int n, const int32_t* cells) {
+  for (long i = 0; i < n; ++i)
+    if (cells[4 * i + 3] <= 0) return (s->err = "cell extent must be positive", HEXOPT_EINVAL);
+  HIPCHK(s, s->dcells.ensure((size_t)n * 16));
+  HIPCHK(s, hipMemcpyAsync(s->dcells.p, cells, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+  return HEXOPT_OK;
+}
+
+int hexopt_occ_classify_cells(hexopt_surface* s, int n, const int32_t* cells, const double* origin, double voxel_size,
+                              const double* mesh_transform, int stop_extent, uint8_t* split) {
+  if (!s || n < 0 || (n > 0 && (!cells || !split)) || !origin) return HEXOPT_EINVAL;
+  if (n == 0) return HEXOPT_OK;
+  int rc = upload_cells(s, n, cells);
+  if (rc) return rc;
+  HIPCHK(s, s->dflags.ensure((size_t)n));
+  double base[3];
+  for (int d = 0; d < 3; ++d) base[d] = (mesh_transform ? mesh_transform[d] : 0.0) + origin[d];
+  launch_occ_classify(s->view, n, s->dcells.as<int>(), base, voxel_size, stop_extent, s->dflags.as<uint8_t>(), s->stream);
+  HIPCHK(s, hipGetLastError());
+  HIPCHK(s, hipMemcpyAsync(split, s->dflags.p, (size_t)n, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return HEXOPT_OK;
+}
+
+int hexopt_occ_voxel_parity(hexopt_surface* s, int nx, int ny, int nz, const double* origin, double spacing,
+                            uint8_t* inside) {
+  if (!s || nx < 0 || ny < 0 || nz < 0 || !origin) return HEXOPT_EINVAL;
+  const size_t nvox = (size_t)nx * ny * nz;
+  if (nvox == 0) return HEXOPT_OK;
+  if (!inside || ny > 65535) return HEXOPT_EINVAL;
+  int rc = surface_bins(s);
+  if (rc) return rc;
+  HIPCHK(s, s->dflags.ensure(nvox));
+  launch_occ_voxel_parity(s->view, s->bview, nx, ny, nz, origin, spacing, s->dflags.as<uint8_t>(), s->stream);
+  HIPCHK(s, hipGetLastError());
+  HIPCHK(s, hipMemcpyAsync(inside, s->dflags.p, nvox, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return HEXOPT_OK;
+}
+
+int hexopt_occ_cell_parity(hexopt_surface* s, int n, const int32_t* cells, const double* origin, double spacing,
+                           float* inside) {
+  if (!s || n < 0 || (n > 0 && (!cells || !inside)) || !origin) return HEXOPT_EINVAL;
+  if (n == 0) return HEXOPT_OK;
+  int rc = surface_bins(s);
+  if (rc) return rc;
+  rc = upload_cells(s, n, cells);
+  if (rc) return rc;
+  HIPCHK(s, s->dflags.ensure((size_t)n * 4));
+  launch_occ_cell_parity(s->view, s->bview, n, s->dcells.as<int>(), origin, spacing, s->dflags.as<float>(), s->stream);
+  HIPCHK(s, hipGetLastError());
+  HIPCHK(s, hipMemcpyAsync(inside, s->dflags.p, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return HEXOPT_OK;
+}
+
+int hexopt_hex_inside(hexopt_ctx* c, hexopt_surface* s, int which, double* signed_dis, uint8_t* H_flag) {
+  if (!c || !s || (which != HEXOPT_REST && which != HEXOPT_CURRENT)) return HEXOPT_EINVAL;
+  if (s->device != c->device) return fail(c, HEXOPT_EINVAL, "surface and context on different devices");
+  const int nH = c->L.nH;
+  if (nH == 0) return HEXOPT_OK;
+  hipStream_t st = c->stream;
+  HIPCHK(c, s->dP.ensure((size_t)nH * 24));
+  HIPCHK(c, s->dS.ensure((size_t)nH * 8));
+  HIPCHK(c, hipMemsetAsync(s->dstat.p, 0, sizeof(int), st));
+  launch_hex_centres((which == HEXOPT_REST ? c->d_rest : c->d_cur).as<double>(), c->L.nV, c->tm.conn, nH,
+                     s->dP.as<double>(), st);
+  launch_closest(s->view, nH, s->dP.as<double>(), 0, 0, nullptr, s->dS.as<double>(), nullptr, nullptr, nullptr, nullptr,
+                 s->dstat.as<int>(), st);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> S(nH);
+  int stat = 0;
+  HIPCHK(c, hipMemcpyAsync(S.data(), s->dS.p, (size_t)nH * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(&stat, s->dstat.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  if (stat) return fail(c, HEXOPT_EUNSUPPORTED, "traversal stack overflow");
+  // internal hex i is the caller's hex hperm[i]
+  for (int i = 0; i < nH; ++i) {
+    const int h = c->L.hperm[i];
+    if (signed_dis) signed_dis[h] = S[i];
+    if (H_flag) H_flag[h] = S[i] < 0 ? 1 : 0;
+  }
   return HEXOPT_OK;
 }
 

@@ -17,7 +17,10 @@ EXPORTS = ["hexopt_create", "hexopt_create_error", "hexopt_destroy", "hexopt_las
            "hexopt_get_positions", "hexopt_device_positions", "hexopt_reference", "hexopt_set_constraints",
            "hexopt_slim_solve", "hexopt_energy", "hexopt_energy_grad", "hexopt_hvp", "hexopt_scaled_jacobian",
            "hexopt_synchronize", "hexopt_layout_info", "hexopt_debug_system", "hexopt_opt32_solve",
-           "hexopt_time_kernel", "hexopt_copy_positions"]
+           "hexopt_time_kernel", "hexopt_copy_positions", "hexopt_surface_create", "hexopt_surface_error",
+           "hexopt_surface_last_error", "hexopt_surface_destroy", "hexopt_surface_normals",
+           "hexopt_compute_surface_normals", "hexopt_surface_closest", "hexopt_project_planes",
+           "hexopt_occ_classify_cells", "hexopt_occ_voxel_parity", "hexopt_occ_cell_parity", "hexopt_hex_inside"]
 
 _LIB = None
 
@@ -68,8 +71,18 @@ def lib():
         for name in ["hexopt_set_positions", "hexopt_get_positions", "hexopt_reference", "hexopt_set_constraints",
                      "hexopt_slim_solve", "hexopt_energy", "hexopt_energy_grad", "hexopt_hvp",
                      "hexopt_scaled_jacobian", "hexopt_synchronize", "hexopt_layout_info", "hexopt_debug_system",
-                     "hexopt_opt32_solve", "hexopt_time_kernel", "hexopt_copy_positions"]:
+                     "hexopt_opt32_solve", "hexopt_time_kernel", "hexopt_copy_positions",
+                     "hexopt_surface_normals", "hexopt_compute_surface_normals", "hexopt_surface_closest",
+                     "hexopt_project_planes", "hexopt_occ_classify_cells", "hexopt_occ_voxel_parity",
+                     "hexopt_occ_cell_parity", "hexopt_hex_inside"]:
             getattr(L, name).restype = C.c_int
+        L.hexopt_surface_create.restype = C.c_void_p
+        L.hexopt_surface_create.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_int]
+        L.hexopt_surface_error.restype = C.c_char_p
+        L.hexopt_surface_last_error.restype = C.c_char_p
+        L.hexopt_surface_last_error.argtypes = [C.c_void_p]
+        L.hexopt_surface_destroy.argtypes = [C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -222,6 +235,18 @@ class HexOpt:
         log = np.array([[getattr(l, k) for k in cols] for l in logs[:iters]]).reshape(iters, len(cols))
         return E.value, log
 
+    def project_planes(self, surface, lamT):
+        self._chk(lib().hexopt_project_planes(C.c_void_p(self.h), C.c_void_p(surface.h), C.c_double(lamT)),
+                  "project_planes")
+
+    def hex_inside(self, surface, which=CURRENT):
+        """Signed distance of every hex's bounding-box centre and the inside flags."""
+        S = np.zeros(self.nH)
+        flag = np.zeros(self.nH, dtype=np.uint8)
+        self._chk(lib().hexopt_hex_inside(C.c_void_p(self.h), C.c_void_p(surface.h), which, _ptr(S),
+                                          _ptr(flag, C.c_uint8)), "hex_inside")
+        return S, flag
+
     def time_kernel(self, which, reps=20):
         ms = C.c_double()
         self._chk(lib().hexopt_time_kernel(C.c_void_p(self.h), which, reps, C.byref(ms)), "time_kernel")
@@ -234,3 +259,80 @@ class HexOpt:
         out = np.zeros(5, dtype=np.int32)
         self._chk(lib().hexopt_layout_info(C.c_void_p(self.h), _ptr(out, C.c_int32)), "layout_info")
         return dict(zip(["tiles", "shared", "slots", "max_colours", "ref_mode"], out.tolist()))
+
+
+def surface_normals(V, F):
+    """Face, angle-weighted vertex and per-face-edge normals (host only)."""
+    V, F = _d(V), _i(F)
+    FN = np.zeros((F.shape[0], 3)); VN = np.zeros((V.shape[0], 3)); FEN = np.zeros((F.shape[0], 9))
+    rc = lib().hexopt_compute_surface_normals(V.shape[0], _ptr(V), F.shape[0], _ptr(F, C.c_int32), _ptr(FN), _ptr(VN),
+                                              _ptr(FEN))
+    if rc != 0:
+        raise RuntimeError("hexopt_compute_surface_normals failed (%d)" % rc)
+    return FN, VN, FEN
+
+
+class Surface:
+    """Input triangle surface for closest-point queries (include/hexopt.h)."""
+
+    def __init__(self, V, F, FN=None, VN=None, FEN=None, device=0):
+        self.V, self.F = _d(V), _i(F)
+        arrs = [None if a is None else _d(a) for a in (FN, VN, FEN)]
+        self._keep = arrs
+        ptrs = [None if a is None else a.ctypes.data for a in arrs]
+        self.h = lib().hexopt_surface_create(self.V.shape[0], self.V.ctypes.data, self.F.shape[0], self.F.ctypes.data,
+                                             ptrs[0], ptrs[1], ptrs[2], device)
+        if not self.h:
+            raise RuntimeError("hexopt_surface_create: " + lib().hexopt_surface_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().hexopt_surface_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def closest(self, P):
+        P = _d(P)
+        nq = P.shape[0]
+        S = np.zeros(nq); I = np.zeros(nq, dtype=np.int32); Cc = np.zeros((nq, 3)); N = np.zeros((nq, 3))
+        rc = lib().hexopt_surface_closest(C.c_void_p(self.h), nq, _ptr(P), _ptr(S), _ptr(I, C.c_int32), _ptr(Cc),
+                                          _ptr(N))
+        if rc != 0:
+            raise RuntimeError("hexopt_surface_closest failed (%d): %s" %
+                               (rc, lib().hexopt_surface_last_error(C.c_void_p(self.h)).decode()))
+        return S, I, Cc, N
+
+    def _schk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError("%s failed (%d): %s" % (what, rc, lib().hexopt_surface_last_error(C.c_void_p(self.h)).decode()))
+
+    def classify_cells(self, cells, origin, voxel_size, mesh_transform=None, stop_extent=1):
+        """Subdivision predicate for n x 4 cells (x, y, z, extent): uint8 flags."""
+        cells = _i(cells).reshape(-1, 4)
+        out = np.zeros(cells.shape[0], dtype=np.uint8)
+        mt = None if mesh_transform is None else _d(mesh_transform)
+        self._schk(lib().hexopt_occ_classify_cells(C.c_void_p(self.h), cells.shape[0], _ptr(cells, C.c_int32),
+                                                    _ptr(_d(origin)), C.c_double(voxel_size), _ptr(mt),
+                                                    int(stop_extent), _ptr(out, C.c_uint8)), "occ_classify_cells")
+        return out
+
+    def voxel_parity(self, nx, ny, nz, origin, spacing):
+        """Inside flags of voxel centres, shaped (nz, ny, nx)."""
+        out = np.zeros(nx * ny * nz, dtype=np.uint8)
+        self._schk(lib().hexopt_occ_voxel_parity(C.c_void_p(self.h), nx, ny, nz, _ptr(_d(origin)), C.c_double(spacing),
+                                                  _ptr(out, C.c_uint8)), "occ_voxel_parity")
+        return out.reshape(nz, ny, nx)
+
+    def cell_parity(self, cells, origin, spacing):
+        """Inside attribute (1.0 / 0.0) of octree cells n x 4 (x, y, z, extent)."""
+        cells = _i(cells).reshape(-1, 4)
+        out = np.zeros(cells.shape[0], dtype=np.float32)
+        self._schk(lib().hexopt_occ_cell_parity(C.c_void_p(self.h), cells.shape[0], _ptr(cells, C.c_int32),
+                                                 _ptr(_d(origin)), C.c_double(spacing), _ptr(out, C.c_float)),
+                   "occ_cell_parity")
+        return out

@@ -113,7 +113,7 @@ typedef struct {
   double energy, e_quality, e_soft, step_bound, step;
   int cg_iters;
   double cg_error;     /* relative residual estimate of the FP32 solve */
-  double residual64;   /* |b - L d| computed in FP64 before the correction */
+  double residual64;   /* |b - L d| / |b| computed in FP64 before the correction */
   int inverted_tets;   /* corner tets with negative determinant at the iterate (FP64 test) */
 } hexopt_opt32_log;
 
@@ -146,6 +146,57 @@ int hexopt_scaled_jacobian(hexopt_ctx* ctx, int which, double* min_j, double* av
  * [x(0..nV) y(0..nV) z(0..nV) a_L(0..nL)] in the caller's numbering; outputs nullable. */
 int hexopt_debug_system(hexopt_ctx* ctx, const hexopt_params* prm, const double* p_in, double* rhs, double* inv_diag,
                         double* Lp);
+
+/* ---- closest point on the input surface (igl::AABB + signed_distance_pseudonormal) ----
+ * A surface is built once per input triangle mesh (build_aabb_tree,
+ * grid_hex_meshing.cpp:4231-4248): the tree has igl::AABB::init's exact shape so the face
+ * chosen on ties is the reference's. FN/VN/FEN are nullable (all or none): given, they are
+ * used as is (igl's per_face_normals, per_vertex_normals(ANGLE), per_edge_normals(UNIFORM)
+ * with FEN[f*9 + x*3 ..] = EN(EMAP(f + x*nf))); absent, the library computes them. */
+typedef struct hexopt_surface hexopt_surface;
+hexopt_surface* hexopt_surface_create(int nv, const double* V, int nf, const int32_t* F, const double* FN,
+                                      const double* VN, const double* FEN, int device);
+const char* hexopt_surface_error(void);
+const char* hexopt_surface_last_error(hexopt_surface* s);
+void hexopt_surface_destroy(hexopt_surface* s);
+int hexopt_surface_normals(hexopt_surface* s, double* FN, double* VN, double* FEN);
+/* Host-only: the three normal sets of build_aabb_tree for a triangle mesh (FN nf x 3,
+ * VN nv x 3, FEN nf x 9 as above); needs no GPU. */
+int hexopt_compute_surface_normals(int nv, const double* V, int nf, const int32_t* F, double* FN, double* VN,
+                                   double* FEN);
+/* igl::signed_distance_pseudonormal(P, V, F, tree, FN, VN, EN, EMAP, S, I, C, N)
+ * (signed_distance.h:80-92, signed_distance.cpp:186-257): nq x 3 queries; outputs nullable. */
+int hexopt_surface_closest(hexopt_surface* s, int nq, const double* P, double* S, int32_t* I, double* C, double* N);
+/* projection_smooth (grid_hex_meshing.cpp:3760-3781) on the device: the plane vertices of the
+ * context's constraints (ids_T) are projected at CURRENT; normal_T = pseudonormal,
+ * dis_T = normal_T . closest point, and the plane weight becomes lamda_T. */
+int hexopt_project_planes(hexopt_ctx* ctx, hexopt_surface* s, double lamda_T);
+
+/* ---- occupancy (octree subdivision predicate, +z ray parity, inside test of hexes) ----
+ * All arrays are host arrays; cells are n x 4 int32 (x, y, z, extent) in grid units. */
+
+/* should_subdivide of octree_mesh (grid_hex_meshing.cpp:502-517) for a batch of cells (one
+ * BFS level of OctreeGrid::subdivide, octree.cpp:648-690): split[i] = 1 iff extent >
+ * stop_extent and the closed box [mesh_transform + origin + voxel_size*(x,y,z),
+ * + voxel_size*extent] touches the closed box of at least one facet
+ * (GEO::MeshFacetsAABB::compute_bbox_facet_bbox_intersections). mesh_transform may be NULL
+ * (compute_octree's predicate, voxelization.cpp:371-383, with stop_extent = 1). */
+int hexopt_occ_classify_cells(hexopt_surface* s, int n, const int32_t* cells, const double* origin, double voxel_size,
+                              const double* mesh_transform, int stop_extent, uint8_t* split);
+/* compute_sign(M, aabb, VoxelGrid&) (voxelization.h:221-272): voxel centres
+ * (i + 0.5) * spacing + origin; inside[(z*ny + y)*nx + x] = 1 where the signed count of +z
+ * ray crossings below the centre is negative. ny <= 65535. */
+int hexopt_occ_voxel_parity(hexopt_surface* s, int nx, int ny, int nz, const double* origin, double spacing,
+                            uint8_t* inside);
+/* compute_sign(M, aabb, OctreeGrid&, origin, spacing) (voxelization.cpp:101-163): the
+ * octree cells' "inside" attribute (1.0f / 0.0f) for the given cells. */
+int hexopt_occ_cell_parity(hexopt_surface* s, int n, const int32_t* cells, const double* origin, double spacing,
+                           float* inside);
+/* clean_hex_mesh's inside test (grid_hex_meshing.cpp:1937-1952 -> points_inside_mesh,
+ * global_functions.cpp:4024-4048): pseudonormal signed distance of every hex's bounding-box
+ * centre at REST or CURRENT (signed_dis, nH, nullable) and H_flag[h] = signed_dis < 0
+ * (nH bytes, nullable), in the caller's hex order. */
+int hexopt_hex_inside(hexopt_ctx* ctx, hexopt_surface* s, int which, double* signed_dis, uint8_t* H_flag);
 
 /* Blocks until the context's stream is idle. */
 int hexopt_synchronize(hexopt_ctx* ctx);

@@ -1186,8 +1186,15 @@ void orc_signed_distance_pseudonormal(int nq, const double* P, int nv, const dou
     const double* B_ = &V[(size_t)F[(size_t)f * 3 + 1] * 3];
     const double* C_ = &V[(size_t)F[(size_t)f * 3 + 2] * 3];
     V3 a = {A_[0], A_[1], A_[2]}, b = {B_[0], B_[1], B_[2]}, cc = {C_[0], C_[1], C_[2]};
-    // doublearea(A,B,C) for 3D rows: norm of the cross product of the edge vectors
-    double area = norm(cross(sub(b, a), sub(cc, a)));
+    // doublearea(A,B,C) for 3D rows (doublearea.cpp:81-111,131-170): Kahan's Heron formula
+    // on the edge lengths sorted in decreasing order
+    double area;
+    {
+      double l[3] = {norm(sub(b, cc)), norm(sub(cc, a)), norm(sub(a, b))};
+      std::sort(l, l + 3, [](double x, double y) { return x > y; });
+      double arg = (l[0] + (l[1] + l[2])) * (l[2] - (l[0] - l[1])) * (l[2] + (l[0] - l[1])) * (l[0] + (l[1] - l[2]));
+      area = 2.0 * 0.25 * std::sqrt(arg);
+    }
     double n[3];
     const double MIN_DOUBLE_AREA = 1e-4, epsilon = 1e-12;
     if (area > MIN_DOUBLE_AREA) {
@@ -1278,11 +1285,13 @@ void orc_cell_facet_overlap(int n, const int* cells, int nv, const double* V, in
 }
 }  // extern "C"
 
-// +z ray parity inside test of voxel centres: compute_sign(VoxelGrid) (voxelization.h:221-272)
-// with intersect_ray_z (:195-217), point_in_triangle_2d and its SOS orientation
-// (voxelization.cpp:57-93), brute force over all triangles (the facet-bbox prefilter of the
-// reference is a closed-interval test that never rejects a facet the 2D test accepts).
-// inside: nx*ny*nz bytes, index (z*ny + y)*nx + x.
+// +z ray parity inside tests: compute_sign(VoxelGrid) (voxelization.h:221-272) and
+// compute_sign(OctreeGrid) (voxelization.cpp:101-163) with intersect_ray_z
+// (voxelization.h:195-217), point_in_triangle_2d and its SOS orientation
+// (voxelization.cpp:57-93). Literal restatements: every facet goes through the reference's
+// closed-box prefilter (its query box spans the mesh in z, so only x and y can reject),
+// crossings are sorted by (z, sign), and the octree variant keeps the reference's reduced
+// crossing list and odd-count rule.
 namespace {
 int sos_orientation(double x1, double y1, double x2, double y2, double& twice_signed_area) {
   twice_signed_area = y1 * x2 - x1 * y2;
@@ -1321,6 +1330,31 @@ int intersect_ray_z(const double* p1, const double* p2, const double* p3, double
 }
 }  // namespace
 
+namespace {
+// crossings of the vertical line at (qx, qy) with the facets whose closed xy box touches
+// [bx0, bx1] x [by0, by1] (geogram get_facet_bbox + bboxes_overlap), sorted by (z, sign)
+std::vector<std::pair<double, int>> column_crossings(const double* V, int nf, const int* F, double qx, double qy,
+                                                     double bx0, double bx1, double by0, double by1) {
+  std::vector<std::pair<double, int>> inter;
+  for (int f = 0; f < nf; ++f) {
+    const double* p[3] = {&V[(size_t)F[(size_t)f * 3] * 3], &V[(size_t)F[(size_t)f * 3 + 1] * 3],
+                          &V[(size_t)F[(size_t)f * 3 + 2] * 3]};
+    double mn[2], mx[2];
+    for (int d = 0; d < 2; ++d) {
+      mn[d] = mx[d] = p[0][d];
+      for (int c = 1; c < 3; ++c) { mn[d] = std::min(mn[d], p[c][d]); mx[d] = std::max(mx[d], p[c][d]); }
+    }
+    if (bx1 < mn[0] || bx0 > mx[0] || by1 < mn[1] || by0 > mx[1]) continue;
+    double z;
+    int s = intersect_ray_z(p[0], p[1], p[2], qx, qy, z);
+    if (s) inter.emplace_back(z, s);
+  }
+  std::sort(inter.begin(), inter.end());
+  return inter;
+}
+}  // namespace
+
+// inside: nx*ny*nz bytes, index (z*ny + y)*nx + x.
 extern "C" void orc_voxel_parity(int nv, const double* V, int nf, const int* F, int nx, int ny, int nz,
                                  const double* origin, double spacing, uint8_t* inside) {
   (void)nv;
@@ -1328,14 +1362,7 @@ extern "C" void orc_voxel_parity(int nv, const double* V, int nf, const int* F, 
     for (int x = 0; x < nx; ++x) {
       double cx = (x + 0.5) * spacing + origin[0];
       double cy = (y + 0.5) * spacing + origin[1];
-      std::vector<std::pair<double, int>> inter;
-      for (int f = 0; f < nf; ++f) {
-        double z;
-        int s = intersect_ray_z(&V[(size_t)F[(size_t)f * 3] * 3], &V[(size_t)F[(size_t)f * 3 + 1] * 3],
-                                &V[(size_t)F[(size_t)f * 3 + 2] * 3], cx, cy, z);
-        if (s) inter.emplace_back(z, s);
-      }
-      std::sort(inter.begin(), inter.end());
+      std::vector<std::pair<double, int>> inter = column_crossings(V, nf, F, cx, cy, cx, cx, cy, cy);
       size_t i = 0;
       int s = 0;
       for (int z = 0; z < nz; ++z) {
@@ -1344,4 +1371,28 @@ extern "C" void orc_voxel_parity(int nv, const double* V, int nf, const int* F, 
         inside[((size_t)z * ny + y) * nx + x] = (uint8_t)(s < 0 ? 1 : 0);
       }
     }
+}
+
+// OctreeGrid variant: cells n x 4 (x, y, z, extent); inside[i] in {0, 1}.
+extern "C" void orc_cell_parity(int nv, const double* V, int nf, const int* F, int n, const int* cells,
+                                const double* origin, double spacing, float* inside) {
+  (void)nv;
+  for (int i = 0; i < n; ++i) {
+    const int x = cells[4 * i], y = cells[4 * i + 1], z = cells[4 * i + 2], extent = cells[4 * i + 3];
+    double bx0 = origin[0] + spacing * x, by0 = origin[1] + spacing * y;
+    double bx1 = bx0 + spacing * extent, by1 = by0 + spacing * extent;
+    double cx = bx0 + 0.5 * spacing * extent, cy = by0 + 0.5 * spacing * extent;
+    double cz = origin[2] + spacing * z + 0.5 * spacing * extent;
+    std::vector<std::pair<double, int>> inter = column_crossings(V, nf, F, cx, cy, bx0, bx1, by0, by1);
+    std::vector<double> reduced;
+    for (int k = 0, s = 0; k < (int)inter.size(); ++k) {
+      const int ds = inter[k].second;
+      s += ds;
+      if ((s == -1 && ds < 0) || (s == 0 && ds > 0)) reduced.push_back(inter[k].first);
+    }
+    int num_before = 0;
+    for (double zz : reduced)
+      if (zz < cz) ++num_before;
+    inside[i] = (num_before % 2 == 1) ? 1.0f : 0.0f;
+  }
 }

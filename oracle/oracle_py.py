@@ -155,6 +155,14 @@ def voxel_parity(V, F, nx, ny, nz, origin, spacing):
     return out.reshape(nz, ny, nx)
 
 
+def cell_parity(V, F, cells, origin, spacing):
+    V, F, o, cells = _d(V), _i(F), _d(origin), _i(cells)
+    out = np.zeros(cells.shape[0], dtype=np.float32)
+    lib().orc_cell_parity(V.shape[0], _p(V, C.c_double), F.shape[0], _p(F, C.c_int), cells.shape[0],
+                          _p(cells, C.c_int), _p(o, C.c_double), C.c_double(spacing), _p(out, C.c_float))
+    return out
+
+
 def linear_system(V, H, G, UV, energy=2, weight_opt=1.0, lam_C=0.0, lam_T=0.0, lam_L=0.0, idsC=None, Cpos=None,
                   idsT=None, normalT=None, disT=None, idsL=None, axaL=None, originL=None, dense=True, p=None):
     """Global system of the first SLIM iteration at UV: (L dense or None, rhs), plus (L p, diag)

@@ -114,7 +114,11 @@ def test_slim_larger_grid_vs_oracle(seed):
         _, rhs_o, Lp_o, dg_o = O.linear_system(V, H, G, X, dense=False, p=p, **kw)
         h.set_positions(capi.CURRENT, X)
         rhs, invd, Lp = h.debug_system(p)
-        assert rel(rhs, rhs_o) < 1e-12 and rel(1.0 / invd, dg_o) < 1e-12 and rel(Lp, Lp_o) < 1e-12
+        # the weights and rotations come from each corner tet's SVD: a weight ~ s^-3/2 at a
+        # small singular value s and a rotation ~ 1/(s2 - s3) near a double singular value
+        # amplify the last-bit differences of two accurate SVDs past 1e-12 once tets are
+        # strongly distorted
+        assert rel(rhs, rhs_o) < 1e-10 and rel(1.0 / invd, dg_o) < 1e-10 and rel(Lp, Lp_o) < 1e-10
         X, *_ = O.slim_m_opt(V, H, G, X, 1, **kw)
     h.set_positions(capi.CURRENT, V)
     E, Eq, Es, log, _ = h.slim_solve(1)
