@@ -20,6 +20,7 @@
 #include "layout.h"
 #include "bvh.h"
 #include "occupancy.h"
+#include "lines.h"
 
 namespace hexopt {
 namespace {
@@ -1038,6 +1039,95 @@ int hexopt_hex_inside(hexopt_ctx* c, hexopt_surface* s, int which, double* signe
     if (signed_dis) signed_dis[h] = S[i];
     if (H_flag) H_flag[h] = S[i] < 0 ? 1 : 0;
   }
+  return HEXOPT_OK;
+}
+
+struct hexopt_curves {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nv = 0, ncurves = 0;
+  std::vector<int> off;
+  DevBuf dV, doff, dvs, dcirc, drows, dcid, dorig, daxis;
+  std::string err;
+};
+
+namespace {
+thread_local std::string g_curves_error;
+}
+
+const char* hexopt_curves_error(void) { return g_curves_error.c_str(); }
+
+hexopt_curves* hexopt_curves_create(int nv, const double* V, int ncurves, const int32_t* curve_off,
+                                    const int32_t* curve_vs, const uint8_t* circles, int device) {
+  g_curves_error.clear();
+  if (nv <= 0 || ncurves <= 0 || !V || !curve_off || !curve_vs || !circles) { g_curves_error = "invalid curves"; return nullptr; }
+  if (curve_off[0] != 0) { g_curves_error = "curve_off[0] must be 0"; return nullptr; }
+  for (int c = 0; c < ncurves; ++c)
+    if (curve_off[c + 1] <= curve_off[c]) { g_curves_error = "every curve needs at least one vertex"; return nullptr; }
+  for (int k = 0; k < curve_off[ncurves]; ++k)
+    if (curve_vs[k] < 0 || curve_vs[k] >= nv) { g_curves_error = "curve vertex out of range"; return nullptr; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) { g_curves_error = "no such HIP device"; return nullptr; }
+  hexopt_curves* cv = new hexopt_curves();
+  cv->device = device;
+  cv->nv = nv;
+  cv->ncurves = ncurves;
+  cv->off.assign(curve_off, curve_off + ncurves + 1);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&cv->stream) != hipSuccess) {
+    g_curves_error = "hipStreamCreate failed";
+    delete cv;
+    return nullptr;
+  }
+  const size_t nk = (size_t)curve_off[ncurves];
+  bool ok = cv->dV.ensure((size_t)nv * 24) == hipSuccess && cv->doff.ensure((size_t)(ncurves + 1) * 4) == hipSuccess &&
+            cv->dvs.ensure(nk * 4) == hipSuccess && cv->dcirc.ensure((size_t)ncurves) == hipSuccess;
+  ok = ok && hipMemcpyAsync(cv->dV.p, V, (size_t)nv * 24, hipMemcpyHostToDevice, cv->stream) == hipSuccess;
+  ok = ok && hipMemcpyAsync(cv->doff.p, curve_off, (size_t)(ncurves + 1) * 4, hipMemcpyHostToDevice, cv->stream) == hipSuccess;
+  ok = ok && hipMemcpyAsync(cv->dvs.p, curve_vs, nk * 4, hipMemcpyHostToDevice, cv->stream) == hipSuccess;
+  ok = ok && hipMemcpyAsync(cv->dcirc.p, circles, (size_t)ncurves, hipMemcpyHostToDevice, cv->stream) == hipSuccess;
+  ok = ok && hipStreamSynchronize(cv->stream) == hipSuccess;
+  if (!ok) {
+    g_curves_error = "device upload failed";
+    hexopt_curves_destroy(cv);
+    return nullptr;
+  }
+  return cv;
+}
+
+void hexopt_curves_destroy(hexopt_curves* cv) {
+  if (!cv) return;
+  if (cv->stream) (void)hipStreamSynchronize(cv->stream);
+  hipStream_t st = cv->stream;
+  delete cv;
+  if (st) (void)hipStreamDestroy(st);
+}
+
+int hexopt_project_lines(hexopt_ctx* c, hexopt_curves* cv, int which, int nL, const int32_t* ids_L,
+                         const int32_t* curve_L, double* origin_L, double* Axa_L) {
+  if (!c || !cv || (which != HEXOPT_REST && which != HEXOPT_CURRENT) || nL < 0) return HEXOPT_EINVAL;
+  if (nL == 0) return HEXOPT_OK;
+  if (!ids_L || !curve_L || !origin_L || !Axa_L) return HEXOPT_EINVAL;
+  if (cv->device != c->device) return fail(c, HEXOPT_EINVAL, "curves and context on different devices");
+  std::vector<int> rows(nL);
+  for (int i = 0; i < nL; ++i) {
+    if (ids_L[i] < 0 || ids_L[i] >= c->L.nV) return fail(c, HEXOPT_EINVAL, "ids_L out of range");
+    if (curve_L[i] < 0 || curve_L[i] >= cv->ncurves) return fail(c, HEXOPT_EINVAL, "curve_L out of range");
+    rows[i] = c->L.vinv[ids_L[i]];
+  }
+  hipStream_t st = c->stream;
+  HIPCHK(c, cv->drows.ensure((size_t)nL * 4));
+  HIPCHK(c, cv->dcid.ensure((size_t)nL * 4));
+  HIPCHK(c, cv->dorig.ensure((size_t)nL * 24));
+  HIPCHK(c, cv->daxis.ensure((size_t)nL * 24));
+  HIPCHK(c, hipMemcpyAsync(cv->drows.p, rows.data(), (size_t)nL * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(cv->dcid.p, curve_L, (size_t)nL * 4, hipMemcpyHostToDevice, st));
+  CurvesDev view{cv->dV.as<double>(), cv->doff.as<int>(), cv->dvs.as<int>(), cv->dcirc.as<uint8_t>()};
+  launch_project_lines(view, (which == HEXOPT_REST ? c->d_rest : c->d_cur).as<double>(), c->L.nV, nL,
+                       cv->drows.as<int>(), cv->dcid.as<int>(), cv->dorig.as<double>(), cv->daxis.as<double>(), st);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(origin_L, cv->dorig.p, (size_t)nL * 24, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(Axa_L, cv->daxis.p, (size_t)nL * 24, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
   return HEXOPT_OK;
 }
 

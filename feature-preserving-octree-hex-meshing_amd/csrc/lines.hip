@@ -1,0 +1,106 @@
+// lines.hip -- feature-curve projection of LINE vertices, FP64: the LINE branch of
+// dirty_graph_projection (grid_hex_meshing.cpp:3961-4026). One lane per vertex walks the
+// segments of its curve (point_line_projection, global_functions.cpp:3454-3465) and keeps
+// the nearest; a strict '<' keeps the lowest segment index on equal distances, which is the
+// first entry after the reference's sort of (distance, index) pairs. Built without fused
+// multiply-add; sums of three products add x, y, z in that order.
+#include <hip/hip_runtime.h>
+#include "lines.h"
+
+namespace hexopt {
+namespace {
+
+__device__ __forceinline__ double norm3(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
+
+// (a - b).normalized(): Eigen 3.2.9 divides by the norm without a zero test
+__device__ __forceinline__ void normalized_diff(const double* a, const double* b, double o[3]) {
+  const double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+  const double n = norm3(d0, d1, d2);
+  o[0] = d0 / n;
+  o[1] = d1 / n;
+  o[2] = d2 / n;
+}
+
+__global__ void project_lines_kernel(CurvesDev cv, const double* __restrict__ X, int nV, int n,
+                                     const int* __restrict__ rows, const int* __restrict__ curve_of,
+                                     double* __restrict__ origin, double* __restrict__ axis) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int iv = rows[q];
+  const double v[3] = {X[iv], X[(size_t)nV + iv], X[2 * (size_t)nV + iv]};
+  const int c = curve_of[q];
+  const int* curve = cv.vs + cv.off[c];
+  const int csize = cv.off[c + 1] - cv.off[c];
+  const bool circle = cv.circles[c] != 0;
+  const int curve_len = circle ? csize : csize - 1;
+  double pv[3] = {0, 0, 0}, tangent[3] = {1, 0, 0};
+  if (curve_len > 0) {
+    double best = 0;
+    for (int j = 0; j < curve_len; ++j) {
+      const double* p0 = cv.V + (size_t)curve[j] * 3;
+      const double* p1 = cv.V + (size_t)curve[(j + 1) % csize] * 3;
+      // point_line_projection
+      const double a0 = v[0] - p0[0], a1 = v[1] - p0[1], a2 = v[2] - p0[2];
+      const double e0 = p1[0] - p0[0], e1 = p1[1] - p0[1], e2 = p1[2] - p0[2];
+      const double l2 = e0 * e0 + e1 * e1 + e2 * e2;
+      double t = 0;
+      if (l2 >= 1.e-7) t = (a0 * e0 + a1 * e1 + a2 * e2) / l2;
+      double q3[3];
+      if (t >= 0.0 && t <= 1.0) {
+        for (int d = 0; d < 3; ++d) q3[d] = p0[d] + t * (p1[d] - p0[d]);
+      } else if (t < 0.0) {
+        for (int d = 0; d < 3; ++d) q3[d] = p0[d];
+      } else {
+        for (int d = 0; d < 3; ++d) q3[d] = p1[d];
+      }
+      const double dist = norm3(v[0] - q3[0], v[1] - q3[1], v[2] - q3[2]);
+      if (j == 0 || dist < best) {
+        best = dist;
+        for (int d = 0; d < 3; ++d) pv[d] = q3[d];
+        normalized_diff(p1, p0, tangent);
+      }
+    }
+  } else {
+    // no segment (an open curve of one vertex): nearest curve vertex, tangent from its
+    // neighbours added to (1, 0, 0), then normalised
+    int pos = 0;
+    double best = 0;
+    for (int j = 0; j < csize; ++j) {
+      const double* w = cv.V + (size_t)curve[j] * 3;
+      const double dist = norm3(w[0] - v[0], w[1] - v[1], w[2] - v[2]);
+      if (j == 0 || dist < best) { best = dist; pos = j; }
+    }
+    const double* pp = cv.V + (size_t)curve[pos] * 3;
+    for (int d = 0; d < 3; ++d) pv[d] = pp[d];
+    double e[3];
+    if (circle || (pos != 0 && pos != csize - 1)) {
+      normalized_diff(pp, cv.V + (size_t)curve[(pos - 1 + csize) % csize] * 3, e);
+      for (int d = 0; d < 3; ++d) tangent[d] += e[d];
+      normalized_diff(cv.V + (size_t)curve[(pos + 1) % csize] * 3, pp, e);
+      for (int d = 0; d < 3; ++d) tangent[d] += e[d];
+    } else if (pos == 0) {
+      normalized_diff(cv.V + (size_t)curve[(pos + 1) % csize] * 3, pp, e);
+      for (int d = 0; d < 3; ++d) tangent[d] += e[d];
+    } else {
+      normalized_diff(pp, cv.V + (size_t)curve[(pos - 1 + csize) % csize] * 3, e);
+      for (int d = 0; d < 3; ++d) tangent[d] += e[d];
+    }
+    const double nt = norm3(tangent[0], tangent[1], tangent[2]);
+    for (int d = 0; d < 3; ++d) tangent[d] /= nt;
+  }
+  for (int d = 0; d < 3; ++d) {
+    origin[(size_t)q * 3 + d] = pv[d];
+    axis[(size_t)q * 3 + d] = tangent[d];
+  }
+}
+
+}  // namespace
+
+void launch_project_lines(const CurvesDev& cv, const double* X, int nV, int n, const int* rows, const int* curve_of,
+                          double* origin, double* axis, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(project_lines_kernel, dim3((n + 127) / 128), dim3(128), 0, st, cv, X, nV, n, rows, curve_of, origin,
+                     axis);
+}
+
+}  // namespace hexopt

@@ -1,4 +1,4 @@
 """Python side of the MI355X hex-optimisation path: ctypes binding of libhexopt.so
 (capi), synthetic workloads (meshgen) and the torch autograd wrapper of the adjoint
-(autograd, imported on demand)."""
+(pyhexopt.autograd, imported on demand because it needs torch)."""
 from . import capi, meshgen  # noqa: F401

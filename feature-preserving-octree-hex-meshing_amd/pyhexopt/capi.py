@@ -20,7 +20,8 @@ EXPORTS = ["hexopt_create", "hexopt_create_error", "hexopt_destroy", "hexopt_las
            "hexopt_time_kernel", "hexopt_copy_positions", "hexopt_surface_create", "hexopt_surface_error",
            "hexopt_surface_last_error", "hexopt_surface_destroy", "hexopt_surface_normals",
            "hexopt_compute_surface_normals", "hexopt_surface_closest", "hexopt_project_planes",
-           "hexopt_occ_classify_cells", "hexopt_occ_voxel_parity", "hexopt_occ_cell_parity", "hexopt_hex_inside"]
+           "hexopt_occ_classify_cells", "hexopt_occ_voxel_parity", "hexopt_occ_cell_parity", "hexopt_hex_inside",
+           "hexopt_curves_create", "hexopt_curves_error", "hexopt_curves_destroy", "hexopt_project_lines"]
 
 _LIB = None
 
@@ -74,8 +75,12 @@ def lib():
                      "hexopt_opt32_solve", "hexopt_time_kernel", "hexopt_copy_positions",
                      "hexopt_surface_normals", "hexopt_compute_surface_normals", "hexopt_surface_closest",
                      "hexopt_project_planes", "hexopt_occ_classify_cells", "hexopt_occ_voxel_parity",
-                     "hexopt_occ_cell_parity", "hexopt_hex_inside"]:
+                     "hexopt_occ_cell_parity", "hexopt_hex_inside", "hexopt_project_lines"]:
             getattr(L, name).restype = C.c_int
+        L.hexopt_curves_create.restype = C.c_void_p
+        L.hexopt_curves_create.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.hexopt_curves_error.restype = C.c_char_p
+        L.hexopt_curves_destroy.argtypes = [C.c_void_p]
         L.hexopt_surface_create.restype = C.c_void_p
         L.hexopt_surface_create.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_int]
@@ -239,6 +244,15 @@ class HexOpt:
         self._chk(lib().hexopt_project_planes(C.c_void_p(self.h), C.c_void_p(surface.h), C.c_double(lamT)),
                   "project_planes")
 
+    def project_lines(self, curves, ids, curve_of, which=CURRENT):
+        """origin_L and Axa_L (n x 3) of LINE vertices ids on curves curve_of."""
+        ids, curve_of = _i(ids), _i(curve_of)
+        n = ids.size
+        origin = np.zeros((n, 3)); axis = np.zeros((n, 3))
+        self._chk(lib().hexopt_project_lines(C.c_void_p(self.h), C.c_void_p(curves.h), which, n, _ptr(ids, C.c_int32),
+                                             _ptr(curve_of, C.c_int32), _ptr(origin), _ptr(axis)), "project_lines")
+        return origin, axis
+
     def hex_inside(self, surface, which=CURRENT):
         """Signed distance of every hex's bounding-box centre and the inside flags."""
         S = np.zeros(self.nH)
@@ -336,3 +350,28 @@ class Surface:
                                                  _ptr(_d(origin)), C.c_double(spacing), _ptr(out, C.c_float)),
                    "occ_cell_parity")
         return out
+
+
+class Curves:
+    """Feature curves of the input surface: lists of surface vertex ids, closed when circle."""
+
+    def __init__(self, V, curves, circles, device=0):
+        self.V = _d(V)
+        self.off = _i(np.concatenate([[0], np.cumsum([len(c) for c in curves])]))
+        self.vs = _i(np.concatenate([np.asarray(c, dtype=np.int32) for c in curves]))
+        self.circ = np.ascontiguousarray(circles, dtype=np.uint8)
+        self.h = lib().hexopt_curves_create(self.V.shape[0], self.V.ctypes.data, len(curves), self.off.ctypes.data,
+                                            self.vs.ctypes.data, self.circ.ctypes.data, device)
+        if not self.h:
+            raise RuntimeError("hexopt_curves_create: " + lib().hexopt_curves_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().hexopt_curves_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

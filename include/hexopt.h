@@ -198,6 +198,23 @@ int hexopt_occ_cell_parity(hexopt_surface* s, int n, const int32_t* cells, const
  * (nH bytes, nullable), in the caller's hex order. */
 int hexopt_hex_inside(hexopt_ctx* ctx, hexopt_surface* s, int which, double* signed_dis, uint8_t* H_flag);
 
+/* ---- feature-curve projection (LINE branch of dirty_graph_projection,
+ * grid_hex_meshing.cpp:3961-4026, with point_line_projection, global_functions.cpp:3454-3465) ----
+ * A curve set is the input surface's feature curves (mf.curve_vs / mf.circles): curve c is
+ * the surface vertex ids curve_vs[curve_off[c] .. curve_off[c+1]) (at least one), closed
+ * when circles[c]. V is the surface's nv x 3 vertices. */
+typedef struct hexopt_curves hexopt_curves;
+hexopt_curves* hexopt_curves_create(int nv, const double* V, int ncurves, const int32_t* curve_off,
+                                    const int32_t* curve_vs, const uint8_t* circles, int device);
+const char* hexopt_curves_error(void);
+void hexopt_curves_destroy(hexopt_curves* cv);
+/* For the nL LINE vertices ids_L of the context (positions at REST or CURRENT) on curves
+ * curve_L (fc.V_ids): origin_L = closest point of the nearest curve segment (ties to the
+ * lower segment index) and Axa_L = that segment's unit direction, both nL x 3 -- the
+ * fc.origin_L / fc.Axa_L rows of the reference. */
+int hexopt_project_lines(hexopt_ctx* ctx, hexopt_curves* cv, int which, int nL, const int32_t* ids_L,
+                         const int32_t* curve_L, double* origin_L, double* Axa_L);
+
 /* Blocks until the context's stream is idle. */
 int hexopt_synchronize(hexopt_ctx* ctx);
 

@@ -28,6 +28,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <array>
 #include <functional>
 #include <limits>
 #include <numeric>
@@ -1394,5 +1395,91 @@ extern "C" void orc_cell_parity(int nv, const double* V, int nf, const int* F, i
     for (double zz : reduced)
       if (zz < cz) ++num_before;
     inside[i] = (num_before % 2 == 1) ? 1.0f : 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Feature-curve projection of LINE vertices: the LINE branch of dirty_graph_projection
+// (grid_hex_meshing.cpp:3961-4026) with point_line_projection (global_functions.cpp:
+// 3454-3465, Precision = 1e-7 from global_types.h:23). Curves are CSR lists of surface
+// vertex ids (off[c]..off[c+1]); a circle also has the closing segment. Vector norms and
+// dot products sum x, y, z in that order.
+namespace {
+double n3(const double* a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+void point_line_projection(const double* v1, const double* v2, const double* v, double* pv) {
+  double vv1[3], v21[3];
+  for (int d = 0; d < 3; ++d) { vv1[d] = v[d] - v1[d]; v21[d] = v2[d] - v1[d]; }
+  const double nv21_2 = v21[0] * v21[0] + v21[1] * v21[1] + v21[2] * v21[2];
+  double t = 0;
+  if (nv21_2 >= 1.e-7) t = (vv1[0] * v21[0] + vv1[1] * v21[1] + vv1[2] * v21[2]) / nv21_2;
+  if (t >= 0.0 && t <= 1.0) for (int d = 0; d < 3; ++d) pv[d] = v1[d] + t * (v2[d] - v1[d]);
+  else if (t < 0.0) for (int d = 0; d < 3; ++d) pv[d] = v1[d];
+  else if (t > 1.0) for (int d = 0; d < 3; ++d) pv[d] = v2[d];
+}
+void normalized_diff(const double* a, const double* b, double* out) {  // (a - b).normalized()
+  double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+  const double n = n3(d);
+  for (int k = 0; k < 3; ++k) out[k] = d[k] / n;
+}
+}  // namespace
+
+extern "C" void orc_line_projection(int nv, const double* V, int ncurves, const int* off, const int* cvs,
+                                    const uint8_t* circles, int n, const double* P, const int* curve_of,
+                                    double* origin, double* axis) {
+  (void)nv; (void)ncurves;
+  for (int q = 0; q < n; ++q) {
+    const double* v = P + (size_t)q * 3;
+    const int c = curve_of[q];
+    const int* curve = cvs + off[c];
+    const int csize = off[c + 1] - off[c];
+    int curve_len = csize;
+    if (!circles[c]) curve_len--;
+    std::vector<std::array<double, 3>> pvs, tangents;
+    std::vector<std::pair<double, uint32_t>> dis_ids;
+    double tangent[3] = {1, 0, 0}, pv[3] = {0, 0, 0};
+    for (int j = 0; j < curve_len; ++j) {
+      const double* p0 = V + (size_t)curve[j] * 3;
+      const double* p1 = V + (size_t)curve[(j + 1) % csize] * 3;
+      point_line_projection(p0, p1, v, pv);
+      normalized_diff(p1, p0, tangent);
+      double dv[3] = {v[0] - pv[0], v[1] - pv[1], v[2] - pv[2]};
+      dis_ids.emplace_back(n3(dv), (uint32_t)pvs.size());
+      pvs.push_back({pv[0], pv[1], pv[2]});
+      tangents.push_back({tangent[0], tangent[1], tangent[2]});
+    }
+    std::sort(dis_ids.begin(), dis_ids.end());
+    if (!dis_ids.empty()) {
+      const uint32_t k = dis_ids[0].second;
+      for (int d = 0; d < 3; ++d) { pv[d] = pvs[k][d]; tangent[d] = tangents[k][d]; }
+    } else {
+      for (int j = 0; j < csize; ++j) {
+        const double* w = V + (size_t)curve[j] * 3;
+        double dv[3] = {w[0] - v[0], w[1] - v[1], w[2] - v[2]};
+        dis_ids.emplace_back(n3(dv), (uint32_t)j);
+      }
+      std::sort(dis_ids.begin(), dis_ids.end());
+      const int pos = (int)dis_ids[0].second;
+      for (int d = 0; d < 3; ++d) pv[d] = V[(size_t)curve[pos] * 3 + d];
+      curve_len = csize;
+      double e[3];
+      if (circles[c] || (!circles[c] && pos != 0 && pos != curve_len - 1)) {
+        const int pos_0 = (pos - 1 + curve_len) % curve_len, pos_1 = (pos + 1) % curve_len;
+        normalized_diff(V + (size_t)curve[pos] * 3, V + (size_t)curve[pos_0] * 3, e);
+        for (int d = 0; d < 3; ++d) tangent[d] += e[d];
+        normalized_diff(V + (size_t)curve[pos_1] * 3, V + (size_t)curve[pos] * 3, e);
+        for (int d = 0; d < 3; ++d) tangent[d] += e[d];
+      } else if (!circles[c] && pos == 0) {
+        const int pos_1 = (pos + 1) % curve_len;
+        normalized_diff(V + (size_t)curve[pos_1] * 3, V + (size_t)curve[pos] * 3, e);
+        for (int d = 0; d < 3; ++d) tangent[d] += e[d];
+      } else if (!circles[c] && pos == curve_len - 1) {
+        const int pos_0 = (pos - 1 + curve_len) % curve_len;
+        normalized_diff(V + (size_t)curve[pos] * 3, V + (size_t)curve[pos_0] * 3, e);
+        for (int d = 0; d < 3; ++d) tangent[d] += e[d];
+      }
+      const double nt = n3(tangent);
+      for (int d = 0; d < 3; ++d) tangent[d] /= nt;
+    }
+    for (int d = 0; d < 3; ++d) { origin[(size_t)q * 3 + d] = pv[d]; axis[(size_t)q * 3 + d] = tangent[d]; }
   }
 }

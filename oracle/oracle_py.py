@@ -213,3 +213,17 @@ def sd_grad(H, G, X, M=1.0, lam_C=0.0, lam_T=0.0, lam_L=0.0, idsC=None, Cpos=Non
                       C.byref(cs), C.c_double(lam_C), C.c_double(lam_T), C.c_double(lam_L), _p(aL, C.c_double),
                       _p(out, C.c_double), C.byref(E))
     return E.value, out
+
+
+def line_projection(V, curves, circles, P, curve_of):
+    """origin_L, Axa_L of LINE vertices P (n x 3) on their curves (lists of vertex ids)."""
+    V, P, curve_of = _d(V), _d(P), _i(curve_of)
+    off = _i(np.concatenate([[0], np.cumsum([len(c) for c in curves])]))
+    cvs = _i(np.concatenate([np.asarray(c, dtype=np.int32) for c in curves]))
+    circ = np.ascontiguousarray(circles, dtype=np.uint8)
+    n = P.shape[0]
+    origin = np.zeros((n, 3)); axis = np.zeros((n, 3))
+    lib().orc_line_projection(V.shape[0], _p(V, C.c_double), len(curves), _p(off, C.c_int), _p(cvs, C.c_int),
+                              _p(circ, C.c_uint8), n, _p(P, C.c_double), _p(curve_of, C.c_int),
+                              _p(origin, C.c_double), _p(axis, C.c_double))
+    return origin, axis
