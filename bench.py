@@ -1,12 +1,19 @@
 """Benchmark: 1M-hex optimiser iterations/s, FP32+adjoint vs FP64 plain, on one MI355X.
 
-Workload: the clipped torus (R=1, r=0.35, planar cuts z=+-0.2) as a uniform-level octree
-hex mesh of ~1.0M hexes; its boundary vertices carry tangent-plane constraints to the
-analytic surface. One step = one optimiser iteration of the reference's deformation loop
-shape: compute_referenceMesh + one SLIM iteration (local step, 20-step CG global step,
-flip-avoiding line search). `value` is the FP32+adjoint optimiser's iterations/s; the FP64
-plain (reference-faithful) path is timed on the same input and reported beside it, with the
-min scaled Jacobian both reach after the same number of iterations.
+Workload (config 4 shape): the torus R=1, r=0.35 cut by the planes z=+-0.2, triangulated
+with 40,000 triangles (two tube bands and two flat annuli meeting at four sharp rim
+circles), and a uniform-level octree hex mesh of ~1.0M hexes of its inside. Boundary
+vertices near a rim are LINE vertices on that rim (feature curve); the other boundary
+vertices are tangent-plane vertices.
+
+One step = one iteration of the reference's feature-alignment loop
+(grid_hex_meshing.cpp:3855-3869) on the device: rest <- current, compute_referenceMesh,
+tangent-plane projection onto the triangle surface (closest point + pseudonormal), feature
+projection onto the rims, one SLIM iteration (local step, 20-step CG global step,
+flip-avoiding line search), then lamda_T = lamda_L = min(max(E_q/E_s lamda, 1), 1e8).
+`value` is the FP32+adjoint optimiser's iterations/s; the FP64 plain (reference-faithful)
+path runs the same loop from the same start and is reported beside it, with the min scaled
+Jacobian and the largest boundary-vertex distance to the surface each reaches.
 
 Prints one JSON line.
 """
@@ -22,54 +29,84 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "feature-preserving-octree-hex-meshing_amd"))
 
 HBM_PEAK_GBS = 8000.0
+R_MAJ, R_MIN, Z_CUT = 1.0, 0.35, 0.2
+WEIGHT_OPT = 0.001          # feature_alignment's opt.weight_opt = weight_opt * 0.001
+LAMDA0, LAMDA_BOUND = 1.0, 1e8
 
 
 def build_workload(target):
     from pyhexopt import meshgen as mg
-    V, H, bnd = mg.torus_hex_mesh(target)
-    C, N = mg.clipped_torus_closest(V[bnd])
-    dis = (N * C).sum(axis=1)
-    return V, H, bnd, N, dis
+    V, H, bnd = mg.torus_hex_mesh(target, R_MAJ, R_MIN, Z_CUT)
+    SV, SF, rims = mg.clipped_torus_surface(R_MAJ, R_MIN, Z_CUT, nu=200, nband=30, nring=20)
+    h = np.linalg.norm(V[H[0, 1]] - V[H[0, 0]])
+    w = np.sqrt(R_MIN ** 2 - Z_CUT ** 2)
+    rim_geom = [(R_MAJ - w, -Z_CUT), (R_MAJ - w, Z_CUT), (R_MAJ + w, -Z_CUT), (R_MAJ + w, Z_CUT)]
+    P = V[bnd]
+    rho = np.sqrt(P[:, 0] ** 2 + P[:, 1] ** 2)
+    d = np.stack([np.hypot(rho - rr, P[:, 2] - zz) for rr, zz in rim_geom], axis=1)
+    k = d.argmin(axis=1)
+    line = d[np.arange(k.size), k] < 0.75 * h
+    return dict(V=V, H=H, idsT=bnd[~line].astype(np.int32), idsL=bnd[line].astype(np.int32),
+                curveL=k[line].astype(np.int32), SV=SV, SF=SF, curves=[list(r) for r in rims], h=h)
 
 
-def make_ctx(V, H, bnd, N, dis, lamT):
+def make_ctx(w):
     from pyhexopt import capi
-    h = capi.HexOpt(V, H)
-    h.reference()
-    h.set_constraints(idsT=bnd, normalT=N, disT=dis, lamT=lamT)
-    return h
+    h = capi.HexOpt(w["V"], w["H"])
+    nT, nL = w["idsT"].size, w["idsL"].size
+    h.set_constraints(idsT=w["idsT"], normalT=np.zeros((nT, 3)), disT=np.zeros(nT), lamT=LAMDA0,
+                      idsL=w["idsL"], axaL=np.zeros((nL, 3)), originL=np.zeros((nL, 3)), lamL=LAMDA0)
+    surf = capi.Surface(w["SV"], w["SF"])
+    curves = capi.Curves(w["SV"], w["curves"], np.ones(len(w["curves"]), dtype=np.uint8))
+    return h, surf, curves
 
 
-def run(h, mode, steps):
-    """`steps` optimiser iterations; returns seconds and the per-iteration logs."""
-    t0 = time.perf_counter()
+def run(h, surf, curves, w, mode, steps, lam):
+    """`steps` loop iterations; returns seconds, the per-iteration logs and the final lamda."""
     logs = []
-    for _ in range(steps):
-        h.rest_from_current()              # ts.V = ts.UV (deformation loop, grid_hex_meshing.cpp:2956)
-        h.reference()                      # compute_referenceMesh at the current positions
-        if mode == "fp64":
-            E, Eq, Es, log, _ = h.slim_solve(1)
-        else:
-            E, log = h.opt32_solve(1)
-        logs.append(log[0])
     h.synchronize()
-    return time.perf_counter() - t0, np.array(logs)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.rest_from_current()              # ts.V = ts.UV
+        h.reference()                      # compute_referenceMesh at the current positions
+        h.project_planes(surf, lam)        # tangent planes (dirty_graph_projection, REGULAR vertices)
+        h.refresh_lines(curves, w["curveL"], lam)   # feature lines (LINE vertices)
+        if mode == "fp64":
+            E, Eq, Es, log, _ = h.slim_solve(1, weight_opt=WEIGHT_OPT)
+            row = log[0]
+        else:
+            E, log = h.opt32_solve(1, weight_opt=WEIGHT_OPT)
+            row = log[0]
+            Eq, Es = row[1], row[2]
+        logs.append(row)
+        lam = min(LAMDA0 * max(Eq / Es * lam, 1.0), LAMDA_BOUND) if Es > 0 else LAMDA_BOUND
+    h.synchronize()
+    return time.perf_counter() - t0, np.array(logs), lam
 
 
 def cpu_baseline(target_small):
-    """The CPU oracle (serial FP64 restatement) on a bounded sample: one SLIM iteration on a
-    smaller mesh of the same workload; iterations/s scaled to 1M hexes."""
+    """The CPU oracle (serial FP64 restatement of the same loop iteration: reference
+    operator, plane projection through the restated igl tree, line projection, one SLIM
+    iteration) on a bounded sample: a smaller mesh of the same workload; iterations/s scaled
+    to 1M hexes."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_py as O
-    V, H, bnd, N, dis = build_workload(target_small)
-    G = O.grad_ref(V, H)
+    from pyhexopt import capi
+    w = build_workload(target_small)
+    V, H = w["V"], w["H"]
+    FN, VN, FEN = capi.surface_normals(w["SV"], w["SF"])     # host-only, part of the surface set-up
+    circles = np.ones(len(w["curves"]), dtype=np.uint8)
     t0 = time.perf_counter()
-    O.slim_m_opt(V, H, G, V, 1, lam_T=1.0, idsT=bnd, normalT=N, disT=dis)
+    G = O.grad_ref(V, H)
+    _, _, C, N, _ = O.signed_distance_pseudonormal(V[w["idsT"]], w["SV"], w["SF"], FN, VN, FEN)
+    o, a = O.line_projection(w["SV"], w["curves"], circles, V[w["idsL"]], w["curveL"])
+    O.slim_m_opt(V, H, G, V, 1, weight_opt=WEIGHT_OPT, lam_T=LAMDA0, lam_L=LAMDA0, idsT=w["idsT"], normalT=N,
+                 disT=(N * C).sum(axis=1), idsL=w["idsL"], axaL=a, originL=o)
     dt = time.perf_counter() - t0
     return {"value": (1.0 / dt) * H.shape[0] / 1e6, "unit": "iterations/s (scaled to 1M hexes)", "cores": 1,
             "kind": "port",
-            "sample": "1 FP64 SLIM iteration of the CPU oracle on the %d-hex clipped torus (%.1f s), x %d/1e6"
-                      % (H.shape[0], dt, H.shape[0])}
+            "sample": "1 loop iteration (reference operator, projections, FP64 SLIM iteration) of the CPU oracle "
+                      "on the %d-hex clipped torus (%.1f s), scaled by %d/1e6" % (H.shape[0], dt, H.shape[0])}
 
 
 def main():
@@ -84,23 +121,31 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (see the design notes on "
               "how the mesh would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    V, H, bnd, N, dis = build_workload(args.hexes)
-    nV, nH = V.shape[0], H.shape[0]
-    print("workload: %d hexes, %d vertices, %d constrained boundary vertices" % (nH, nV, bnd.size),
-          file=sys.stderr, flush=True)
+    w = build_workload(args.hexes)
+    nV, nH = w["V"].shape[0], w["H"].shape[0]
+    print("workload: %d hexes, %d vertices, %d plane + %d line vertices, %d surface triangles" %
+          (nH, nV, w["idsT"].size, w["idsL"].size, w["SF"].shape[0]), file=sys.stderr, flush=True)
 
     res = {}
     for mode in ("fp64", "fp32"):
-        h = make_ctx(V, H, bnd, N, dis, 1.0)
-        run(h, mode, args.warmup)
-        secs, logs = run(h, mode, args.steps)
+        h, surf, curves = make_ctx(w)
+        _, _, lam = run(h, surf, curves, w, mode, args.warmup, LAMDA0)
+        secs, logs, lam = run(h, surf, curves, w, mode, args.steps, lam)
         mn, avg = h.scaled_jacobian()
+        X = h.get_positions()
+        bnd = np.concatenate([w["idsT"], w["idsL"]])
+        S, _, _, _ = surf.closest(X[bnd])
         which = 0 if mode == "fp64" else 1
         ms_apply = h.time_kernel(which, 20)
-        res[mode] = dict(secs=secs, logs=logs, minsj=mn, avgsj=avg, ms_apply=ms_apply, layout=h.layout_info())
-        print("%s: %.3f ms/iter, min SJ %.6f, apply tile kernel %.1f us, final energy %.9g" %
-              (mode, 1e3 * secs / args.steps, mn, 1e3 * ms_apply, logs[-1][0]), file=sys.stderr, flush=True)
+        res[mode] = dict(secs=secs, logs=logs, minsj=mn, avgsj=avg, ms_apply=ms_apply, maxdist=float(np.abs(S).max()),
+                         lam=lam, X=X)
+        print("%s: %.3f ms/iter, min SJ %.6f, max boundary distance %.3e, apply tile kernel %.1f us, "
+              "final energy %.9g, lamda %.3g, steps %s" %
+              (mode, 1e3 * secs / args.steps, mn, res[mode]["maxdist"], 1e3 * ms_apply, logs[-1][0], lam,
+               np.array2string(logs[:, 4], precision=3)), file=sys.stderr, flush=True)
         h.close()
+        surf.close()
+        curves.close()
 
     r32, r64 = res["fp32"], res["fp64"]
     rV = nV / nH
@@ -110,6 +155,7 @@ def main():
     bytes_per_hex32 = 32 + 192 + 2 * 12 * rV
     achieved = bytes_per_hex32 * nH / (r32["ms_apply"] * 1e-3) / 1e9
     value = args.steps / r32["secs"]
+    diag = float(np.linalg.norm(w["SV"].max(axis=0) - w["SV"].min(axis=0)))
     out = {
         "metric": "1M-hex optimiser iters/s: FP32+adjoint vs FP64 plain, equal min scaled-Jac",
         "value": round(value, 3),
@@ -123,15 +169,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (FP64 step bound, sign test, energy sums, CG dot products and residual)",
         "data": "synthetic",
-        "config": {"workload": "clipped torus R=1 r=0.35 z=+-0.2, uniform-level octree hex mesh",
-                   "hexes": nH, "vertices": nV, "boundary_vertices": int(bnd.size),
-                   "iteration": "compute_referenceMesh + 1 SLIM iteration (20-step CG, flip-avoiding line search)"},
+        "config": {"workload": "clipped torus R=1 r=0.35 z=+-0.2 (40k triangles, 4 rim feature curves), "
+                               "uniform-level octree hex mesh, feature-alignment loop",
+                   "hexes": nH, "vertices": nV, "plane_vertices": int(w["idsT"].size),
+                   "line_vertices": int(w["idsL"].size), "surface_triangles": int(w["SF"].shape[0]),
+                   "iteration": "compute_referenceMesh + plane/line projection + 1 SLIM iteration "
+                                "(20-step CG, flip-avoiding line search), weight_opt 0.001"},
         "fp64_plain": {"iterations_per_s": round(args.steps / r64["secs"], 3),
                        "ms_per_step": round(1e3 * r64["secs"] / args.steps, 4), "min_scaled_jacobian": r64["minsj"],
-                       "apply_tile_kernel_us": round(1e3 * r64["ms_apply"], 2)},
-        "fp32_adjoint": {"min_scaled_jacobian": r32["minsj"], "avg_scaled_jacobian": r32["avgsj"]},
+                       "max_boundary_distance": r64["maxdist"], "apply_tile_kernel_us": round(1e3 * r64["ms_apply"], 2)},
+        "fp32_adjoint": {"min_scaled_jacobian": r32["minsj"], "avg_scaled_jacobian": r32["avgsj"],
+                         "max_boundary_distance": r32["maxdist"],
+                         "max_vertex_diff_vs_fp64": float(np.abs(r32["X"] - r64["X"]).max())},
         "speedup_fp32_vs_fp64": round((args.steps / r32["secs"]) / (args.steps / r64["secs"]), 3),
         "min_sj_abs_diff": abs(r32["minsj"] - r64["minsj"]),
+        "max_boundary_distance_diff_over_bbox_diag": abs(r32["maxdist"] - r64["maxdist"]) / diag,
         "element_evals_per_s": round(value * nH, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -139,7 +191,7 @@ def main():
                      "bytes_per_hex": round(bytes_per_hex32, 2)},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(max(args.hexes // 16, 20000))
+        out["cpu_baseline"] = cpu_baseline(max(args.hexes // 8, 20000))
     print(json.dumps(out))
 
 

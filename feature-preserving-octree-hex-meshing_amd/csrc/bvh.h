@@ -41,6 +41,10 @@ constexpr int kBvhMaxDepth = 48;
 void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int stride, const int* ids, double* S, int* I,
                     double* C, double* N, double* sqrD, int* status, hipStream_t st);
 
+// (max, sum, count) of sqrt(sqrD[i]) over the queries below `upper`, into out3; part needs
+// 3 * 1024 doubles.
+void launch_metro_reduce(int n, const double* sqrD, double upper, double* part, double* out3, hipStream_t st);
+
 struct SoftDev;
 void launch_refold_planes(const SoftDev& sd, int nT, const double* Cp, double* t_d, const double* S0, const double* b0,
                           double* S, double* b, double lam, hipStream_t st);

@@ -8,6 +8,7 @@
 // chosen on ties is the reference's. The recursion becomes an explicit stack of packed
 // frames (node | visit state | visited-children bits) that lives in LDS, one column per lane.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "bvh.h"
 #include "kernels.h"
 
@@ -271,6 +272,39 @@ void launch_refold_planes(const SoftDev& sd, int nT, const double* Cp, double* t
   if (nT <= 0) return;
   hipLaunchKernelGGL(refold_planes_kernel, dim3((nT + 255) / 256), dim3(256), 0, st, sd.t_ids, sd.cidx, nT, Cp, sd.t_n,
                      t_d, S0, b0, S, b, lam);
+}
+
+// Vertex-sampled metro statistics of per-query squared distances: a query counts when its
+// distance is below `upper`; per block (max, sum, count), then one block folds the blocks
+// in order.
+__global__ void metro_reduce_kernel(int n, const double* __restrict__ sqrD, double upper, double* part) {
+  __shared__ double smax[256], ssum[256], scnt[256];
+  const int t = threadIdx.x;
+  double mx = -HUGE_VAL, sm = 0, cn = 0;
+  for (int i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x) {
+    const double d = sqrt(sqrD[i]);
+    if (d < upper) { mx = fmax(mx, d); sm += d; cn += 1; }
+  }
+  smax[t] = mx; ssum[t] = sm; scnt[t] = cn;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) { smax[t] = fmax(smax[t], smax[t + o]); ssum[t] += ssum[t + o]; scnt[t] += scnt[t + o]; }
+    __syncthreads();
+  }
+  if (t == 0) { part[3 * blockIdx.x] = smax[0]; part[3 * blockIdx.x + 1] = ssum[0]; part[3 * blockIdx.x + 2] = scnt[0]; }
+}
+
+__global__ void metro_fin_kernel(int nb, const double* part, double* out) {
+  if (threadIdx.x != 0) return;
+  double mx = -HUGE_VAL, sm = 0, cn = 0;
+  for (int b = 0; b < nb; ++b) { mx = fmax(mx, part[3 * b]); sm += part[3 * b + 1]; cn += part[3 * b + 2]; }
+  out[0] = mx; out[1] = sm; out[2] = cn;
+}
+
+void launch_metro_reduce(int n, const double* sqrD, double upper, double* part, double* out3, hipStream_t st) {
+  const int nb = std::max(1, std::min((n + 255) / 256, 1024));
+  hipLaunchKernelGGL(metro_reduce_kernel, dim3(nb), dim3(256), 0, st, n, sqrD, upper, part);
+  hipLaunchKernelGGL(metro_fin_kernel, dim3(1), dim3(64), 0, st, nb, (const double*)part, out3);
 }
 
 void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int stride, const int* ids, double* S, int* I,

@@ -40,9 +40,12 @@ template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrt
 // One-sided Jacobi SVD of a 3x3 matrix A = U diag(s) V^T, s >= 0 descending, U, V
 // orthogonal (the contract of Eigen::JacobiSVD that igl::polar_svd relies on). Column pairs
 // of B = A V are rotated until orthogonal; the column norms are the singular values.
+// Convergence: a sweep that finds every column pair orthogonal to `tol` (relative) ends the
+// iteration. In FP32 the measure settles at rounding level (~1e-7), so its tolerance sits
+// just above that; with quadratic convergence the last rotating sweep already reached it.
 template <class T>
 __device__ void svd3(const T A[3][3], T U[3][3], T s[3], T V[3][3]) {
-  const T tol = sizeof(T) == 8 ? T(1e-17) : T(1e-9);
+  const T tol = sizeof(T) == 8 ? T(1e-17) : T(1e-6);
   const int max_sweeps = sizeof(T) == 8 ? 12 : 8;
   T B[3][3];
 #pragma unroll

@@ -54,6 +54,9 @@ struct hexopt_ctx {
   std::string err;
   // layout on the device
   DevBuf d_conn, d_tile_hex, d_tile_col, d_tile_lv, d_lv_gid, d_lv_slot, d_lidx, d_sv_gid, d_sv_off, d_sv_slots;
+  DevBuf d_sidx, d_iso;  // vertex -> fixup index (-1: written by its tile); vertices without hexes
+  DevBuf d_cgsave;       // CG state of the last direction solve, read back with the line search
+  int n_iso = 0;
   TileMesh tm{};
   // positions (SoA, Morton numbering)
   DevBuf d_rest, d_cur;
@@ -73,8 +76,9 @@ struct hexopt_ctx {
   // tangent-plane refresh: folded rows without the plane terms
   DevBuf d_S0, d_b0;
   bool planes_refreshable = false;
+  bool lines_refreshable = false;
   // FP32 optimiser state
-  DevBuf f_K, f_rhs, f_invd, f_d, f_r, f_z, f_p, f_tmp, f_e, f_halo, d_Ld64, d_halo64b, d_ninv;
+  DevBuf f_K, f_rhs, f_invd, f_d, f_r, f_z, f_p, f_p2, f_tmp, f_e, f_halo, d_Ld64, d_halo64b, d_ninv;
 };
 
 namespace {
@@ -124,7 +128,8 @@ int ensure_state(hexopt_ctx* c) {
     HIPCHK(c, b->ensure(n * sizeof(double)));
   HIPCHK(c, c->d_halo.ensure((size_t)6 * (c->L.nSlots + 1) * sizeof(double)));
   HIPCHK(c, c->d_cg.ensure(cg_state_bytes()));
-  const size_t np = (size_t)std::max(std::max(std::max(c->L.nTiles, nblocks(n)), (nH + 255) / 256 + 1), 2048) + 64;
+  const size_t np = (size_t)std::max(std::max(std::max(c->L.nTiles, nblocks(n)), (nH + 255) / 256 + 1),
+                                     std::max(2048, (nV + c->nL + 255) / 256 + 1)) + 64;
   const size_t npf = (size_t)(c->L.nShared + 255) / 256 + 1;
   for (int i = 0; i < 8; ++i) HIPCHK(c, c->d_part[i].ensure(std::max(np, npf) * sizeof(double)));
   HIPCHK(c, c->d_scal.ensure(16 * sizeof(double)));
@@ -185,6 +190,16 @@ hexopt_ctx* hexopt_create(int nV, const double* V, int nH, const int32_t* hexes,
   rc |= upload(c, c->d_sv_gid, L.sv_gid);
   rc |= upload(c, c->d_sv_off, L.sv_off);
   rc |= upload(c, c->d_sv_slots, L.sv_slots);
+  {
+    std::vector<int> sidx(L.nV, -1), iso;
+    for (int i = 0; i < L.nShared; ++i) {
+      sidx[L.sv_gid[i]] = i;
+      if (L.sv_off[i + 1] == L.sv_off[i]) iso.push_back(L.sv_gid[i]);
+    }
+    c->n_iso = (int)iso.size();
+    rc |= upload(c, c->d_sidx, sidx);
+    rc |= upload(c, c->d_iso, iso);
+  }
   std::vector<double> soa;
   to_soa(L, V, soa);
   rc |= upload(c, c->d_rest, soa);
@@ -364,6 +379,13 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
   rc |= upload(c, c->d_S0, S0_pre);
   rc |= upload(c, c->d_b0, b0_pre);
   c->planes_refreshable = !dupT;
+  {
+    std::vector<char> other(nV, 0);
+    for (int i = 0; i < cs->nC; ++i) other[L.vinv[cs->ids_C[i]]] = 1;
+    for (int i = 0; i < cs->nT; ++i) other[L.vinv[cs->ids_T[i]]] = 1;
+    c->lines_refreshable = true;
+    for (int i = 0; i < cs->nL; ++i) c->lines_refreshable &= !other[L.vinv[cs->ids_L[i]]];
+  }
   rc |= upload(c, c->d_S, S);
   rc |= upload(c, c->d_b, b);
   rc |= upload(c, c->d_cline, cline);
@@ -393,9 +415,9 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
   return HEXOPT_OK;
 }
 
-// Adds/removes the projection rows (lamda_projection |x_b - bc|^2) to the folded S/b rows.
-static int fold_projection(hexopt_ctx* c, const hexopt_params* prm, std::vector<int>& saved_cidx_dummy) {
-  (void)saved_cidx_dummy;
+// The projection rows (ts.projection: lamda_projection |x_b - bc|^2, slim_m.cpp:600-609)
+// belong to the padding optimisers, which this build does not run.
+static int check_projection(hexopt_ctx* c, const hexopt_params* prm) {
   if (!prm->projection || c->sd.nb == 0) return HEXOPT_OK;
   return fail(c, HEXOPT_EUNSUPPORTED, "projection constraints (ts.projection) are not supported by this build");
 }
@@ -405,8 +427,7 @@ int hexopt_slim_solve(hexopt_ctx* c, const hexopt_params* prm, int iters, hexopt
   if (!c || !prm || iters < 0) return HEXOPT_EINVAL;
   if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");
   if (prm->energy < 0 || prm->energy > 5) return fail(c, HEXOPT_EINVAL, "unknown energy");
-  std::vector<int> dummy;
-  int rc = fold_projection(c, prm, dummy);
+  int rc = check_projection(c, prm);
   if (rc != HEXOPT_OK) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   if ((rc = ensure_state(c)) != HEXOPT_OK) return rc;
@@ -423,8 +444,6 @@ int hexopt_slim_solve(hexopt_ctx* c, const hexopt_params* prm, int iters, hexopt
   double eq = 0, es = 0;
   if ((rc = energy_now(c, prm, nullptr, 0.0, aL, &eq, &es)) != HEXOPT_OK) return rc;
   double energy = (eq + es) / mesh_area;
-  hexopt_params pl = *prm;
-  (void)pl;
   const int nb = nblocks(n);
   double* x = c->d_x.as<double>();
   double* cur = c->d_cur.as<double>();
@@ -561,39 +580,37 @@ static int ensure_state32(hexopt_ctx* c) {
   const int nH = c->L.nH;
   const size_t n = (size_t)3 * c->L.nV + c->nL;
   HIPCHK(c, c->f_K.ensure((size_t)48 * nH * sizeof(float)));
-  for (DevBuf* b : {&c->f_rhs, &c->f_invd, &c->f_d, &c->f_r, &c->f_z, &c->f_p, &c->f_tmp, &c->f_e})
+  for (DevBuf* b : {&c->f_rhs, &c->f_invd, &c->f_d, &c->f_r, &c->f_z, &c->f_p, &c->f_p2, &c->f_tmp, &c->f_e})
     HIPCHK(c, b->ensure(n * sizeof(float)));
   HIPCHK(c, c->f_halo.ensure((size_t)6 * (c->L.nSlots + 1) * sizeof(float)));
   HIPCHK(c, c->d_Ld64.ensure(n * sizeof(double)));
   HIPCHK(c, c->d_ninv.ensure(sizeof(int)));
+  HIPCHK(c, c->d_cgsave.ensure(cg_state_bytes()));
   return HEXOPT_OK;
 }
 
-// FP32 CG on L x = b from x = 0 (Eigen's algorithm, FP64 dot products), `iters` iterations.
+// FP32 CG on L x = b from x = 0 (Eigen's algorithm, FP64 dot products), `iters` iterations,
+// two or three fused launches per iteration and no host round trip. Uses f_r, f_z, f_p,
+// f_p2 (the two direction buffers), f_tmp and f_halo.
 static int cg32(hexopt_ctx* c, const SoftDev& sd, double prox, const float* b, float* x, int iters, int* it_out,
                 double* err_out) {
   const int nV = c->L.nV, nL = c->nL;
   const int n = 3 * nV + nL;
-  const int nb = nblocks(n);
+  if (c->L.nTiles == 0) return fail(c, HEXOPT_EINVAL, "mesh without hexes");
   hipStream_t st = c->stream;
   RefOp ro = refop(c);
-  HIPCHK(c, hipMemsetAsync(x, 0, (size_t)n * sizeof(float), st));
-  HIPCHK(c, hipMemsetAsync(c->f_tmp.p, 0, (size_t)n * sizeof(float), st));
-  launch_cg_init32(n, b, c->f_tmp.as<float>(), c->f_r.as<float>(), c->d_part[0].as<double>(), c->d_part[1].as<double>(),
-                   nb, c->d_cg.p, c->f_invd.as<float>(), c->f_p.as<float>(), c->d_part[5].as<double>(), 1e-8, st);
-  const int nfix = (c->L.nShared + 255) / 256, nline = (nL + 255) / 256;
-  double* pt = c->d_part[2].as<double>();
-  double* pf = c->d_part[3].as<double>();
-  double* pl = c->d_part[4].as<double>();
-  for (int k = 0; k < iters; ++k) {
-    launch_apply32(c->tm, ro, c->f_K.as<float>(), c->f_p.as<float>(), c->f_tmp.as<float>(), prox, sd,
-                   c->f_halo.as<float>(), pt, pf, pl, st);
-    launch_cg_alpha(c->d_cg.p, pt, c->L.nTiles, pf, nfix, pl, nline, st);
-    launch_cg_update32(n, c->d_cg.p, x, c->f_r.as<float>(), c->f_z.as<float>(), c->f_p.as<float>(),
-                       c->f_tmp.as<float>(), c->f_invd.as<float>(), c->d_part[6].as<double>(),
-                       c->d_part[7].as<double>(), nb, c->f_p.as<float>(), st);
-  }
-  launch_cg_zero_if32(n, c->d_cg.p, x, nb, st);
+  HIPCHK(c, hipMemsetAsync(c->d_cg.p, 0, cg_state_bytes(), st));
+  float* P[2] = {c->f_p.as<float>(), c->f_p2.as<float>()};
+  float* z = c->f_z.as<float>();
+  float* r = c->f_r.as<float>();
+  launch_cg32_init(n, b, c->f_invd.as<float>(), x, r, z, P[0], c->d_cg.p, c->d_part[0].as<double>(),
+                   c->d_part[1].as<double>(), 1e-8, st);
+  for (int k = 0; k < iters; ++k)
+    launch_cg32_iteration(c->tm, ro, c->f_K.as<float>(), prox, sd, c->d_sidx.as<int>(), c->d_iso.as<int>(), c->n_iso, z,
+                          P[k & 1], P[(k + 1) & 1], c->f_tmp.as<float>(), c->f_halo.as<float>(), c->f_invd.as<float>(),
+                          x, r, z, c->d_cg.p, c->d_part[2].as<double>(), c->d_part[3].as<double>(),
+                          c->d_part[4].as<double>(), c->d_part[5].as<double>(), st);
+  HIPCHK(c, hipGetLastError());
   if (it_out || err_out) {
     std::vector<unsigned char> cgs(cg_state_bytes());
     HIPCHK(c, hipMemcpyAsync(cgs.data(), c->d_cg.p, cgs.size(), hipMemcpyDeviceToHost, st));
@@ -635,57 +652,44 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
     launch_adj_local32(c->tm, ro, cur, c->f_K.as<float>(), M, prox, sd, c->f_rhs.as<float>(), c->f_invd.as<float>(),
                        c->f_halo.as<float>(), c->d_part[0].as<double>(), c->d_ninv.as<int>(), st);
     // direction: L d = -dE/dx / 2 by FP32 CG, then one FP64-residual refinement solve
-    int cg_it = 0;
-    double cg_err = 0;
     float* d = c->f_d.as<float>();
-    if ((rc = cg32(c, sd, prox, c->f_rhs.as<float>(), d, opt->cg_iters, &cg_it, &cg_err))) return rc;
+    if ((rc = cg32(c, sd, prox, c->f_rhs.as<float>(), d, opt->cg_iters, nullptr, nullptr))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_cgsave.p, c->d_cg.p, cg_state_bytes(), hipMemcpyDeviceToDevice, st));
     launch_apply32_f64(c->tm, ro, c->f_K.as<float>(), d, c->d_Ld64.as<double>(), prox, sd, c->d_halo.as<double>(), st);
     const int nb = nblocks(n);
     launch_residual64(n, c->f_rhs.as<float>(), c->d_Ld64.as<double>(), c->f_r.as<float>(), c->d_part[1].as<double>(),
                       nb, scal + 6, st);
-    launch_sum_partials(c->d_part[0].as<double>(), c->L.nTiles, scal + 8, st);
-    double h3[3];
-    HIPCHK(c, hipMemcpyAsync(h3, scal + 6, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    double res64 = h3[0];
-    // |b| of the direction system from the CG state's rhs norm (FP64 sum of the FP32 rhs)
-    double rhs_norm = 0;
-    {
-      std::vector<unsigned char> cgs(cg_state_bytes());
-      HIPCHK(c, hipMemcpy(cgs.data(), c->d_cg.p, cgs.size(), hipMemcpyDeviceToHost));
-      rhs_norm = std::sqrt(*(const double*)cgs.data());
-    }
     if (opt->refine_iters > 0) {
-      // residual r (FP32 copy of the FP64 residual) -> correction e, d += e
-      HIPCHK(c, hipMemcpyAsync(c->f_z.p, c->f_r.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      // FP32 copy of the FP64 residual -> correction e, d += e (the copy keeps the right-hand
+      // side apart from the CG's own r, z and p)
+      HIPCHK(c, hipMemcpyAsync(c->d_Ld64.p, c->f_r.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
       float* e = c->f_e.as<float>();
-      // cg32 overwrites f_r/f_p/f_tmp; the right-hand side lives in f_z only until cg_init copies it
-      HIPCHK(c, hipMemcpyAsync(c->d_Ld64.p, c->f_z.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
       if ((rc = cg32(c, sd, prox, (const float*)c->d_Ld64.p, e, opt->refine_iters, nullptr, nullptr))) return rc;
       launch_add_inplace(n, d, e, nb, st);
     }
     // a_L is the line unknowns' solution (as slim_m.cpp:501-503), used by the soft energy
     double* aL = c->d_aL.as<double>();
-    if (nL > 0) {
-      std::vector<float> ha(nL);
-      std::vector<double> hd(nL);
-      HIPCHK(c, hipMemcpyAsync(ha.data(), d + 3 * (size_t)nV, nL * sizeof(float), hipMemcpyDeviceToHost, st));
-      HIPCHK(c, hipStreamSynchronize(st));
-      for (int i = 0; i < nL; ++i) hd[i] = ha[i];
-      HIPCHK(c, hipMemcpyAsync(aL, hd.data(), nL * sizeof(double), hipMemcpyHostToDevice, st));
-    }
+    if (nL > 0) launch_f2d(nL, d + 3 * (size_t)nV, aL, st);
     // FP64 flip-free bound, then the reference's bisection on FP32 energies summed in FP64
     launch_max_step_dir32(cur, d, nV, c->tm.conn, nH, c->d_part[0].as<double>(), scal + 4, st);
-    double tmin = 0;
-    int ninv = 0;
-    HIPCHK(c, hipMemcpyAsync(&tmin, scal + 4, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipMemcpyAsync(&ninv, c->d_ninv.p, sizeof(int), hipMemcpyDeviceToHost, st));
     launch_energy_dir32(ro, cur, d, 0.0, nV, c->tm.conn, nH, M, sd, aL, c->d_part[0].as<double>(),
                         c->d_part[1].as<double>(), nsb, scal, st);
-    double h2[2];
-    HIPCHK(c, hipMemcpyAsync(h2, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    double hs[8];
+    int ninv = 0;
+    std::vector<unsigned char> cgs(cg_state_bytes());
+    HIPCHK(c, hipMemcpyAsync(hs, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(&ninv, c->d_ninv.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(cgs.data(), c->d_cgsave.p, cgs.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(st));
+    int cg_it = 0, cg_done = 0;
+    double cg_err = 0;
+    read_cg_state(cgs.data(), &cg_it, &cg_err, &cg_done);
+    // |b| of the direction system: the CG state's rhs norm (FP64 sum of the FP32 rhs)
+    const double rhs_norm = std::sqrt(*(const double*)cgs.data());
+    const double res64 = hs[6];
+    const double tmin = hs[4];
+    double h2[2] = {hs[0], hs[1]};
     const double old_energy = h2[0] + h2[1];
     double step = std::min(1., tmin * 0.8);
     const double bound = step;
@@ -886,32 +890,26 @@ int hexopt_surface_normals(hexopt_surface* s, double* FN, double* VN, double* FE
 int hexopt_surface_closest(hexopt_surface* s, int nq, const double* P, double* S, int32_t* I, double* C, double* N) {
   if (!s || nq < 0 || (nq > 0 && !P)) return HEXOPT_EINVAL;
   if (nq == 0) return HEXOPT_OK;
-  hexopt_ctx dummy;  // error plumbing for HIPCHK
-  auto fin = [&](int rc) {
-    if (rc) s->err = dummy.err;
-    dummy.stream = nullptr;
-    return rc;
-  };
   hipStream_t st = s->stream;
-  if (s->dP.ensure((size_t)nq * 3 * 8) || s->dS.ensure((size_t)nq * 8) || s->dI.ensure((size_t)nq * 4) ||
-      s->dC.ensure((size_t)nq * 24) || s->dN.ensure((size_t)nq * 24))
-    return fin(fail(&dummy, HEXOPT_EHIP, "hipMalloc failed"));
-  if (hipMemcpyAsync(s->dP.p, P, (size_t)nq * 24, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemsetAsync(s->dstat.p, 0, sizeof(int), st) != hipSuccess)
-    return fin(fail(&dummy, HEXOPT_EHIP, "copy failed"));
+  HIPCHK(s, s->dP.ensure((size_t)nq * 24));
+  HIPCHK(s, s->dS.ensure((size_t)nq * 8));
+  HIPCHK(s, s->dI.ensure((size_t)nq * 4));
+  HIPCHK(s, s->dC.ensure((size_t)nq * 24));
+  HIPCHK(s, s->dN.ensure((size_t)nq * 24));
+  HIPCHK(s, hipMemcpyAsync(s->dP.p, P, (size_t)nq * 24, hipMemcpyHostToDevice, st));
+  HIPCHK(s, hipMemsetAsync(s->dstat.p, 0, sizeof(int), st));
   launch_closest(s->view, nq, s->dP.as<double>(), 0, 0, nullptr, s->dS.as<double>(), s->dI.as<int>(), s->dC.as<double>(),
                  s->dN.as<double>(), nullptr, s->dstat.as<int>(), st);
+  HIPCHK(s, hipGetLastError());
   int stat = 0;
-  bool ok = hipGetLastError() == hipSuccess;
-  ok = ok && hipMemcpyAsync(&stat, s->dstat.p, sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess;
-  if (S) ok = ok && hipMemcpyAsync(S, s->dS.p, (size_t)nq * 8, hipMemcpyDeviceToHost, st) == hipSuccess;
-  if (I) ok = ok && hipMemcpyAsync(I, s->dI.p, (size_t)nq * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
-  if (C) ok = ok && hipMemcpyAsync(C, s->dC.p, (size_t)nq * 24, hipMemcpyDeviceToHost, st) == hipSuccess;
-  if (N) ok = ok && hipMemcpyAsync(N, s->dN.p, (size_t)nq * 24, hipMemcpyDeviceToHost, st) == hipSuccess;
-  ok = ok && hipStreamSynchronize(st) == hipSuccess;
-  if (!ok) return fin(fail(&dummy, HEXOPT_EHIP, "closest-point kernel failed"));
-  if (stat) return fin(fail(&dummy, HEXOPT_EUNSUPPORTED, "traversal stack overflow"));
-  return fin(HEXOPT_OK);
+  HIPCHK(s, hipMemcpyAsync(&stat, s->dstat.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  if (S) HIPCHK(s, hipMemcpyAsync(S, s->dS.p, (size_t)nq * 8, hipMemcpyDeviceToHost, st));
+  if (I) HIPCHK(s, hipMemcpyAsync(I, s->dI.p, (size_t)nq * 4, hipMemcpyDeviceToHost, st));
+  if (C) HIPCHK(s, hipMemcpyAsync(C, s->dC.p, (size_t)nq * 24, hipMemcpyDeviceToHost, st));
+  if (N) HIPCHK(s, hipMemcpyAsync(N, s->dN.p, (size_t)nq * 24, hipMemcpyDeviceToHost, st));
+  HIPCHK(s, hipStreamSynchronize(st));
+  if (stat) return (s->err = "traversal stack overflow", HEXOPT_EUNSUPPORTED);
+  return HEXOPT_OK;
 }
 
 int hexopt_project_planes(hexopt_ctx* c, hexopt_surface* s, double lamda_T) {
@@ -1128,6 +1126,99 @@ int hexopt_project_lines(hexopt_ctx* c, hexopt_curves* cv, int which, int nL, co
   HIPCHK(c, hipMemcpyAsync(origin_L, cv->dorig.p, (size_t)nL * 24, hipMemcpyDeviceToHost, st));
   HIPCHK(c, hipMemcpyAsync(Axa_L, cv->daxis.p, (size_t)nL * 24, hipMemcpyDeviceToHost, st));
   HIPCHK(c, hipStreamSynchronize(st));
+  return HEXOPT_OK;
+}
+
+int hexopt_refresh_lines(hexopt_ctx* c, hexopt_curves* cv, const int32_t* curve_L, double lamda_L) {
+  if (!c || !cv) return HEXOPT_EINVAL;
+  const int nL = c->sd.nL;
+  if (nL == 0) return HEXOPT_OK;
+  if (!curve_L) return HEXOPT_EINVAL;
+  if (cv->device != c->device) return fail(c, HEXOPT_EINVAL, "curves and context on different devices");
+  if (!c->lines_refreshable) return fail(c, HEXOPT_EUNSUPPORTED, "line vertices with corner or plane terms");
+  for (int i = 0; i < nL; ++i)
+    if (curve_L[i] < 0 || curve_L[i] >= cv->ncurves) return fail(c, HEXOPT_EINVAL, "curve_L out of range");
+  hipStream_t st = c->stream;
+  HIPCHK(c, cv->dcid.ensure((size_t)nL * 4));
+  HIPCHK(c, cv->dorig.ensure((size_t)nL * 24));
+  HIPCHK(c, cv->daxis.ensure((size_t)nL * 24));
+  HIPCHK(c, hipMemcpyAsync(cv->dcid.p, curve_L, (size_t)nL * 4, hipMemcpyHostToDevice, st));
+  CurvesDev view{cv->dV.as<double>(), cv->doff.as<int>(), cv->dvs.as<int>(), cv->dcirc.as<uint8_t>()};
+  launch_project_lines(view, c->d_cur.as<double>(), c->L.nV, nL, c->sd.lv, cv->dcid.as<int>(), cv->dorig.as<double>(),
+                       cv->daxis.as<double>(), st);
+  c->sd.lamL = lamda_L;
+  launch_refold_lines(c->sd, cv->dorig.as<double>(), cv->daxis.as<double>(), c->d_S0.as<double>(),
+                      c->d_b0.as<double>(), c->d_rhs_a.as<double>(), c->d_invd_a.as<double>(), st);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(st));
+  return HEXOPT_OK;
+}
+
+// One metro direction: every face-referenced vertex of (V, F) against surface `to`.
+static int metro_direction(hexopt_surface* to, int nv, const double* V, int nf, const int32_t* F, double upper,
+                           double out3[3]) {
+  std::vector<char> ref(nv, 0);
+  for (long i = 0; i < 3L * nf; ++i) ref[F[i]] = 1;
+  std::vector<double> P;
+  P.reserve((size_t)nv * 3);
+  for (int v = 0; v < nv; ++v)
+    if (ref[v]) P.insert(P.end(), V + (size_t)v * 3, V + (size_t)v * 3 + 3);
+  const int nq = (int)(P.size() / 3);
+  hipStream_t st = to->stream;
+  HIPCHK(to, to->dP.ensure(P.size() * 8 + 8));
+  HIPCHK(to, to->dS.ensure((size_t)nq * 8 + 8));
+  HIPCHK(to, to->dC.ensure((size_t)3 * 1024 * 8 + 3 * 8));
+  HIPCHK(to, hipMemcpyAsync(to->dP.p, P.data(), P.size() * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(to, hipMemsetAsync(to->dstat.p, 0, sizeof(int), st));
+  launch_closest(to->view, nq, to->dP.as<double>(), 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                 to->dS.as<double>(), to->dstat.as<int>(), st);
+  double* red = to->dC.as<double>();
+  launch_metro_reduce(nq, to->dS.as<double>(), upper, red, red + 3 * 1024, st);
+  HIPCHK(to, hipGetLastError());
+  int stat = 0;
+  HIPCHK(to, hipMemcpyAsync(out3, red + 3 * 1024, 3 * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(to, hipMemcpyAsync(&stat, to->dstat.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(to, hipStreamSynchronize(st));
+  if (stat) return (to->err = "traversal stack overflow", HEXOPT_EUNSUPPORTED);
+  return HEXOPT_OK;
+}
+
+int hexopt_metro_hausdorff(int nv0, const double* V0, int nf0, const int32_t* F0, int nv1, const double* V1, int nf1,
+                           const int32_t* F1, int device, double* bbox_diagonal, double* max_dist, double* mean_dist) {
+  g_surface_error.clear();
+  if (nv0 <= 0 || nf0 <= 0 || nv1 <= 0 || nf1 <= 0 || !V0 || !F0 || !V1 || !F1) return HEXOPT_EINVAL;
+  // box of all vertices of both meshes, grown by 2% of its diagonal on every side
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int m = 0; m < 2; ++m) {
+    const double* V = m ? V1 : V0;
+    const int nv = m ? nv1 : nv0;
+    for (int i = 0; i < nv; ++i)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], V[(size_t)i * 3 + d]);
+        hi[d] = std::max(hi[d], V[(size_t)i * 3 + d]);
+      }
+  }
+  auto diag = [&]() {
+    const double a = hi[0] - lo[0], b = hi[1] - lo[1], c = hi[2] - lo[2];
+    return std::sqrt(a * a + b * b + c * c);
+  };
+  const double off = diag() * 0.02;
+  for (int d = 0; d < 3; ++d) { lo[d] -= off; hi[d] += off; }
+  const double D = diag();
+  hexopt_surface* s0 = hexopt_surface_create(nv0, V0, nf0, F0, nullptr, nullptr, nullptr, device);
+  if (!s0) return HEXOPT_EINVAL;
+  hexopt_surface* s1 = hexopt_surface_create(nv1, V1, nf1, F1, nullptr, nullptr, nullptr, device);
+  if (!s1) { hexopt_surface_destroy(s0); return HEXOPT_EINVAL; }
+  double fwd[3] = {0, 0, 0}, bwd[3] = {0, 0, 0};
+  int rc = metro_direction(s1, nv0, V0, nf0, F0, D, fwd);
+  if (rc == HEXOPT_OK) rc = metro_direction(s0, nv1, V1, nf1, F1, D, bwd);
+  if (rc != HEXOPT_OK) g_surface_error = s0->err.empty() ? s1->err : s0->err;
+  hexopt_surface_destroy(s0);
+  hexopt_surface_destroy(s1);
+  if (rc != HEXOPT_OK) return rc;
+  if (bbox_diagonal) *bbox_diagonal = D;
+  if (max_dist) *max_dist = std::max(fwd[0], bwd[0]);
+  if (mean_dist) *mean_dist = std::max(bwd[1] / bwd[2], fwd[1] / fwd[2]);
   return HEXOPT_OK;
 }
 

@@ -89,8 +89,20 @@ void launch_cg_init32(int n, const float* rhs, const float* tmp, float* r, doubl
 void launch_cg_update32(int n, void* state, float* x, float* r, float* z, const float* p, const float* tmp,
                         const float* invd, double* prr, double* prz, int nb, float* p_out, hipStream_t st);
 void launch_cg_zero_if32(int n, void* state, float* x, int nb, hipStream_t st);
+// Fused FP32 CG (Eigen's algorithm, FP64 dot products) on the CG operator of launch_apply32:
+// launch_cg32_init sets x = 0, r = b, z = b / diag, p_prev = 0 and the state; each
+// launch_cg32_iteration is one CG iteration reading z and p_prev, writing p_new and the
+// updated x, r, z (zout may alias z). sidx[v] is the fixup index of vertex v (-1 when its
+// tile writes it); iso lists the vertices without hexes.
+void launch_cg32_init(int n, const float* b, const float* invd, float* x, float* r, float* z, float* pprev, void* state,
+                      double* part_b, double* part_rz, double tol, hipStream_t st);
+void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
+                           const int* sidx, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
+                           float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
+                           double* part_tile, double* part_line, double* part_rr, double* part_rz, hipStream_t st);
 void launch_residual64(int n, const float* b, const double* Ld, float* r32, double* part, int nb, double* out,
                        hipStream_t st);
+void launch_f2d(int n, const float* a, double* b, hipStream_t st);
 void launch_add_inplace(int n, float* d, const float* e, int nb, hipStream_t st);
 void launch_max_step_dir32(const double* cur, const float* d, int nV, const int* conn, int nH, double* part,
                            double* out, hipStream_t st);

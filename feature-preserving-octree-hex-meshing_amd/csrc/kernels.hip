@@ -12,6 +12,7 @@
 //   connectivity    conn[k*nH + h]         k = hex corner in the reference's Hybrid::vs order
 //   per-tet scalars f[(c*F + j)*nH + h]    c = corner tet, j = field component
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 #include "device_math.h"
 #include "kernels.h"
@@ -458,6 +459,43 @@ struct LocalStepOp {
 
 // CG operator L p = A^T M A p + prox p + soft terms; finish() also returns p.(Lp).
 // KT: stored K precision, PT: vector precision, CT: arithmetic, OT: output precision.
+// Element part of the CG operator for hex h: corner forces o (added) from the corner
+// values q of the direction, with the per-tet K = invs^2 M W W^T (6 symmetric components).
+template <class KT, class CT>
+__device__ __forceinline__ void hex_forces(const RefOp& ro, const KT* K, int nH, int h, const CT q[8][3], CT o[8][3]) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    CT E[3][3], B[3][3], Y[3][3];
+    tet_edges(q, c, E);
+    const KT* kk = K + (size_t)c * 6 * nH + h;
+    const CT k00 = (CT)kk[0], k11 = (CT)kk[(size_t)nH], k22 = (CT)kk[2 * (size_t)nH], k01 = (CT)kk[3 * (size_t)nH],
+             k02 = (CT)kk[4 * (size_t)nH], k12 = (CT)kk[5 * (size_t)nH];
+    if (ro.mode == REF_CUBE) {
+      // invs^2 is folded into K: forces on the tet edges are K E (columns = edges)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const CT f0 = k00 * E[0][k] + k01 * E[1][k] + k02 * E[2][k];
+        const CT f1 = k01 * E[0][k] + k11 * E[1][k] + k12 * E[2][k];
+        const CT f2 = k02 * E[0][k] + k12 * E[1][k] + k22 * E[2][k];
+        const int vk = kHexTet[c][k + 1], v0 = kHexTet[c][0];
+        o[vk][0] += f0; o[vk][1] += f1; o[vk][2] += f2;
+        o[v0][0] -= f0; o[v0][1] -= f1; o[v0][2] -= f2;
+      }
+    } else {
+      tet_B(ro, nH, h, c, B);
+      CT J[3][3];
+      matmul3(E, B, J);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        Y[0][j] = k00 * J[0][j] + k01 * J[1][j] + k02 * J[2][j];
+        Y[1][j] = k01 * J[0][j] + k11 * J[1][j] + k12 * J[2][j];
+        Y[2][j] = k02 * J[0][j] + k12 * J[1][j] + k22 * J[2][j];
+      }
+      scatter_tet(Y, B, c, o);
+    }
+  }
+}
+
 template <class KT, class PT, class CT, class OT>
 struct ApplyOp {
   RefOp ro; int nV, nH; const int* conn; const KT* K; const PT* p; OT* out; double prox;
@@ -470,37 +508,7 @@ struct ApplyOp {
 #pragma unroll
       for (int d = 0; d < 3; ++d) q[k][d] = (CT)p[(size_t)d * nV + v];
     }
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      CT E[3][3], B[3][3], Y[3][3];
-      tet_edges(q, c, E);
-      const KT* kk = K + (size_t)c * 6 * nH + h;
-      const CT k00 = (CT)kk[0], k11 = (CT)kk[(size_t)nH], k22 = (CT)kk[2 * (size_t)nH], k01 = (CT)kk[3 * (size_t)nH],
-               k02 = (CT)kk[4 * (size_t)nH], k12 = (CT)kk[5 * (size_t)nH];
-      if (ro.mode == REF_CUBE) {
-        // invs^2 is folded into K: forces on the tet edges are K E (columns = edges)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const CT f0 = k00 * E[0][k] + k01 * E[1][k] + k02 * E[2][k];
-          const CT f1 = k01 * E[0][k] + k11 * E[1][k] + k12 * E[2][k];
-          const CT f2 = k02 * E[0][k] + k12 * E[1][k] + k22 * E[2][k];
-          const int vk = kHexTet[c][k + 1], v0 = kHexTet[c][0];
-          o[vk][0] += f0; o[vk][1] += f1; o[vk][2] += f2;
-          o[v0][0] -= f0; o[v0][1] -= f1; o[v0][2] -= f2;
-        }
-      } else {
-        tet_B(ro, nH, h, c, B);
-        CT J[3][3];
-        matmul3(E, B, J);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          Y[0][j] = k00 * J[0][j] + k01 * J[1][j] + k02 * J[2][j];
-          Y[1][j] = k01 * J[0][j] + k11 * J[1][j] + k12 * J[2][j];
-          Y[2][j] = k02 * J[0][j] + k12 * J[1][j] + k22 * J[2][j];
-        }
-        scatter_tet(Y, B, c, o);
-      }
-    }
+    hex_forces<KT, CT>(ro, K, nH, h, q, o);
     return 0.0;
   }
   __device__ double finish(int g, CT acc[3]) const {
@@ -660,6 +668,9 @@ __global__ void line_apply_kernel(SoftDev sd, int nV, const PT* p, const PT* pa,
 struct CGState {
   double rhsNorm2, threshold, absNew, alpha, beta, rn2, tol_error;
   int done, iters, zero_rhs, pad;
+  // fused FP32 CG: arrival counters of the last-block reductions and p.Lp of the tiles
+  unsigned int ticket_init, ticket_apply, ticket_lines, ticket_update;
+  double pLp_tiles;
 };
 
 // r = rhs - tmp, partial |rhs|^2, |r|^2
@@ -786,6 +797,346 @@ __global__ void cg_zero_if_kernel(int n, const CGState* st, T* x) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// FP32 CG with the vector work fused into the operator pass (the FP32 optimiser's global
+// step, Eigen's algorithm with FP64 dot products). Per CG iteration:
+//   cg32_apply   per tile: p = z + beta p_prev staged in LDS (p double-buffered, each vertex
+//                written once: by its own tile, the first tile of its halo slots, or the
+//                reducing block for vertices without hexes), L p of the tile's elements,
+//                p.Lp from the element quadratic forms p_e.(K_e p_e) plus the vertex rows;
+//                the last block to arrive reduces the tile partials in tile order and sets
+//                alpha (or leaves p.Lp to the line kernel)
+//   cg32_lines   a_L rows of L p (feature lines only); its last block sets alpha
+//   cg32_update  L p of halo vertices from their slots, x += alpha p, r -= alpha L p,
+//                z = r / diag, FP64 |r|^2 and r.z; its last block sets beta and the
+//                convergence flag
+// Last-block reductions: every block writes its partial, fences, and takes a ticket from an
+// arrival counter; the block that draws the final ticket fences, reduces all partials in a
+// fixed order and resets the counter. No block ever waits for another.
+
+// rows of L p beyond the elements at vertex g: prox p + S p - lamL t a
+__device__ __forceinline__ void vertex_rows(const SoftDev& sd, double prox, int ci, int li, const double pv[3], double pa,
+                                            double r[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) r[d] = prox * pv[d];
+  if (ci >= 0) {
+    const double* S = sd.S + (size_t)ci * 6;
+    r[0] += S[0] * pv[0] + S[3] * pv[1] + S[4] * pv[2];
+    r[1] += S[3] * pv[0] + S[1] * pv[1] + S[5] * pv[2];
+    r[2] += S[4] * pv[0] + S[5] * pv[1] + S[2] * pv[2];
+    if (li >= 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) r[d] -= sd.lamL * sd.lt[(size_t)li * 3 + d] * pa;
+    }
+  }
+}
+
+// Draws a ticket after this block's partial is written; true in the block that arrives last.
+__device__ __forceinline__ bool last_to_arrive(unsigned int* ticket, int* flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned int t = atomicAdd(ticket, 1u);
+    *flag = (t == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (*flag) __threadfence();
+  return *flag != 0;
+}
+
+__device__ double sum_parts_fresh(const double* p, int n, double* red) {
+  const volatile double* q = p;
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += q[i];
+  return block_sum(s, red);
+}
+
+// x = 0, r = b, z = b / diag, p_prev = 0; |b|^2 and r.z; the last block sets up the state
+// (ConjugateGradient.h:44-62 with x0 = 0)
+__global__ __launch_bounds__(256) void cg32_init_kernel(int n, const float* __restrict__ b, const float* __restrict__ invd,
+                                                        float* __restrict__ x, float* __restrict__ r,
+                                                        float* __restrict__ z, float* __restrict__ pprev, CGState* st,
+                                                        double* part_b, double* part_rz, double tol) {
+  __shared__ double red[16];
+  __shared__ int last;
+  double sb = 0, srz = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float bv = b[i];
+    const float zv = invd[i] * bv;
+    x[i] = 0.0f;
+    r[i] = bv;
+    z[i] = zv;
+    pprev[i] = 0.0f;
+    sb += (double)bv * (double)bv;
+    srz += (double)bv * (double)zv;
+  }
+  const double a = block_sum(sb, red);
+  const double c = block_sum(srz, red);
+  if (threadIdx.x == 0) { part_b[blockIdx.x] = a; part_rz[blockIdx.x] = c; }
+  if (!last_to_arrive(&st->ticket_init, &last)) return;
+  const double b2 = sum_parts_fresh(part_b, gridDim.x, red);
+  const double rz = sum_parts_fresh(part_rz, gridDim.x, red);
+  if (threadIdx.x == 0) {
+    st->ticket_init = 0;
+    st->rhsNorm2 = b2;
+    st->iters = 0;
+    st->done = 0;
+    st->zero_rhs = 0;
+    st->beta = 0;
+    st->tol_error = 0;
+    if (b2 == 0) { st->done = 1; st->zero_rhs = 1; return; }
+    st->threshold = tol * tol * b2;
+    st->rn2 = b2;
+    if (b2 < st->threshold) { st->done = 1; st->tol_error = 1; return; }
+    st->absNew = rz;
+  }
+}
+
+__global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp ro, const float* __restrict__ K,
+                                                              const float* __restrict__ z,
+                                                              const float* __restrict__ pprev, float* __restrict__ pnew,
+                                                              float* __restrict__ out, float* __restrict__ halo,
+                                                              double prox, SoftDev sd, const int* __restrict__ sidx,
+                                                              const int* __restrict__ iso, int niso, CGState* st,
+                                                              double* partial, int set_alpha) {
+  __shared__ float sp[3 * kMaxLV];
+  __shared__ float acc[3 * kMaxLV];
+  __shared__ double red[16];
+  __shared__ int last;
+  if (st->done) return;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int nV = tm.nV, nH = tm.nH;
+  const size_t nA = 3 * (size_t)nV;
+  const int h0 = tm.tile_hex[tile], nh = tm.tile_hex[tile + 1] - h0;
+  const int lv0 = tm.tile_lv[tile], nlv = tm.tile_lv[tile + 1] - lv0;
+  const float beta = (float)st->beta;
+  // stage p of the tile's vertices
+  for (int lv = tid; lv < nlv; lv += kTileHex) {
+    const int g = tm.lv_gid[lv0 + lv], s = tm.lv_slot[lv0 + lv];
+    float pv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      pv[d] = z[(size_t)d * nV + g] + beta * pprev[(size_t)d * nV + g];
+      sp[d * kMaxLV + lv] = pv[d];
+      acc[d * kMaxLV + lv] = 0.0f;
+    }
+    const bool mine = s < 0 || tm.sv_slots[tm.sv_off[sidx[g]]] == s;
+    if (mine) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) pnew[(size_t)d * nV + g] = pv[d];
+    }
+  }
+  __syncthreads();
+  float c[8][3];
+  int li[8];
+  double part = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c[k][d] = 0.0f;
+  const bool active = tid < nh;
+  if (active) {
+    const int h = h0 + tid;
+    float q[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      li[k] = tm.lidx[(size_t)k * nH + h];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q[k][d] = sp[d * kMaxLV + li[k]];
+    }
+    hex_forces<float, float>(ro, K, nH, h, q, c);
+    float e = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) e += q[k][d] * c[k][d];
+    part = (double)e;
+  }
+  const uint16_t* off = tm.tile_col + (size_t)tile * (kMaxCol + 1);
+  for (int col = 0; col < kMaxCol; ++col) {
+    const int a = off[col], b = off[col + 1];
+    if (a >= nh) break;
+    if (active && tid >= a && tid < b) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[d * kMaxLV + li[k]] += c[k][d];
+    }
+    __syncthreads();
+  }
+  for (int lv = tid; lv < nlv; lv += kTileHex) {
+    const int g = tm.lv_gid[lv0 + lv], s = tm.lv_slot[lv0 + lv];
+    const bool owned = s < 0;
+    const bool counted = owned || tm.sv_slots[tm.sv_off[sidx[g]]] == s;
+    if (!owned) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) halo[(size_t)d * tm.nSlots + s] = acc[d * kMaxLV + lv];
+    }
+    if (!counted) continue;
+    double pv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pv[d] = (double)sp[d * kMaxLV + lv];
+    const int ci = sd.cidx ? sd.cidx[g] : -1;
+    const int ln = ci >= 0 ? sd.cline[ci] : -1;
+    const double pa = ln >= 0 ? (double)(z[nA + ln] + beta * pprev[nA + ln]) : 0.0;
+    double vr[3];
+    vertex_rows(sd, prox, ci, ln, pv, pa, vr);
+    part += pv[0] * vr[0] + pv[1] * vr[1] + pv[2] * vr[2];
+    if (owned) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) out[(size_t)d * nV + g] = (float)((double)acc[d * kMaxLV + lv] + vr[d]);
+    }
+  }
+  const double bs = block_sum(part, red);
+  if (tid == 0) partial[tile] = bs;
+  if (!last_to_arrive(&st->ticket_apply, &last)) return;
+  // vertices without hexes: their p and vertex rows (L p of those rows is the vertex term)
+  double pi = 0.0;
+  for (int k = tid; k < niso; k += kTileHex) {
+    const int g = iso[k];
+    double pv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float v = z[(size_t)d * nV + g] + beta * pprev[(size_t)d * nV + g];
+      pnew[(size_t)d * nV + g] = v;
+      pv[d] = (double)v;
+    }
+    const int ci = sd.cidx ? sd.cidx[g] : -1;
+    const int ln = ci >= 0 ? sd.cline[ci] : -1;
+    const double pa = ln >= 0 ? (double)(z[nA + ln] + beta * pprev[nA + ln]) : 0.0;
+    double vr[3];
+    vertex_rows(sd, prox, ci, ln, pv, pa, vr);
+    pi += pv[0] * vr[0] + pv[1] * vr[1] + pv[2] * vr[2];
+  }
+  const double ti = block_sum(pi, red);
+  const double tot = sum_parts_fresh(partial, gridDim.x, red) + ti;
+  if (tid == 0) {
+    st->ticket_apply = 0;
+    if (set_alpha) st->alpha = st->absNew / tot;
+    else st->pLp_tiles = tot;
+  }
+}
+
+// a_L rows: p_a = z_a + beta p_prev_a, (L p)_a = lamL (|t|^2 p_a - t . p_v); sets alpha
+__global__ __launch_bounds__(256) void cg32_lines_kernel(SoftDev sd, int nV, const float* __restrict__ z,
+                                                         const float* __restrict__ pprev, float* __restrict__ pnew,
+                                                         float* __restrict__ out, CGState* st, double* partial) {
+  __shared__ double red[16];
+  __shared__ int last;
+  if (st->done) return;
+  const size_t nA = 3 * (size_t)nV;
+  const float beta = (float)st->beta;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double part = 0.0;
+  if (i < sd.nL) {
+    const int v = sd.lv[i];
+    const float pa = z[nA + i] + beta * pprev[nA + i];
+    pnew[nA + i] = pa;
+    const double* t = sd.lt + (size_t)i * 3;
+    double tp = 0, tt = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double pv = (double)(z[(size_t)d * nV + v] + beta * pprev[(size_t)d * nV + v]);
+      tp += t[d] * pv;
+      tt += t[d] * t[d];
+    }
+    const double r = sd.lamL * (tt * (double)pa - tp);
+    out[nA + i] = (float)r;
+    part = (double)pa * r;
+  }
+  const double bs = block_sum(part, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  if (!last_to_arrive(&st->ticket_lines, &last)) return;
+  const double tot = sum_parts_fresh(partial, gridDim.x, red);
+  if (threadIdx.x == 0) {
+    st->ticket_lines = 0;
+    st->alpha = st->absNew / (st->pLp_tiles + tot);
+  }
+}
+
+// vertices (3 unknowns each) then line unknowns
+__global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev sd, double prox,
+                                                          const int* __restrict__ sidx, const float* __restrict__ halo,
+                                                          const float* __restrict__ p, const float* __restrict__ out,
+                                                          const float* __restrict__ invd, float* __restrict__ x,
+                                                          float* __restrict__ r, float* __restrict__ z, CGState* st,
+                                                          double* part_rr, double* part_rz) {
+  __shared__ double red[16];
+  __shared__ int last;
+  if (st->done) return;
+  const int nV = tm.nV;
+  const size_t nA = 3 * (size_t)nV;
+  const float alpha = (float)st->alpha;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double srr = 0, srz = 0;
+  if (i < nV) {
+    const int g = i;
+    float lp[3];
+    const int si = sidx[g];
+    if (si < 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) lp[d] = out[(size_t)d * nV + g];
+    } else {
+      double a[3] = {0, 0, 0};
+      for (int k = tm.sv_off[si]; k < tm.sv_off[si + 1]; ++k) {
+        const int s = tm.sv_slots[k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) a[d] += (double)halo[(size_t)d * tm.nSlots + s];
+      }
+      double pv[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) pv[d] = (double)p[(size_t)d * nV + g];
+      const int ci = sd.cidx ? sd.cidx[g] : -1;
+      const int ln = ci >= 0 ? sd.cline[ci] : -1;
+      const double pa = ln >= 0 ? (double)p[nA + ln] : 0.0;
+      double vr[3];
+      vertex_rows(sd, prox, ci, ln, pv, pa, vr);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) lp[d] = (float)(a[d] + vr[d]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const size_t j = (size_t)d * nV + g;
+      x[j] += alpha * p[j];
+      const float rv = r[j] - alpha * lp[d];
+      r[j] = rv;
+      const float zv = invd[j] * rv;
+      z[j] = zv;
+      srr += (double)rv * (double)rv;
+      srz += (double)rv * (double)zv;
+    }
+  } else if (i < nV + sd.nL) {
+    const size_t j = nA + (size_t)(i - nV);
+    x[j] += alpha * p[j];
+    const float rv = r[j] - alpha * out[j];
+    r[j] = rv;
+    const float zv = invd[j] * rv;
+    z[j] = zv;
+    srr += (double)rv * (double)rv;
+    srz += (double)rv * (double)zv;
+  }
+  const double a = block_sum(srr, red);
+  const double b = block_sum(srz, red);
+  if (threadIdx.x == 0) { part_rr[blockIdx.x] = a; part_rz[blockIdx.x] = b; }
+  if (!last_to_arrive(&st->ticket_update, &last)) return;
+  const double rr = sum_parts_fresh(part_rr, gridDim.x, red);
+  const double rz = sum_parts_fresh(part_rz, gridDim.x, red);
+  if (threadIdx.x == 0) {
+    st->ticket_update = 0;
+    st->rn2 = rr;
+    if (rr < st->threshold) {
+      st->done = 1;
+    } else {
+      const double absOld = st->absNew;
+      st->absNew = rz;
+      st->beta = rz / absOld;
+      st->iters += 1;
+    }
+    st->tol_error = sqrt(rr / st->rhsNorm2);
+  }
+}
+
 // residual of the direction system in FP64: r64 = b - L d (L d given in FP64), partial |r64|^2
 __global__ void residual64_kernel(int n, const float* b, const double* Ld, float* r32, double* part) {
   __shared__ double red[16];
@@ -797,6 +1148,10 @@ __global__ void residual64_kernel(int n, const float* b, const double* Ld, float
   }
   double a = block_sum(s, red);
   if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+__global__ void f2d_kernel(int n, const float* a, double* b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (double)a[i];
 }
 
 __global__ void add_inplace_kernel(int n, float* d, const float* e) {
@@ -871,6 +1226,13 @@ __device__ __forceinline__ double dd3(const double a[3], const double b[3], cons
 // those of det(E + tD) = det E + t(...) + t^2(...) + t^3 det D, negated to match the
 // reference's 4x4 volume determinant (the sign does not move the roots but enters the
 // |a| <= 1e-10 test only through its magnitude).
+//
+// Only roots below 1.25 can matter: the line search uses min(1, 0.8 t_min). A tet whose
+// cubic p(t) = dd + c t + b t^2 + a t^3 provably keeps |p(t)| >= |dd|/2 > 0 on [0, kRootCut]
+// (|c| T + |b| T^2 + |a| T^3 < |dd|/2, which also covers the quadratic branch) has no root
+// there; it reports INFINITY without solving, which leaves min(1, 0.8 t_min) unchanged since
+// 0.8 kRootCut > 1. NaN coefficients fail the test and take the exact path.
+constexpr double kRootCut = 1.3;
 __device__ double tet_min_root(const double x[4][3], const double d[4][3]) {
   double e0[3], e1[3], e2[3], f0[3], f1[3], f2[3];
   for (int k = 0; k < 3; ++k) {
@@ -882,6 +1244,8 @@ __device__ double tet_min_root(const double x[4][3], const double d[4][3]) {
   double c1 = dd3(f0, e1, e2) + dd3(e0, f1, e2) + dd3(e0, e1, f2);
   double c0 = dd3(e0, e1, e2);
   double a = -c3, b = -c2, c = -c1, dd = -c0;
+  const double T = kRootCut;
+  if (fabs(c) * T + fabs(b) * (T * T) + fabs(a) * (T * T * T) < 0.5 * fabs(dd)) return INFINITY;
   if (fabs(a) <= 1.e-10) return smallest_pos_quad_zero(b, c, dd);
   b /= a; c /= a; dd /= a;
   double res[3];
@@ -1291,6 +1655,33 @@ void read_cg_state(const void* host_state, int* iters, double* tol_error, int* d
   *iters = S->iters;
   *tol_error = S->tol_error;
   *done = S->done;
+}
+
+void launch_f2d(int n, const float* a, double* b, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(f2d_kernel, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, st, n, a, b);
+}
+
+void launch_cg32_init(int n, const float* b, const float* invd, float* x, float* r, float* z, float* pprev, void* state,
+                      double* part_b, double* part_rz, double tol, hipStream_t st) {
+  const int nb = std::min((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(cg32_init_kernel, dim3(std::max(nb, 1)), dim3(256), 0, st, n, b, invd, x, r, z, pprev,
+                     (CGState*)state, part_b, part_rz, tol);
+}
+
+void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
+                           const int* sidx, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
+                           float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
+                           double* part_tile, double* part_line, double* part_rr, double* part_rz, hipStream_t st) {
+  CGState* cs = (CGState*)state;
+  const int nL = sd.nL;
+  hipLaunchKernelGGL(cg32_apply_kernel, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out, halo,
+                     prox, sd, sidx, iso, niso, cs, part_tile, nL > 0 ? 0 : 1);
+  if (nL > 0)
+    hipLaunchKernelGGL(cg32_lines_kernel, dim3((nL + 255) / 256), dim3(256), 0, st, sd, tm.nV, z, pprev, pnew, out, cs,
+                       part_line);
+  hipLaunchKernelGGL(cg32_update_kernel, dim3((tm.nV + nL + 255) / 256), dim3(256), 0, st, tm, sd, prox, sidx,
+                     (const float*)halo, (const float*)pnew, (const float*)out, invd, x, r, zout, cs, part_rr, part_rz);
 }
 
 void launch_max_step(const double* cur, const double* dst, int nV, const int* conn, int nH, double* part,

@@ -19,4 +19,11 @@ struct CurvesDev {
 void launch_project_lines(const CurvesDev& cv, const double* X, int nV, int n, const int* rows, const int* curve_of,
                           double* origin, double* axis, hipStream_t st);
 
+struct SoftDev;
+// Writes projected origins/axes and the weight sd.lamL into the line constraint rows (axis
+// lt, origin lo, folded S0/S and b0/b rows, a_L rhs and inverse diagonal) for line vertices
+// without other constraints.
+void launch_refold_lines(const SoftDev& sd, const double* origin, const double* axis, double* S0, double* b0,
+                         double* rhs_a, double* invd_a, hipStream_t st);
+
 }  // namespace hexopt

@@ -6,6 +6,7 @@
 // multiply-add; sums of three products add x, y, z in that order.
 #include <hip/hip_runtime.h>
 #include "lines.h"
+#include "kernels.h"
 
 namespace hexopt {
 namespace {
@@ -94,7 +95,47 @@ __global__ void project_lines_kernel(CurvesDev cv, const double* __restrict__ X,
   }
 }
 
+// Line rows of the folded constraint form after a projection (set_constraints' folding with
+// the new origin o, axis a and weight lamL; a line vertex carries no corner or plane term):
+// S = S0 = lamL I, b = b0 = lamL o, and the a_L row has rhs lamL sum_d(-a_d o_d) and
+// inverse diagonal 1 / (lamL |a|^2).
+__global__ void refold_lines_kernel(const int* __restrict__ lv, const int* __restrict__ cidx, int nL,
+                                    const double* __restrict__ origin, const double* __restrict__ axis, double lamL,
+                                    double* lt, double* lo, double* S0, double* S, double* b0, double* b,
+                                    double* rhs_a, double* invd_a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nL) return;
+  const int r = cidx[lv[i]];
+  for (int k = 0; k < 6; ++k) {
+    const double v = k < 3 ? 0.0 + lamL : 0.0;
+    S0[(size_t)r * 6 + k] = v;
+    S[(size_t)r * 6 + k] = v;
+  }
+  double s = 0, tt = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double o = origin[(size_t)i * 3 + d], a = axis[(size_t)i * 3 + d];
+    lt[(size_t)i * 3 + d] = a;
+    lo[(size_t)i * 3 + d] = o;
+    const double bo = 0.0 + lamL * o;
+    b0[(size_t)r * 3 + d] = bo;
+    b[(size_t)r * 3 + d] = bo;
+    s += -a * o;
+    tt += a * a;
+  }
+  rhs_a[i] = lamL * s;
+  const double dg = lamL * tt;
+  invd_a[i] = dg != 0 ? 1.0 / dg : 1.0;
+}
+
 }  // namespace
+
+void launch_refold_lines(const SoftDev& sd, const double* origin, const double* axis, double* S0, double* b0,
+                         double* rhs_a, double* invd_a, hipStream_t st) {
+  if (sd.nL <= 0) return;
+  hipLaunchKernelGGL(refold_lines_kernel, dim3((sd.nL + 127) / 128), dim3(128), 0, st, sd.lv, sd.cidx, sd.nL, origin,
+                     axis, sd.lamL, const_cast<double*>(sd.lt), const_cast<double*>(sd.lo), S0,
+                     const_cast<double*>(sd.S), b0, const_cast<double*>(sd.b), rhs_a, invd_a);
+}
 
 void launch_project_lines(const CurvesDev& cv, const double* X, int nV, int n, const int* rows, const int* curve_of,
                           double* origin, double* axis, hipStream_t st) {

@@ -21,7 +21,8 @@ EXPORTS = ["hexopt_create", "hexopt_create_error", "hexopt_destroy", "hexopt_las
            "hexopt_surface_last_error", "hexopt_surface_destroy", "hexopt_surface_normals",
            "hexopt_compute_surface_normals", "hexopt_surface_closest", "hexopt_project_planes",
            "hexopt_occ_classify_cells", "hexopt_occ_voxel_parity", "hexopt_occ_cell_parity", "hexopt_hex_inside",
-           "hexopt_curves_create", "hexopt_curves_error", "hexopt_curves_destroy", "hexopt_project_lines"]
+           "hexopt_curves_create", "hexopt_curves_error", "hexopt_curves_destroy", "hexopt_project_lines",
+           "hexopt_refresh_lines", "hexopt_metro_hausdorff"]
 
 _LIB = None
 
@@ -75,7 +76,8 @@ def lib():
                      "hexopt_opt32_solve", "hexopt_time_kernel", "hexopt_copy_positions",
                      "hexopt_surface_normals", "hexopt_compute_surface_normals", "hexopt_surface_closest",
                      "hexopt_project_planes", "hexopt_occ_classify_cells", "hexopt_occ_voxel_parity",
-                     "hexopt_occ_cell_parity", "hexopt_hex_inside", "hexopt_project_lines"]:
+                     "hexopt_occ_cell_parity", "hexopt_hex_inside", "hexopt_project_lines",
+                     "hexopt_refresh_lines", "hexopt_metro_hausdorff"]:
             getattr(L, name).restype = C.c_int
         L.hexopt_curves_create.restype = C.c_void_p
         L.hexopt_curves_create.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
@@ -253,6 +255,12 @@ class HexOpt:
                                              _ptr(curve_of, C.c_int32), _ptr(origin), _ptr(axis)), "project_lines")
         return origin, axis
 
+    def refresh_lines(self, curves, curve_of, lamL):
+        """Re-project the line constraints onto their curves with weight lamL, on the device."""
+        curve_of = _i(curve_of)
+        self._chk(lib().hexopt_refresh_lines(C.c_void_p(self.h), C.c_void_p(curves.h), _ptr(curve_of, C.c_int32),
+                                             C.c_double(lamL)), "refresh_lines")
+
     def hex_inside(self, surface, which=CURRENT):
         """Signed distance of every hex's bounding-box centre and the inside flags."""
         S = np.zeros(self.nH)
@@ -284,6 +292,18 @@ def surface_normals(V, F):
     if rc != 0:
         raise RuntimeError("hexopt_compute_surface_normals failed (%d)" % rc)
     return FN, VN, FEN
+
+
+def metro_hausdorff(V0, F0, V1, F1, device=0):
+    """(bbox diagonal, max, mean) of the vertex-sampled metro Hausdorff distance."""
+    V0, F0, V1, F1 = _d(V0), _i(F0), _d(V1), _i(F1)
+    D, mx, mean = C.c_double(), C.c_double(), C.c_double()
+    rc = lib().hexopt_metro_hausdorff(V0.shape[0], _ptr(V0), F0.shape[0], _ptr(F0, C.c_int32), V1.shape[0], _ptr(V1),
+                                      F1.shape[0], _ptr(F1, C.c_int32), device, C.byref(D), C.byref(mx),
+                                      C.byref(mean))
+    if rc != 0:
+        raise RuntimeError("hexopt_metro_hausdorff failed (%d): %s" % (rc, lib().hexopt_surface_error().decode()))
+    return D.value, mx.value, mean.value
 
 
 class Surface:

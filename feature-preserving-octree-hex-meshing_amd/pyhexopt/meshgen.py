@@ -226,3 +226,77 @@ def torus_hex_mesh(target_hexes=1_000_000, R=1.0, r=0.35, zcut=0.2):
     V, H = voxel_hex_mesh(lambda P: clipped_torus_inside(P, R, r, zcut), -ext, -ext + n * h, n)
     bnd = np.nonzero(boundary_vertices(H, V.shape[0]))[0].astype(np.int32)
     return V, H, bnd
+
+
+def clipped_torus_surface(R=1.0, r=0.35, zcut=0.2, nu=200, nband=20, nring=10):
+    """Watertight triangulation of the torus (R, r) cut by the planes z = +-zcut: the outer
+    and inner tube bands (nu x nband quads each), the two flat annuli (nu x nring quads
+    each) and the four shared rim circles (nu vertices each). Outward orientation.
+    Returns V, F and the rim vertex lists (inner bottom, inner top, outer bottom, outer top)."""
+    a = np.arcsin(zcut / r)
+    w = np.sqrt(r * r - zcut * zcut)
+    u = np.arange(nu) * 2 * np.pi / nu
+    V = []
+
+    def ring(rad, z):
+        base = len(V)
+        V.extend([[rad * np.cos(t), rad * np.sin(t), z] for t in u])
+        return np.arange(base, base + nu)
+
+    # tube angle v: outer band v in [-a, a], inner band v in [pi - a, pi + a]
+    def band(v0, v1, first, last):
+        rows = [first]
+        for k in range(1, nband):
+            v = v0 + (v1 - v0) * k / nband
+            rows.append(ring(R + r * np.cos(v), r * np.sin(v)))
+        rows.append(last)
+        return rows
+
+    ib, it = ring(R - w, -zcut), ring(R - w, zcut)
+    ob, ot = ring(R + w, -zcut), ring(R + w, zcut)
+    F = []
+
+    def strip(rows):
+        for lo, hi in zip(rows[:-1], rows[1:]):
+            for i in range(nu):
+                j = (i + 1) % nu
+                F.append([lo[i], lo[j], hi[j]])
+                F.append([lo[i], hi[j], hi[i]])
+
+    strip(band(-a, a, ob, ot))                      # outer band, bottom rim to top rim
+    strip(band(np.pi - a, np.pi + a, it, ib))       # inner band, top rim to bottom rim
+    # annuli: rings from the inner rim to the outer rim
+    for z, first, last, up in ((zcut, it, ot, True), (-zcut, ib, ob, False)):
+        rows = [first] + [ring(R - w + 2 * w * k / nring, z) for k in range(1, nring)] + [last]
+        m0 = len(F)
+        strip(rows)
+        if not up:
+            for f in F[m0:]:
+                f[1], f[2] = f[2], f[1]
+    V = np.array(V)
+    F = np.array(F, dtype=np.int32)
+    # orient every face outward: band faces by the tube normal, annuli by +-z
+    C = V[F].mean(axis=1)
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    rho = np.sqrt(C[:, 0] ** 2 + C[:, 1] ** 2)
+    out = np.stack([C[:, 0] / rho * (rho - R), C[:, 1] / rho * (rho - R), C[:, 2]], axis=1)
+    flat = np.abs(np.abs(C[:, 2]) - zcut) < 1e-12
+    out[flat] = np.stack([np.zeros(flat.sum()), np.zeros(flat.sum()), np.sign(C[flat, 2])], axis=1)
+    flip = (n * out).sum(axis=1) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return V, F, [ib, it, ob, ot]
+
+
+HEX_FACE_TABLE = np.array([[0, 1, 2, 3], [4, 7, 6, 5], [0, 4, 5, 1], [0, 3, 7, 4], [3, 2, 6, 7], [1, 5, 6, 2]])
+
+
+def hex_boundary_triangles(H):
+    """Boundary quads of a hex mesh in the reference's hex_face_table vertex order
+    (global_types.h:154-162, kept by build_connectivity for faces of one hex), each split
+    into (q0, q1, q2) and (q2, q3, q0) as extract_surface_conforming_mesh does
+    (global_functions.cpp:1038-1054). Returns F (2 * n_boundary_quads, 3) in vertex ids of H."""
+    Q = H[:, HEX_FACE_TABLE].reshape(-1, 4)
+    key = np.sort(Q, axis=1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    B = Q[cnt[inv.ravel()] == 1]
+    return np.concatenate([B[:, [0, 1, 2]], B[:, [2, 3, 0]]]).astype(np.int32)

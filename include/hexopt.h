@@ -172,6 +172,16 @@ int hexopt_surface_closest(hexopt_surface* s, int nq, const double* P, double* S
  * dis_T = normal_T . closest point, and the plane weight becomes lamda_T. */
 int hexopt_project_planes(hexopt_ctx* ctx, hexopt_surface* s, double lamda_T);
 
+/* Metro Hausdorff distance between two triangle meshes with vertex sampling only, as the
+ * reference's compute(const Mesh&, const Mesh&, bbox_diagonal, max, mean)
+ * (metro_hausdorff.cpp:358-505; vcg Sampling, extern/vcg/sampling.h): the box of all vertices
+ * of both meshes grown by 2% of its diagonal gives *bbox_diagonal; every face-referenced vertex
+ * of each mesh is measured against the other mesh (exact point-triangle distance, samples at
+ * or beyond the diagonal dropped); *max_dist is the larger of the two directions' maxima and
+ * *mean_dist the larger of their means. On failure hexopt_surface_error() says why. */
+int hexopt_metro_hausdorff(int nv0, const double* V0, int nf0, const int32_t* F0, int nv1, const double* V1, int nf1,
+                           const int32_t* F1, int device, double* bbox_diagonal, double* max_dist, double* mean_dist);
+
 /* ---- occupancy (octree subdivision predicate, +z ray parity, inside test of hexes) ----
  * All arrays are host arrays; cells are n x 4 int32 (x, y, z, extent) in grid units. */
 
@@ -214,6 +224,12 @@ void hexopt_curves_destroy(hexopt_curves* cv);
  * fc.origin_L / fc.Axa_L rows of the reference. */
 int hexopt_project_lines(hexopt_ctx* ctx, hexopt_curves* cv, int which, int nL, const int32_t* ids_L,
                          const int32_t* curve_L, double* origin_L, double* Axa_L);
+/* dirty_graph_projection's LINE branch feeding the next slim_m_opt, on the device: projects
+ * the context's line vertices (ids_L of hexopt_set_constraints, at CURRENT) onto their
+ * curves curve_L (nL entries, in ids_L order) and replaces origin_L / Axa_L and lamda_L in
+ * the folded constraint rows, as hexopt_set_constraints would with the projected values.
+ * Line vertices must carry no corner or plane constraint (HEXOPT_EUNSUPPORTED otherwise). */
+int hexopt_refresh_lines(hexopt_ctx* ctx, hexopt_curves* cv, const int32_t* curve_L, double lamda_L);
 
 /* Blocks until the context's stream is idle. */
 int hexopt_synchronize(hexopt_ctx* ctx);

@@ -1483,3 +1483,63 @@ extern "C" void orc_line_projection(int nv, const double* V, int ncurves, const 
     for (int d = 0; d < 3; ++d) { origin[(size_t)q * 3 + d] = pv[d]; axis[(size_t)q * 3 + d] = tangent[d]; }
   }
 }
+
+// ---------------------------------------------------------------------------------------
+// Metro Hausdorff distance with vertex sampling only (compute(const Mesh&, const Mesh&, ...),
+// metro_hausdorff.cpp:358-505; vcg Sampling::Hausdorff / VertexSampling / AddSample,
+// extern/vcg/sampling.h:224-300,548-601): the box of all vertices of both meshes is grown by
+// 2% of its diagonal on every side; each face-referenced vertex of one mesh is sampled
+// against the other mesh's triangles (exact point-triangle distance, brute force here); a
+// sample counts when its distance is below the grown box's diagonal. out[0] = that diagonal,
+// out[1] = the larger of the two directions' maxima, out[2] = the larger of their means.
+namespace {
+void metro_direction(int nvA, const double* VA, int nfA, const int* FA, int nfB, const double* VB, const int* FB,
+                     double upper, double& mx, double& mean) {
+  std::vector<char> ref(nvA, 0);
+  for (long i = 0; i < 3L * nfA; ++i) ref[FA[i]] = 1;
+  Tree tb;  // only its Ericson point-triangle routine is used
+  tb.V = VB;
+  tb.F = FB;
+  mx = -HUGE_VAL;
+  double sum = 0;
+  long n = 0;
+  for (int v = 0; v < nvA; ++v) {
+    if (!ref[v]) continue;
+    const double* p = VA + (size_t)v * 3;
+    double d = upper;
+    for (int f = 0; f < nfB; ++f) {
+      double c[3], bary[3];
+      tb.closest_tri(p, f, c, bary);
+      const double d0 = p[0] - c[0], d1 = p[1] - c[1], d2 = p[2] - c[2];
+      d = std::min(d, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+    }
+    if (!(d < upper)) continue;
+    mx = std::max(mx, d);
+    sum += d;
+    ++n;
+  }
+  mean = sum / (double)n;
+}
+}  // namespace
+
+extern "C" void orc_metro_hausdorff(int nv0, const double* V0, int nf0, const int* F0, int nv1, const double* V1,
+                                    int nf1, const int* F1, double* out) {
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int i = 0; i < nv0; ++i)
+    for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], V0[(size_t)i * 3 + d]); hi[d] = std::max(hi[d], V0[(size_t)i * 3 + d]); }
+  for (int i = 0; i < nv1; ++i)
+    for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], V1[(size_t)i * 3 + d]); hi[d] = std::max(hi[d], V1[(size_t)i * 3 + d]); }
+  auto diag = [&]() {
+    const double a = hi[0] - lo[0], b = hi[1] - lo[1], c = hi[2] - lo[2];
+    return std::sqrt(a * a + b * b + c * c);
+  };
+  const double off = diag() * 0.02;
+  for (int d = 0; d < 3; ++d) { lo[d] -= off; hi[d] += off; }
+  const double D = diag();
+  double m1, a1, m2, a2;
+  metro_direction(nv0, V0, nf0, F0, nf1, V1, F1, D, m1, a1);
+  metro_direction(nv1, V1, nf1, F1, nf0, V0, F0, D, m2, a2);
+  out[0] = D;
+  out[1] = std::max(m1, m2);
+  out[2] = std::max(a2, a1);
+}

@@ -227,3 +227,12 @@ def line_projection(V, curves, circles, P, curve_of):
                               _p(circ, C.c_uint8), n, _p(P, C.c_double), _p(curve_of, C.c_int),
                               _p(origin, C.c_double), _p(axis, C.c_double))
     return origin, axis
+
+
+def metro_hausdorff(V0, F0, V1, F1):
+    """(bbox diagonal, max, mean) of the vertex-sampled metro Hausdorff distance."""
+    V0, F0, V1, F1 = _d(V0), _i(F0), _d(V1), _i(F1)
+    out = np.zeros(3)
+    lib().orc_metro_hausdorff(V0.shape[0], _p(V0, C.c_double), F0.shape[0], _p(F0, C.c_int), V1.shape[0],
+                              _p(V1, C.c_double), F1.shape[0], _p(F1, C.c_int), _p(out, C.c_double))
+    return tuple(out)

@@ -186,5 +186,35 @@ def main():
     save("aabb_cube8", SV=CV, SF=CF, P=P, **{"ref_" + k: v for k, v in r.items()})
 
 
+
+def metro_cases():
+    """Vertex-sampled metro Hausdorff distance (metro_hausdorff.cpp compute, const overload)."""
+    if not os.path.exists(refio.METRO_BIN):
+        raise SystemExit("oracle/_ref/metro_ref is not built (make -C oracle/ref)")
+    # sphere vs superellipsoid; plus an unreferenced far vertex that grows the box only
+    V0, F0 = mg.icosphere(3)
+    V1, F1 = mg.superellipsoid(3, 4.0)
+    r = refio.run_metro(V0, F0, V1, F1)
+    save("metro_sphere_superellipsoid", V0=V0, F0=F0, V1=V1, F1=F1, **{"ref_" + k: v for k, v in r.items()})
+    V0x = np.concatenate([V0, [[3.0, -2.0, 2.5]]])
+    r = refio.run_metro(V0x, F0, V1, F1)
+    save("metro_unreferenced_vertex", V0=V0x, F0=F0, V1=V1, F1=F1, **{"ref_" + k: v for k, v in r.items()})
+    # clipped torus surface vs the triangulated boundary of a voxel hex mesh of its inside
+    SV, SF, _ = mg.clipped_torus_surface(nu=60, nband=6, nring=4)
+    V, H, _ = mg.torus_hex_mesh(4000)
+    BF = mg.hex_boundary_triangles(H)
+    used = np.unique(BF)
+    remap = -np.ones(V.shape[0], dtype=np.int64)
+    remap[used] = np.arange(used.size)
+    HV, HF = V[used], remap[BF].astype(np.int32)
+    r = refio.run_metro(SV, SF, HV, HF)
+    save("metro_torus_hexboundary", V0=SV, F0=SF, V1=HV, F1=HF, **{"ref_" + k: v for k, v in r.items()})
+
+
 if __name__ == "__main__":
-    main()
+    import sys
+    if sys.argv[1:] == ["metro"]:
+        metro_cases()
+    else:
+        main()
+        metro_cases()

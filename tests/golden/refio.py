@@ -79,3 +79,19 @@ def run_aabb(V, F, P):
             _arr(f, V, "<f8"); _arr(f, F, "<i4"); _arr(f, P, "<f8")
         subprocess.run([REF_BIN, pin, pout], check=True, stdout=subprocess.DEVNULL)
         return read_out(pout)
+
+
+METRO_BIN = os.path.join(REPO, "oracle", "_ref", "metro_ref")
+
+
+def run_metro(V0, F0, V1, F1):
+    """The reference's compute(mesh0, mesh1, bbox_diagonal, max, mean) (metro_hausdorff.cpp)."""
+    with tempfile.TemporaryDirectory() as td:
+        pin, pout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(pin, "wb") as f:
+            for V, F in ((V0, F0), (V1, F1)):
+                _w(f, "ii", V.shape[0], F.shape[0])
+                _arr(f, V, "<f8")
+                _arr(f, F, "<i4")
+        subprocess.run([METRO_BIN, pin, pout], check=True, stdout=subprocess.DEVNULL)
+        return read_out(pout)

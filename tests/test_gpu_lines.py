@@ -64,3 +64,32 @@ def test_project_lines_vs_oracle():
     # the one-vertex curve: the vertex itself and a NaN tangent, as the reference computes
     o3, a3 = h.project_lines(cv, ids[:5], np.full(5, len(curves) - 1, dtype=np.int32))
     assert (o3 == SV[-1]).all() and np.isnan(a3).all()
+
+
+def test_refresh_lines_equals_set_constraints():
+    """hexopt_refresh_lines (projection and re-folding on the device) gives bit for bit the
+    system and the SLIM iteration that hexopt_set_constraints gives with the same projected
+    origins and axes."""
+    from pyhexopt import capi
+    SV, curves, circles = _curves()
+    n = 8
+    V, H = mg.grid_hex(n, -1.5, 1.5)
+    V = mg.jitter(V, ~mg.grid_boundary_mask(n), 0.02, seed=4)
+    cv = capi.Curves(SV, curves, circles)
+    rng = np.random.default_rng(3)
+    ids = np.sort(rng.choice(V.shape[0], size=60, replace=False)).astype(np.int32)
+    cid = rng.integers(0, 6, size=ids.size).astype(np.int32)
+    lam = 10.0
+    h1 = capi.HexOpt(V, H)
+    h1.reference()
+    h1.set_constraints(idsL=ids, axaL=np.zeros((ids.size, 3)), originL=np.zeros((ids.size, 3)), lamL=1.0)
+    h1.refresh_lines(cv, cid, lam)
+    o, a = h1.project_lines(cv, ids, cid)
+    h2 = capi.HexOpt(V, H)
+    h2.reference()
+    h2.set_constraints(idsL=ids, axaL=a, originL=o, lamL=lam)
+    p = np.random.default_rng(0).normal(size=3 * V.shape[0] + ids.size)
+    for x, y in zip(h1.debug_system(p), h2.debug_system(p)):
+        assert np.array_equal(x, y)
+    r1, r2 = h1.slim_solve(2), h2.slim_solve(2)
+    assert r1[0] == r2[0] and np.array_equal(h1.get_positions(), h2.get_positions())
