@@ -83,19 +83,27 @@ __device__ void svd3(const T A[3][3], T U[3][3], T s[3], T V[3][3]) {
   T n[3];
 #pragma unroll
   for (int j = 0; j < 3; ++j) n[j] = dsqrt(B[0][j] * B[0][j] + B[1][j] * B[1][j] + B[2][j] * B[2][j]);
-  // sort descending (3-element network, ties keep the original order)
-  int o0 = 0, o1 = 1, o2 = 2;
-  if (n[o1] > n[o0]) { int t = o0; o0 = o1; o1 = t; }
-  if (n[o2] > n[o1]) { int t = o1; o1 = o2; o2 = t; }
-  if (n[o1] > n[o0]) { int t = o0; o0 = o1; o1 = t; }
-  const int ord[3] = {o0, o1, o2};
+  // sort descending by a 3-element compare-and-swap network on whole columns (ties keep the
+  // original order); columns move by selects, so every array stays in registers
   T Vs[3][3], Bs[3][3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    s[k] = n[ord[k]];
+    s[k] = n[k];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) { Vs[i][k] = V[i][ord[k]]; Bs[i][k] = B[i][ord[k]]; }
+    for (int i = 0; i < 3; ++i) { Vs[i][k] = V[i][k]; Bs[i][k] = B[i][k]; }
   }
+  auto cswap = [&](int a, int b) {  // a < b are constants after unrolling
+    const bool sw = s[b] > s[a];
+    T t = s[a]; s[a] = sw ? s[b] : t; s[b] = sw ? t : s[b];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      T v = Vs[i][a]; Vs[i][a] = sw ? Vs[i][b] : v; Vs[i][b] = sw ? v : Vs[i][b];
+      T w = Bs[i][a]; Bs[i][a] = sw ? Bs[i][b] : w; Bs[i][b] = sw ? w : Bs[i][b];
+    }
+  };
+  cswap(0, 1);
+  cswap(1, 2);
+  cswap(0, 1);
   const T tiny = sizeof(T) == 8 ? T(1e-300) : T(1e-37);
   T u[3][3];
 #pragma unroll

@@ -590,8 +590,8 @@ static int ensure_state32(hexopt_ctx* c) {
 }
 
 // FP32 CG on L x = b from x = 0 (Eigen's algorithm, FP64 dot products), `iters` iterations,
-// two or three fused launches per iteration and no host round trip. Uses f_r, f_z, f_p,
-// f_p2 (the two direction buffers), f_tmp and f_halo.
+// four launches per iteration (five with feature lines) and no host round trip. Uses f_r,
+// f_z, f_p, f_p2 (the two direction buffers), f_tmp and f_halo.
 static int cg32(hexopt_ctx* c, const SoftDev& sd, double prox, const float* b, float* x, int iters, int* it_out,
                 double* err_out) {
   const int nV = c->L.nV, nL = c->nL;
@@ -599,7 +599,6 @@ static int cg32(hexopt_ctx* c, const SoftDev& sd, double prox, const float* b, f
   if (c->L.nTiles == 0) return fail(c, HEXOPT_EINVAL, "mesh without hexes");
   hipStream_t st = c->stream;
   RefOp ro = refop(c);
-  HIPCHK(c, hipMemsetAsync(c->d_cg.p, 0, cg_state_bytes(), st));
   float* P[2] = {c->f_p.as<float>(), c->f_p2.as<float>()};
   float* z = c->f_z.as<float>();
   float* r = c->f_r.as<float>();
@@ -631,7 +630,8 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
   if (prm->energy != HEXOPT_SYMMETRIC_DIRICHLET)
     return fail(c, HEXOPT_EUNSUPPORTED, "the FP32 adjoint optimiser supports SYMMETRIC_DIRICHLET");
   if (prm->projection && c->sd.nb > 0) return fail(c, HEXOPT_EUNSUPPORTED, "projection constraints");
-  if (opt->cg_iters < 1 || opt->refine_iters < 0) return fail(c, HEXOPT_EINVAL, "bad iteration counts");
+  if (opt->cg_iters < 1 || opt->refine_iters < 0 || opt->residual_every < 0)
+    return fail(c, HEXOPT_EINVAL, "bad iteration counts");
   HIPCHK(c, hipSetDevice(c->device));
   int rc = ensure_state(c);
   if (rc) return rc;
@@ -655,10 +655,15 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
     float* d = c->f_d.as<float>();
     if ((rc = cg32(c, sd, prox, c->f_rhs.as<float>(), d, opt->cg_iters, nullptr, nullptr))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->d_cgsave.p, c->d_cg.p, cg_state_bytes(), hipMemcpyDeviceToDevice, st));
-    launch_apply32_f64(c->tm, ro, c->f_K.as<float>(), d, c->d_Ld64.as<double>(), prox, sd, c->d_halo.as<double>(), st);
     const int nb = nblocks(n);
-    launch_residual64(n, c->f_rhs.as<float>(), c->d_Ld64.as<double>(), c->f_r.as<float>(), c->d_part[1].as<double>(),
-                      nb, scal + 6, st);
+    const bool measure = opt->refine_iters > 0 ||
+                         (opt->residual_every > 0 && (it % opt->residual_every == 0 || it == iters - 1));
+    if (measure) {
+      launch_apply32_f64(c->tm, ro, c->f_K.as<float>(), d, c->d_Ld64.as<double>(), prox, sd, c->d_halo.as<double>(),
+                         st);
+      launch_residual64(n, c->f_rhs.as<float>(), c->d_Ld64.as<double>(), c->f_r.as<float>(),
+                        c->d_part[1].as<double>(), nb, scal + 6, st);
+    }
     if (opt->refine_iters > 0) {
       // FP32 copy of the FP64 residual -> correction e, d += e (the copy keeps the right-hand
       // side apart from the CG's own r, z and p)
@@ -718,7 +723,8 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
     if (log) {
       hexopt_opt32_log& lg = log[it];
       lg.energy = energy; lg.e_quality = eq; lg.e_soft = es; lg.step_bound = bound; lg.step = step;
-      lg.cg_iters = cg_it; lg.cg_error = cg_err; lg.residual64 = rhs_norm > 0 ? std::sqrt(res64) / rhs_norm : 0.0;
+      lg.cg_iters = cg_it; lg.cg_error = cg_err;
+      lg.residual64 = !measure ? -1.0 : (rhs_norm > 0 ? std::sqrt(res64) / rhs_norm : 0.0);
       lg.inverted_tets = ninv;
     }
   }
@@ -749,8 +755,9 @@ int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch
                      c->d_halo.as<double>(), c->d_part[2].as<double>(), nullptr, nullptr, st);
         break;
       case 1:
-        launch_apply32(tm, ro, c->f_K.as<float>(), c->f_p.as<float>(), c->f_tmp.as<float>(), 1e-5, sd,
-                       c->f_halo.as<float>(), c->d_part[2].as<double>(), nullptr, nullptr, st);
+        launch_cg32_apply(c->tm, ro, c->f_K.as<float>(), 1e-5, sd, c->d_sidx.as<int>(), c->f_z.as<float>(),
+                          c->f_p.as<float>(), c->f_p2.as<float>(), c->f_tmp.as<float>(), c->f_halo.as<float>(),
+                          c->d_cg.p, c->d_part[2].as<double>(), st);
         break;
       case 2:
         launch_local_step(tm, ro, c->d_cur.as<double>(), c->d_K.as<double>(), HEXOPT_SYMMETRIC_DIRICHLET, 1.0, 1.0,
@@ -763,6 +770,13 @@ int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch
     }
   };
   sd.nL = 0;
+  if (which == 1) {
+    // a fresh CG state (not converged) on the current FP32 system, so the pass runs in full
+    HIPCHK(c, hipMemsetAsync(c->d_cg.p, 0, cg_state_bytes(), st));
+    launch_cg32_init(3 * c->L.nV + c->nL, c->f_rhs.as<float>(), c->f_invd.as<float>(), c->f_d.as<float>(),
+                     c->f_r.as<float>(), c->f_z.as<float>(), c->f_p.as<float>(), c->d_cg.p, c->d_part[0].as<double>(),
+                     c->d_part[1].as<double>(), 1e-8, st);
+  }
   one();  // warm
   HIPCHK(c, hipEventRecord(e0, st));
   for (int i = 0; i < reps; ++i) one();

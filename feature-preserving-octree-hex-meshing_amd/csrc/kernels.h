@@ -96,6 +96,10 @@ void launch_cg_zero_if32(int n, void* state, float* x, int nb, hipStream_t st);
 // tile writes it); iso lists the vertices without hexes.
 void launch_cg32_init(int n, const float* b, const float* invd, float* x, float* r, float* z, float* pprev, void* state,
                       double* part_b, double* part_rz, double tol, hipStream_t st);
+// The operator pass of one iteration alone (feature lines ignored), for timing.
+void launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
+                       const int* sidx, const float* z, const float* pprev, float* pnew, float* out, float* halo,
+                       const void* state, double* part_tile, hipStream_t st);
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
                            const int* sidx, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 #include "device_math.h"
 #include "kernels.h"
 
@@ -76,14 +77,42 @@ __device__ __forceinline__ void tet_B(const RefOp& ro, int nH, int h, int c, T B
   }
 }
 
+// Rows of the per-hex corner arrays are selected, not indexed: a corner-table entry is the
+// same in every lane, but a run-time index into a register array would move the array to
+// scratch memory. With a constant index (unrolled corner loops) the selects fold away.
+template <class T, int N>
+__device__ __forceinline__ void pick_row(const T a[8][N], int i, T r[N]) {
+#pragma unroll
+  for (int d = 0; d < N; ++d) r[d] = a[0][d];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    const bool m = i == k;
+#pragma unroll
+    for (int d = 0; d < N; ++d) r[d] = m ? a[k][d] : r[d];
+  }
+}
+
+// a[i][OFF + d] += v[d] for d < M, by selects (adds 0 to the other rows)
+template <int OFF, int M, class T, int N>
+__device__ __forceinline__ void add_row(T a[8][N], int i, const T v[M]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const bool m = i == k;
+#pragma unroll
+    for (int d = 0; d < M; ++d) a[k][OFF + d] += m ? v[d] : T(0);
+  }
+}
+
 template <class T>
 __device__ __forceinline__ void tet_edges(const T p[8][3], int c, T E[3][3]) {
-  const int v0 = kHexTet[c][0];
+  T a[3];
+  pick_row<T, 3>(p, kHexTet[c][0], a);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const int vk = kHexTet[c][k + 1];
+    T b[3];
+    pick_row<T, 3>(p, kHexTet[c][k + 1], b);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) E[d][k] = p[vk][d] - p[v0][d];
+    for (int d = 0; d < 3; ++d) E[d][k] = b[d] - a[d];
   }
 }
 
@@ -102,13 +131,14 @@ __device__ __forceinline__ void scatter_tet(const T F[3][3], const T B[3][3], in
   const int v0 = kHexTet[c][0];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const int vk = kHexTet[c][k + 1];
+    T f[3], mf[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      T f = F[d][0] * B[k][0] + F[d][1] * B[k][1] + F[d][2] * B[k][2];
-      out[vk][d] += f;
-      out[v0][d] -= f;
+      f[d] = F[d][0] * B[k][0] + F[d][1] * B[k][1] + F[d][2] * B[k][2];
+      mf[d] = -f[d];
     }
+    add_row<0, 3>(out, kHexTet[c][k + 1], f);
+    add_row<0, 3>(out, v0, mf);
   }
 }
 
@@ -252,10 +282,11 @@ __device__ double hex_volume24(const double p[8][3]) {
 
 // signed tet volume (igl::volume convention) of corner tet c
 __device__ __forceinline__ double corner_volume(const double p[8][3], int c) {
-  const double* a = p[kHexTet[c][0]];
-  const double* b = p[kHexTet[c][1]];
-  const double* cc = p[kHexTet[c][2]];
-  const double* d = p[kHexTet[c][3]];
+  double a[3], b[3], cc[3], d[3];
+  pick_row<double, 3>(p, kHexTet[c][0], a);
+  pick_row<double, 3>(p, kHexTet[c][1], b);
+  pick_row<double, 3>(p, kHexTet[c][2], cc);
+  pick_row<double, 3>(p, kHexTet[c][3], d);
   double ad[3] = {a[0] - d[0], a[1] - d[1], a[2] - d[2]};
   double bd[3] = {b[0] - d[0], b[1] - d[1], b[2] - d[2]};
   double cd[3] = {cc[0] - d[0], cc[1] - d[1], cc[2] - d[2]};
@@ -347,8 +378,11 @@ __global__ void ref_general_kernel(const double* X, int nV, const int* conn, int
   }
   for (int c = 0; c < 8; ++c) {
     double x[4][3], r[4][3];
-    for (int j = 0; j < 4; ++j)
-      for (int d = 0; d < 3; ++d) { x[j][d] = p[kHexTet[c][j]][d]; r[j][d] = r8[kHexTet[c][j]][d]; }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      pick_row<double, 3>(p, kHexTet[c][j], x[j]);
+      pick_row<double, 3>(r8, kHexTet[c][j], r[j]);
+    }
     double B[3][3];
     general_B(x, r, corner_volume(p, c), B);
     for (int i = 0; i < 3; ++i)
@@ -425,13 +459,17 @@ struct LocalStepOp {
       // diagonal: Kt_bb * |G[:,j]|^2; G[:,k] = B[k-1,:], G[:,0] = -sum_k B[k,:]
       double g0[3] = {-(B[0][0] + B[1][0] + B[2][0]), -(B[0][1] + B[1][1] + B[2][1]), -(B[0][2] + B[1][2] + B[2][2])};
       double n0 = g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) out[kHexTet[c][0]][3 + d] += Kt[d][d] * n0;
+      {
+        std::remove_reference_t<decltype(Kt[0][0])> v[3] = {Kt[0][0] * n0, Kt[1][1] * n0, Kt[2][2] * n0};
+        add_row<3, 3>(out, kHexTet[c][0], v);
+      }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         double nk = B[k][0] * B[k][0] + B[k][1] * B[k][1] + B[k][2] * B[k][2];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) out[kHexTet[c][k + 1]][3 + d] += Kt[d][d] * nk;
+        {
+          std::remove_reference_t<decltype(Kt[0][0])> v[3] = {Kt[0][0] * nk, Kt[1][1] * nk, Kt[2][2] * nk};
+          add_row<3, 3>(out, kHexTet[c][k + 1], v);
+        }
       }
     }
     return 0.0;
@@ -555,12 +593,19 @@ struct AdjLocalOp32 {
     int inv = 0;
 #pragma unroll 1
     for (int c = 0; c < 8; ++c) {
-      {  // FP64 orientation test of the corner tet: sign of det J = det(E B)
-        double E64[3][3], B64[3][3], J64[3][3];
+      {  // FP64 orientation test of the corner tet: sign of det J = det(E B). For a cube
+         // reference B = invs * (a permutation with determinant +1 for every corner) and
+         // invs > 0, so the sign is that of det E.
+        double E64[3][3];
         tet_edges(pd, c, E64);
-        tet_B(ro, nH, h, c, B64);
-        matmul3(E64, B64, J64);
-        inv += det3(J64) < 0.0;
+        if (ro.mode == REF_CUBE) {
+          inv += det3(E64) < 0.0;
+        } else {
+          double B64[3][3], J64[3][3];
+          tet_B(ro, nH, h, c, B64);
+          matmul3(E64, B64, J64);
+          inv += det3(J64) < 0.0;
+        }
       }
       float B[3][3], E[3][3], J[3][3], W[3][3], R[3][3], P[3][3];
       tet_B(ro, nH, h, c, B);
@@ -593,13 +638,17 @@ struct AdjLocalOp32 {
         for (int d = 0; d < 3; ++d) out[k][d] += tmp[k][d];
       float g0[3] = {-(B[0][0] + B[1][0] + B[2][0]), -(B[0][1] + B[1][1] + B[2][1]), -(B[0][2] + B[1][2] + B[2][2])};
       float n0 = g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) out[kHexTet[c][0]][3 + d] += Kt[d][d] * n0;
+      {
+        std::remove_reference_t<decltype(Kt[0][0])> v[3] = {Kt[0][0] * n0, Kt[1][1] * n0, Kt[2][2] * n0};
+        add_row<3, 3>(out, kHexTet[c][0], v);
+      }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         float nk = B[k][0] * B[k][0] + B[k][1] * B[k][1] + B[k][2] * B[k][2];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) out[kHexTet[c][k + 1]][3 + d] += Kt[d][d] * nk;
+        {
+          std::remove_reference_t<decltype(Kt[0][0])> v[3] = {Kt[0][0] * nk, Kt[1][1] * nk, Kt[2][2] * nk};
+          add_row<3, 3>(out, kHexTet[c][k + 1], v);
+        }
       }
       (void)R;
     }
@@ -668,9 +717,6 @@ __global__ void line_apply_kernel(SoftDev sd, int nV, const PT* p, const PT* pa,
 struct CGState {
   double rhsNorm2, threshold, absNew, alpha, beta, rn2, tol_error;
   int done, iters, zero_rhs, pad;
-  // fused FP32 CG: arrival counters of the last-block reductions and p.Lp of the tiles
-  unsigned int ticket_init, ticket_apply, ticket_lines, ticket_update;
-  double pLp_tiles;
 };
 
 // r = rhs - tmp, partial |rhs|^2, |r|^2
@@ -801,18 +847,17 @@ __global__ void cg_zero_if_kernel(int n, const CGState* st, T* x) {
 // FP32 CG with the vector work fused into the operator pass (the FP32 optimiser's global
 // step, Eigen's algorithm with FP64 dot products). Per CG iteration:
 //   cg32_apply   per tile: p = z + beta p_prev staged in LDS (p double-buffered, each vertex
-//                written once: by its own tile, the first tile of its halo slots, or the
-//                reducing block for vertices without hexes), L p of the tile's elements,
-//                p.Lp from the element quadratic forms p_e.(K_e p_e) plus the vertex rows;
-//                the last block to arrive reduces the tile partials in tile order and sets
-//                alpha (or leaves p.Lp to the line kernel)
-//   cg32_lines   a_L rows of L p (feature lines only); its last block sets alpha
+//                written once: by its own tile or by the first tile of its halo slots),
+//                L p of the tile's elements, and a per-tile p.Lp partial from the element
+//                quadratic forms p_e.(K_e p_e) plus the vertex rows
+//   cg32_lines   a_L rows of L p and their p.Lp partials (feature lines only)
+//   cg32_alpha   one block: vertices without hexes, then alpha = r.z / p.Lp
 //   cg32_update  L p of halo vertices from their slots, x += alpha p, r -= alpha L p,
-//                z = r / diag, FP64 |r|^2 and r.z; its last block sets beta and the
-//                convergence flag
-// Last-block reductions: every block writes its partial, fences, and takes a ticket from an
-// arrival counter; the block that draws the final ticket fences, reduces all partials in a
-// fixed order and resets the counter. No block ever waits for another.
+//                z = r / diag, FP64 partials of |r|^2 and r.z
+//   cg_beta      one block: convergence test and beta
+// Every reduction is two-level in a fixed order; kernel boundaries order the levels (an
+// in-kernel last-block reduction would need a device-scope fence per block, which on this
+// multi-XCD part writes back L2 each time).
 
 // rows of L p beyond the elements at vertex g: prox p + S p - lamL t a
 __device__ __forceinline__ void vertex_rows(const SoftDev& sd, double prox, int ci, int li, const double pv[3], double pa,
@@ -831,34 +876,12 @@ __device__ __forceinline__ void vertex_rows(const SoftDev& sd, double prox, int 
   }
 }
 
-// Draws a ticket after this block's partial is written; true in the block that arrives last.
-__device__ __forceinline__ bool last_to_arrive(unsigned int* ticket, int* flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned int t = atomicAdd(ticket, 1u);
-    *flag = (t == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (*flag) __threadfence();
-  return *flag != 0;
-}
-
-__device__ double sum_parts_fresh(const double* p, int n, double* red) {
-  const volatile double* q = p;
-  double s = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += q[i];
-  return block_sum(s, red);
-}
-
-// x = 0, r = b, z = b / diag, p_prev = 0; |b|^2 and r.z; the last block sets up the state
-// (ConjugateGradient.h:44-62 with x0 = 0)
+// x = 0, r = b, z = b / diag, p_prev = 0; partials of |b|^2 and r.z
 __global__ __launch_bounds__(256) void cg32_init_kernel(int n, const float* __restrict__ b, const float* __restrict__ invd,
                                                         float* __restrict__ x, float* __restrict__ r,
-                                                        float* __restrict__ z, float* __restrict__ pprev, CGState* st,
-                                                        double* part_b, double* part_rz, double tol) {
+                                                        float* __restrict__ z, float* __restrict__ pprev,
+                                                        double* part_b, double* part_rz) {
   __shared__ double red[16];
-  __shared__ int last;
   double sb = 0, srz = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float bv = b[i];
@@ -873,11 +896,14 @@ __global__ __launch_bounds__(256) void cg32_init_kernel(int n, const float* __re
   const double a = block_sum(sb, red);
   const double c = block_sum(srz, red);
   if (threadIdx.x == 0) { part_b[blockIdx.x] = a; part_rz[blockIdx.x] = c; }
-  if (!last_to_arrive(&st->ticket_init, &last)) return;
-  const double b2 = sum_parts_fresh(part_b, gridDim.x, red);
-  const double rz = sum_parts_fresh(part_rz, gridDim.x, red);
+}
+
+// ConjugateGradient.h:44-62 with x0 = 0 (so r0 = b and |r0|^2 = |b|^2)
+__global__ void cg32_init_fin_kernel(CGState* st, const double* part_b, const double* part_rz, int nb, double tol) {
+  __shared__ double red[16];
+  const double b2 = sum_parts(part_b, nb, red);
+  const double rz = sum_parts(part_rz, nb, red);
   if (threadIdx.x == 0) {
-    st->ticket_init = 0;
     st->rhsNorm2 = b2;
     st->iters = 0;
     st->done = 0;
@@ -897,12 +923,10 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
                                                               const float* __restrict__ pprev, float* __restrict__ pnew,
                                                               float* __restrict__ out, float* __restrict__ halo,
                                                               double prox, SoftDev sd, const int* __restrict__ sidx,
-                                                              const int* __restrict__ iso, int niso, CGState* st,
-                                                              double* partial, int set_alpha) {
+                                                              const CGState* st, double* partial) {
   __shared__ float sp[3 * kMaxLV];
   __shared__ float acc[3 * kMaxLV];
   __shared__ double red[16];
-  __shared__ int last;
   if (st->done) return;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
@@ -990,10 +1014,21 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
   }
   const double bs = block_sum(part, red);
   if (tid == 0) partial[tile] = bs;
-  if (!last_to_arrive(&st->ticket_apply, &last)) return;
-  // vertices without hexes: their p and vertex rows (L p of those rows is the vertex term)
+}
+
+// Vertices without hexes (their p and vertex rows; no tile touches them), then
+// alpha = absNew / p.Lp over the tile, line and isolated-vertex partials.
+__global__ __launch_bounds__(1024) void cg32_alpha_kernel(CGState* st, const double* part_tile, int ntiles,
+                                                          const double* part_line, int nline, SoftDev sd, double prox,
+                                                          int nV, const int* __restrict__ iso, int niso,
+                                                          const float* __restrict__ z,
+                                                          const float* __restrict__ pprev, float* __restrict__ pnew) {
+  __shared__ double red[16];
+  if (st->done) return;
+  const size_t nA = 3 * (size_t)nV;
+  const float beta = (float)st->beta;
   double pi = 0.0;
-  for (int k = tid; k < niso; k += kTileHex) {
+  for (int k = threadIdx.x; k < niso; k += blockDim.x) {
     const int g = iso[k];
     double pv[3];
 #pragma unroll
@@ -1009,21 +1044,17 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
     vertex_rows(sd, prox, ci, ln, pv, pa, vr);
     pi += pv[0] * vr[0] + pv[1] * vr[1] + pv[2] * vr[2];
   }
-  const double ti = block_sum(pi, red);
-  const double tot = sum_parts_fresh(partial, gridDim.x, red) + ti;
-  if (tid == 0) {
-    st->ticket_apply = 0;
-    if (set_alpha) st->alpha = st->absNew / tot;
-    else st->pLp_tiles = tot;
-  }
+  const double a = sum_parts(part_tile, ntiles, red);
+  const double b = nline > 0 ? sum_parts(part_line, nline, red) : 0.0;
+  const double c = block_sum(pi, red);
+  if (threadIdx.x == 0) st->alpha = st->absNew / (a + b + c);
 }
 
 // a_L rows: p_a = z_a + beta p_prev_a, (L p)_a = lamL (|t|^2 p_a - t . p_v); sets alpha
 __global__ __launch_bounds__(256) void cg32_lines_kernel(SoftDev sd, int nV, const float* __restrict__ z,
                                                          const float* __restrict__ pprev, float* __restrict__ pnew,
-                                                         float* __restrict__ out, CGState* st, double* partial) {
+                                                         float* __restrict__ out, const CGState* st, double* partial) {
   __shared__ double red[16];
-  __shared__ int last;
   if (st->done) return;
   const size_t nA = 3 * (size_t)nV;
   const float beta = (float)st->beta;
@@ -1047,12 +1078,6 @@ __global__ __launch_bounds__(256) void cg32_lines_kernel(SoftDev sd, int nV, con
   }
   const double bs = block_sum(part, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
-  if (!last_to_arrive(&st->ticket_lines, &last)) return;
-  const double tot = sum_parts_fresh(partial, gridDim.x, red);
-  if (threadIdx.x == 0) {
-    st->ticket_lines = 0;
-    st->alpha = st->absNew / (st->pLp_tiles + tot);
-  }
 }
 
 // vertices (3 unknowns each) then line unknowns
@@ -1060,10 +1085,9 @@ __global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev s
                                                           const int* __restrict__ sidx, const float* __restrict__ halo,
                                                           const float* __restrict__ p, const float* __restrict__ out,
                                                           const float* __restrict__ invd, float* __restrict__ x,
-                                                          float* __restrict__ r, float* __restrict__ z, CGState* st,
-                                                          double* part_rr, double* part_rz) {
+                                                          float* __restrict__ r, float* __restrict__ z,
+                                                          const CGState* st, double* part_rr, double* part_rz) {
   __shared__ double red[16];
-  __shared__ int last;
   if (st->done) return;
   const int nV = tm.nV;
   const size_t nA = 3 * (size_t)nV;
@@ -1119,22 +1143,6 @@ __global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev s
   const double a = block_sum(srr, red);
   const double b = block_sum(srz, red);
   if (threadIdx.x == 0) { part_rr[blockIdx.x] = a; part_rz[blockIdx.x] = b; }
-  if (!last_to_arrive(&st->ticket_update, &last)) return;
-  const double rr = sum_parts_fresh(part_rr, gridDim.x, red);
-  const double rz = sum_parts_fresh(part_rz, gridDim.x, red);
-  if (threadIdx.x == 0) {
-    st->ticket_update = 0;
-    st->rn2 = rr;
-    if (rr < st->threshold) {
-      st->done = 1;
-    } else {
-      const double absOld = st->absNew;
-      st->absNew = rz;
-      st->beta = rz / absOld;
-      st->iters += 1;
-    }
-    st->tol_error = sqrt(rr / st->rhsNorm2);
-  }
 }
 
 // residual of the direction system in FP64: r64 = b - L d (L d given in FP64), partial |r64|^2
@@ -1293,11 +1301,14 @@ __global__ void max_step_kernel(const double* cur, const DT* dst, int nV, const 
     gather8(cur, nV, conn, nH, h, vid, p);
     for (int k = 0; k < 8; ++k)
       for (int d = 0; d < 3; ++d) q[k][d] = line_dir<DIFF>(p[k][d], dst[(size_t)d * nV + vid[k]]);
-#pragma unroll 1
+#pragma unroll
     for (int c = 0; c < 8; ++c) {
       double x[4][3], dd[4][3];
-      for (int j = 0; j < 4; ++j)
-        for (int d = 0; d < 3; ++d) { x[j][d] = p[kHexTet[c][j]][d]; dd[j][d] = q[kHexTet[c][j]][d]; }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        pick_row<double, 3>(p, kHexTet[c][j], x[j]);
+        pick_row<double, 3>(q, kHexTet[c][j], dd[j]);
+      }
       m = fmin(m, tet_min_root(x, dd));
     }
   }
@@ -1332,7 +1343,7 @@ __global__ void energy_kernel(RefOp ro, const double* cur, const DT* dst, double
     T pt[8][3];
     for (int k = 0; k < 8; ++k)
       for (int d = 0; d < 3; ++d) pt[k][d] = (T)p[k][d];
-#pragma unroll 1
+#pragma unroll
     for (int c = 0; c < 8; ++c) {
       T B[3][3], E[3][3], J[3][3];
       tet_B(ro, nH, h, c, B);
@@ -1664,9 +1675,17 @@ void launch_f2d(int n, const float* a, double* b, hipStream_t st) {
 
 void launch_cg32_init(int n, const float* b, const float* invd, float* x, float* r, float* z, float* pprev, void* state,
                       double* part_b, double* part_rz, double tol, hipStream_t st) {
-  const int nb = std::min((n + 255) / 256, 2048);
-  hipLaunchKernelGGL(cg32_init_kernel, dim3(std::max(nb, 1)), dim3(256), 0, st, n, b, invd, x, r, z, pprev,
-                     (CGState*)state, part_b, part_rz, tol);
+  const int nb = std::max(1, std::min((n + 255) / 256, 2048));
+  hipLaunchKernelGGL(cg32_init_kernel, dim3(nb), dim3(256), 0, st, n, b, invd, x, r, z, pprev, part_b, part_rz);
+  hipLaunchKernelGGL(cg32_init_fin_kernel, dim3(1), dim3(1024), 0, st, (CGState*)state, (const double*)part_b,
+                     (const double*)part_rz, nb, tol);
+}
+
+void launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
+                       const int* sidx, const float* z, const float* pprev, float* pnew, float* out, float* halo,
+                       const void* state, double* part_tile, hipStream_t st) {
+  hipLaunchKernelGGL(cg32_apply_kernel, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out, halo,
+                     prox, sd, sidx, (const CGState*)state, part_tile);
 }
 
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
@@ -1675,13 +1694,18 @@ void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, 
                            double* part_tile, double* part_line, double* part_rr, double* part_rz, hipStream_t st) {
   CGState* cs = (CGState*)state;
   const int nL = sd.nL;
-  hipLaunchKernelGGL(cg32_apply_kernel, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out, halo,
-                     prox, sd, sidx, iso, niso, cs, part_tile, nL > 0 ? 0 : 1);
+  const int nline = (nL + 255) / 256;
+  launch_cg32_apply(tm, ro, K, prox, sd, sidx, z, pprev, pnew, out, halo, cs, part_tile, st);
   if (nL > 0)
-    hipLaunchKernelGGL(cg32_lines_kernel, dim3((nL + 255) / 256), dim3(256), 0, st, sd, tm.nV, z, pprev, pnew, out, cs,
-                       part_line);
-  hipLaunchKernelGGL(cg32_update_kernel, dim3((tm.nV + nL + 255) / 256), dim3(256), 0, st, tm, sd, prox, sidx,
-                     (const float*)halo, (const float*)pnew, (const float*)out, invd, x, r, zout, cs, part_rr, part_rz);
+    hipLaunchKernelGGL(cg32_lines_kernel, dim3(nline), dim3(256), 0, st, sd, tm.nV, z, pprev, pnew, out,
+                       (const CGState*)cs, part_line);
+  hipLaunchKernelGGL(cg32_alpha_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_tile, tm.nTiles,
+                     (const double*)part_line, nL > 0 ? nline : 0, sd, prox, tm.nV, iso, niso, z, pprev, pnew);
+  const int nbu = (tm.nV + nL + 255) / 256;
+  hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, sidx, (const float*)halo,
+                     (const float*)pnew, (const float*)out, invd, x, r, zout, (const CGState*)cs, part_rr, part_rz);
+  hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_rr, (const double*)part_rz,
+                     nbu);
 }
 
 void launch_max_step(const double* cur, const double* dst, int nV, const int* conn, int nH, double* part,

@@ -47,7 +47,7 @@ class IterLog(C.Structure):
 
 
 class Opt32Options(C.Structure):
-    _fields_ = [("cg_iters", C.c_int), ("refine_iters", C.c_int)]
+    _fields_ = [("cg_iters", C.c_int), ("refine_iters", C.c_int), ("residual_every", C.c_int)]
 
 
 class Opt32Log(C.Structure):
@@ -229,10 +229,10 @@ class HexOpt:
                                             _ptr(Lp)), "debug_system")
         return rhs, invd, Lp
 
-    def opt32_solve(self, iters, cg_iters=20, refine_iters=0, weight_opt=1.0):
+    def opt32_solve(self, iters, cg_iters=20, refine_iters=0, weight_opt=1.0, residual_every=1):
         """FP32 + adjoint optimiser iterations from CURRENT (include/hexopt.h)."""
         prm = self.params(SYMMETRIC_DIRICHLET, weight_opt)
-        opt = Opt32Options(cg_iters, refine_iters)
+        opt = Opt32Options(cg_iters, refine_iters, residual_every)
         logs = (Opt32Log * max(iters, 1))()
         E = C.c_double()
         self._chk(lib().hexopt_opt32_solve(C.c_void_p(self.h), C.byref(prm), C.byref(opt), iters, logs, C.byref(E)),

@@ -105,15 +105,18 @@ int hexopt_slim_solve(hexopt_ctx* ctx, const hexopt_params* prm, int iters, hexo
  * corner-determinant sign test and the energy sums are FP64; the line search is the
  * reference's bisection. Symmetric Dirichlet only. */
 typedef struct {
-  int cg_iters;      /* FP32 CG iterations of the direction solve (the reference uses 20) */
-  int refine_iters;  /* FP32 CG iterations of the FP64-residual correction (0: none) */
+  int cg_iters;        /* FP32 CG iterations of the direction solve (the reference uses 20) */
+  int refine_iters;    /* FP32 CG iterations of the FP64-residual correction (0: none) */
+  int residual_every;  /* without correction, the FP64 residual is still measured on every
+                          residual_every-th iteration and the last (0: never); a correction
+                          always measures it */
 } hexopt_opt32_options;
 
 typedef struct {
   double energy, e_quality, e_soft, step_bound, step;
   int cg_iters;
   double cg_error;     /* relative residual estimate of the FP32 solve */
-  double residual64;   /* |b - L d| / |b| computed in FP64 before the correction */
+  double residual64;   /* |b - L d| / |b| computed in FP64 before the correction (-1: not measured) */
   int inverted_tets;   /* corner tets with negative determinant at the iterate (FP64 test) */
 } hexopt_opt32_log;
 
@@ -121,8 +124,9 @@ int hexopt_opt32_solve(hexopt_ctx* ctx, const hexopt_params* prm, const hexopt_o
                        hexopt_opt32_log* log, double* E);
 
 /* Average duration (ms, HIP events on the context's stream) of one launch of a kernel:
- * 0 FP64 CG operator tile pass, 1 FP32 CG operator tile pass, 2 FP64 local step tile pass,
- * 3 FP32 adjoint/local tile pass. Needs a prior solve (operands resident). */
+ * 0 FP64 CG operator tile pass, 1 FP32 fused CG operator pass (cg32_apply), 2 FP64 local step
+ * tile pass, 3 FP32 adjoint/local tile pass. Needs a prior solve of the same precision
+ * (operands resident). */
 int hexopt_time_kernel(hexopt_ctx* ctx, int which, int reps, double* ms_per_launch);
 
 /* Unnormalised SLIM energy of CURRENT: quality part and soft part (compute_energy_total,
