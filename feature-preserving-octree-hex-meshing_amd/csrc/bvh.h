@@ -38,8 +38,10 @@ constexpr int kBvhMaxDepth = 48;
 // Queries: P (SoA x[d*stride + i] when soa, else nq*3 rows), optional index list `ids`
 // (query i uses point ids[i]). Outputs (nullable): S, I, C (nq*3), N (nq*3), sqrD. status:
 // set to 1 if any traversal exceeded the stack depth (never for trees of depth <= 48).
+// `order` (nullable, a permutation of the queries) sets which query each lane takes: queries
+// of nearby points in neighbouring lanes walk the same tree paths.
 void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int stride, const int* ids, double* S, int* I,
-                    double* C, double* N, double* sqrD, int* status, hipStream_t st);
+                    double* C, double* N, double* sqrD, int* status, hipStream_t st, const int* order = nullptr);
 
 // (max, sum, count) of sqrt(sqrD[i]) over the queries below `upper`, into out3; part needs
 // 3 * 1024 doubles.

@@ -89,11 +89,12 @@ __device__ __forceinline__ unsigned fr_seen(unsigned f) { return (f >> 29) & 3u;
 __device__ __forceinline__ unsigned fr_make(int n, unsigned st, unsigned seen) { return (unsigned)n | (st << 26) | (seen << 29); }
 
 __global__ __launch_bounds__(kBvhBlock) void closest_kernel(SurfaceDev s, int nq, const double* P, int soa, int stride,
-                                                            const int* ids, double* S, int* I, double* Cout,
-                                                            double* Nout, double* sqrDout, int* status) {
+                                                            const int* ids, const int* order, double* S, int* I,
+                                                            double* Cout, double* Nout, double* sqrDout, int* status) {
   __shared__ unsigned stack[kBvhMaxDepth * kBvhBlock];
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nq) return;
+  const int q = order ? order[k] : k;
   const int pi = ids ? ids[q] : q;
   double p[3];
   if (soa) {
@@ -308,10 +309,10 @@ void launch_metro_reduce(int n, const double* sqrD, double upper, double* part, 
 }
 
 void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int stride, const int* ids, double* S, int* I,
-                    double* C, double* N, double* sqrD, int* status, hipStream_t st) {
+                    double* C, double* N, double* sqrD, int* status, hipStream_t st, const int* order) {
   if (nq <= 0) return;
   hipLaunchKernelGGL(closest_kernel, dim3((nq + kBvhBlock - 1) / kBvhBlock), dim3(kBvhBlock), 0, st, s, nq, P, soa,
-                     stride, ids, S, I, C, N, sqrD, status);
+                     stride, ids, order, S, I, C, N, sqrD, status);
 }
 
 }  // namespace hexopt

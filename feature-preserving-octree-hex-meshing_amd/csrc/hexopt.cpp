@@ -9,6 +9,7 @@
 //                                                             the accept test on the host
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -53,8 +54,10 @@ struct hexopt_ctx {
   MeshLayout L;
   std::string err;
   // layout on the device
-  DevBuf d_conn, d_tile_hex, d_tile_col, d_tile_lv, d_lv_gid, d_lv_slot, d_lidx, d_sv_gid, d_sv_off, d_sv_slots;
+  DevBuf d_conn, d_tile_hex, d_tile_col, d_tile_lv, d_lv_gid, d_lv_slot, d_lv_first, d_lidx, d_sv_gid, d_sv_off,
+      d_sv_slots;
   DevBuf d_sidx, d_iso;  // vertex -> fixup index (-1: written by its tile); vertices without hexes
+  DevBuf d_torder;       // plane queries sorted by vertex (Morton) order
   DevBuf d_cgsave;       // CG state of the last direction solve, read back with the line search
   int n_iso = 0;
   TileMesh tm{};
@@ -186,6 +189,7 @@ hexopt_ctx* hexopt_create(int nV, const double* V, int nH, const int32_t* hexes,
   rc |= upload(c, c->d_tile_lv, L.tile_lv);
   rc |= upload(c, c->d_lv_gid, L.lv_gid);
   rc |= upload(c, c->d_lv_slot, L.lv_slot);
+  rc |= upload(c, c->d_lv_first, L.lv_first);
   rc |= upload(c, c->d_lidx, L.lidx);
   rc |= upload(c, c->d_sv_gid, L.sv_gid);
   rc |= upload(c, c->d_sv_off, L.sv_off);
@@ -211,8 +215,8 @@ hexopt_ctx* hexopt_create(int nV, const double* V, int nH, const int32_t* hexes,
   }
   c->tm = TileMesh{L.nV, L.nH, L.nTiles, L.nShared, L.nSlots, c->d_conn.as<int>(), c->d_tile_hex.as<int>(),
                    c->d_tile_col.as<uint16_t>(), c->d_tile_lv.as<int>(), c->d_lv_gid.as<int>(),
-                   c->d_lv_slot.as<int>(), c->d_lidx.as<uint16_t>(), c->d_sv_gid.as<int>(), c->d_sv_off.as<int>(),
-                   c->d_sv_slots.as<int>()};
+                   c->d_lv_slot.as<int>(), c->d_lv_first.as<uint8_t>(), c->d_lidx.as<uint16_t>(),
+                   c->d_sv_gid.as<int>(), c->d_sv_off.as<int>(), c->d_sv_slots.as<int>()};
   hexopt_constraints none{};
   if (hexopt_set_constraints(c, &none) != HEXOPT_OK) {
     g_create_error = c->err;
@@ -394,6 +398,13 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
   rc |= upload(c, c->d_cids, cids);
   rc |= upload(c, c->d_cpos, vec(cs->C, (size_t)cs->nC * 3));
   rc |= upload(c, c->d_tids, tids);
+  {
+    // plane queries in the vertices' Morton order (for coherent closest-point traversals)
+    std::vector<int> order(cs->nT);
+    for (int i = 0; i < cs->nT; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tids[a] < tids[b]; });
+    rc |= upload(c, c->d_torder, order);
+  }
   rc |= upload(c, c->d_tn, vec(cs->normal_T, (size_t)cs->nT * 3));
   rc |= upload(c, c->d_td, vec(cs->dis_T, (size_t)cs->nT));
   rc |= upload(c, c->d_lv, lv);
@@ -938,7 +949,8 @@ int hexopt_project_planes(hexopt_ctx* c, hexopt_surface* s, double lamda_T) {
   // closest point C and pseudonormal N of every plane vertex at CURRENT (projection_smooth,
   // grid_hex_meshing.cpp:3760-3781): normal_T = N, dis_T = N . C
   launch_closest(s->view, nT, c->d_cur.as<double>(), 1, c->L.nV, c->d_tids.as<int>(), nullptr, nullptr,
-                 s->dC.as<double>(), const_cast<double*>(c->sd.t_n), nullptr, s->dstat.as<int>(), st);
+                 s->dC.as<double>(), const_cast<double*>(c->sd.t_n), nullptr, s->dstat.as<int>(), st,
+                 c->d_torder.as<int>());
   launch_refold_planes(c->sd, nT, s->dC.as<double>(), const_cast<double*>(c->sd.t_d), c->d_S0.as<double>(),
                        c->d_b0.as<double>(), const_cast<double*>(c->sd.S), const_cast<double*>(c->sd.b), lamda_T, st);
   c->sd.lamT = lamda_T;

@@ -15,6 +15,7 @@ struct TileMesh {
   const int* tile_lv;
   const int* lv_gid;
   const int* lv_slot;
+  const uint8_t* lv_first;
   const uint16_t* lidx;
   const int* sv_gid;
   const int* sv_off;

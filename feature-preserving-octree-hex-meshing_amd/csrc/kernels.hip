@@ -945,8 +945,7 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
       sp[d * kMaxLV + lv] = pv[d];
       acc[d * kMaxLV + lv] = 0.0f;
     }
-    const bool mine = s < 0 || tm.sv_slots[tm.sv_off[sidx[g]]] == s;
-    if (mine) {
+    if (tm.lv_first[lv0 + lv]) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) pnew[(size_t)d * nV + g] = pv[d];
     }
@@ -992,7 +991,7 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
   for (int lv = tid; lv < nlv; lv += kTileHex) {
     const int g = tm.lv_gid[lv0 + lv], s = tm.lv_slot[lv0 + lv];
     const bool owned = s < 0;
-    const bool counted = owned || tm.sv_slots[tm.sv_off[sidx[g]]] == s;
+    const bool counted = tm.lv_first[lv0 + lv] != 0;
     if (!owned) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) halo[(size_t)d * tm.nSlots + s] = acc[d * kMaxLV + lv];

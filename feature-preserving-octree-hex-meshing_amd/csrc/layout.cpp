@@ -164,9 +164,11 @@ bool build_layout(int nV, const double* V, int nH, const int* H, MeshLayout& L, 
       if (ntiles_of_v[v] > 1) {
         int s = slot_next++;
         L.lv_slot.push_back(s);
+        L.lv_first.push_back(slot_count[v] == 0 ? 1 : 0);
         L.sv_slots[vslot_start[v] + slot_count[v]++] = s;
       } else {
         L.lv_slot.push_back(-1);
+        L.lv_first.push_back(1);
       }
       local[v] = -1;
     }

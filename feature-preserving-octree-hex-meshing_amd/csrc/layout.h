@@ -29,6 +29,8 @@ struct MeshLayout {
   std::vector<int> tile_lv;     // nTiles+1 local-vertex ranges
   std::vector<int> lv_gid;      // local vertex -> new vertex id
   std::vector<int> lv_slot;     // -1: written directly; else halo slot
+  std::vector<uint8_t> lv_first;  // 1: the vertex is this tile's alone, or this tile holds its
+                                  //    first halo slot (the one tile that writes per-vertex data)
   std::vector<uint16_t> lidx;   // [8][nH] local vertex of each corner within its tile
   std::vector<int> sv_gid;      // vertices finished by the fixup pass (shared or isolated)
   std::vector<int> sv_off;      // nShared+1 ranges into sv_slots

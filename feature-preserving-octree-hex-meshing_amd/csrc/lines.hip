@@ -1,9 +1,9 @@
 // lines.hip -- feature-curve projection of LINE vertices, FP64: the LINE branch of
-// dirty_graph_projection (grid_hex_meshing.cpp:3961-4026). One lane per vertex walks the
-// segments of its curve (point_line_projection, global_functions.cpp:3454-3465) and keeps
-// the nearest; a strict '<' keeps the lowest segment index on equal distances, which is the
-// first entry after the reference's sort of (distance, index) pairs. Built without fused
-// multiply-add; sums of three products add x, y, z in that order.
+// dirty_graph_projection (grid_hex_meshing.cpp:3961-4026). One wave per vertex projects it
+// onto the segments of its curve (point_line_projection, global_functions.cpp:3454-3465) and
+// keeps the nearest, the lowest segment index on equal distances, which is the first entry
+// after the reference's sort of (distance, index) pairs. Built without fused multiply-add;
+// sums of three products add x, y, z in that order.
 #include <hip/hip_runtime.h>
 #include "lines.h"
 #include "kernels.h"
@@ -22,10 +22,34 @@ __device__ __forceinline__ void normalized_diff(const double* a, const double* b
   o[2] = d2 / n;
 }
 
-__global__ void project_lines_kernel(CurvesDev cv, const double* __restrict__ X, int nV, int n,
-                                     const int* __restrict__ rows, const int* __restrict__ curve_of,
-                                     double* __restrict__ origin, double* __restrict__ axis) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// point_line_projection of v onto segment p0 p1: the closest point and its distance
+__device__ __forceinline__ double segment_point(const double v[3], const double* p0, const double* p1, double q3[3]) {
+  const double a0 = v[0] - p0[0], a1 = v[1] - p0[1], a2 = v[2] - p0[2];
+  const double e0 = p1[0] - p0[0], e1 = p1[1] - p0[1], e2 = p1[2] - p0[2];
+  const double l2 = e0 * e0 + e1 * e1 + e2 * e2;
+  double t = 0;
+  if (l2 >= 1.e-7) t = (a0 * e0 + a1 * e1 + a2 * e2) / l2;
+  if (t >= 0.0 && t <= 1.0) {
+    for (int d = 0; d < 3; ++d) q3[d] = p0[d] + t * (p1[d] - p0[d]);
+  } else if (t < 0.0) {
+    for (int d = 0; d < 3; ++d) q3[d] = p0[d];
+  } else {
+    for (int d = 0; d < 3; ++d) q3[d] = p1[d];
+  }
+  return norm3(v[0] - q3[0], v[1] - q3[1], v[2] - q3[2]);
+}
+
+// One wave per LINE vertex: lanes take every 64th segment, then the wave keeps the nearest
+// (ties to the lower segment index, the first entry of the reference's sorted
+// (distance, index) list; a NaN distance at segment 0 wins as in a sequential strict-< scan).
+constexpr int kLineWaves = 4;
+__global__ __launch_bounds__(64 * kLineWaves) void project_lines_kernel(CurvesDev cv, const double* __restrict__ X,
+                                                                        int nV, int n, const int* __restrict__ rows,
+                                                                        const int* __restrict__ curve_of,
+                                                                        double* __restrict__ origin,
+                                                                        double* __restrict__ axis) {
+  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.x * kLineWaves + (threadIdx.x >> 6);
   if (q >= n) return;
   const int iv = rows[q];
   const double v[3] = {X[iv], X[(size_t)nV + iv], X[2 * (size_t)nV + iv]};
@@ -36,34 +60,36 @@ __global__ void project_lines_kernel(CurvesDev cv, const double* __restrict__ X,
   const int curve_len = circle ? csize : csize - 1;
   double pv[3] = {0, 0, 0}, tangent[3] = {1, 0, 0};
   if (curve_len > 0) {
-    double best = 0;
-    for (int j = 0; j < curve_len; ++j) {
-      const double* p0 = cv.V + (size_t)curve[j] * 3;
-      const double* p1 = cv.V + (size_t)curve[(j + 1) % csize] * 3;
-      // point_line_projection
-      const double a0 = v[0] - p0[0], a1 = v[1] - p0[1], a2 = v[2] - p0[2];
-      const double e0 = p1[0] - p0[0], e1 = p1[1] - p0[1], e2 = p1[2] - p0[2];
-      const double l2 = e0 * e0 + e1 * e1 + e2 * e2;
-      double t = 0;
-      if (l2 >= 1.e-7) t = (a0 * e0 + a1 * e1 + a2 * e2) / l2;
+    // sequential scan (best = d[0], replaced when d[j] < best): a NaN at segment 0 is kept;
+    // otherwise the result is the first minimum over the segments with a non-NaN distance
+    double best = INFINITY;
+    int bj = -1, nan0 = 0;
+    for (int j = lane; j < curve_len; j += 64) {
       double q3[3];
-      if (t >= 0.0 && t <= 1.0) {
-        for (int d = 0; d < 3; ++d) q3[d] = p0[d] + t * (p1[d] - p0[d]);
-      } else if (t < 0.0) {
-        for (int d = 0; d < 3; ++d) q3[d] = p0[d];
-      } else {
-        for (int d = 0; d < 3; ++d) q3[d] = p1[d];
+      const double dist = segment_point(v, cv.V + (size_t)curve[j] * 3, cv.V + (size_t)curve[(j + 1) % csize] * 3, q3);
+      if (dist != dist) {
+        nan0 |= j == 0;
+        continue;
       }
-      const double dist = norm3(v[0] - q3[0], v[1] - q3[1], v[2] - q3[2]);
-      if (j == 0 || dist < best) {
-        best = dist;
-        for (int d = 0; d < 3; ++d) pv[d] = q3[d];
-        normalized_diff(p1, p0, tangent);
-      }
+      if (bj < 0 || dist < best) { best = dist; bj = j; }
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int oj = __shfl_xor(bj, o, 64);
+      nan0 |= __shfl_xor(nan0, o, 64);
+      if (oj >= 0 && (bj < 0 || ob < best || (ob == best && oj < bj))) { best = ob; bj = oj; }
+    }
+    if (lane != 0) return;
+    if (nan0) bj = 0;
+    const double* p0 = cv.V + (size_t)curve[bj] * 3;
+    const double* p1 = cv.V + (size_t)curve[(bj + 1) % csize] * 3;
+    segment_point(v, p0, p1, pv);
+    normalized_diff(p1, p0, tangent);
   } else {
     // no segment (an open curve of one vertex): nearest curve vertex, tangent from its
     // neighbours added to (1, 0, 0), then normalised
+    if (lane != 0) return;
     int pos = 0;
     double best = 0;
     for (int j = 0; j < csize; ++j) {
@@ -140,8 +166,8 @@ void launch_refold_lines(const SoftDev& sd, const double* origin, const double* 
 void launch_project_lines(const CurvesDev& cv, const double* X, int nV, int n, const int* rows, const int* curve_of,
                           double* origin, double* axis, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(project_lines_kernel, dim3((n + 127) / 128), dim3(128), 0, st, cv, X, nV, n, rows, curve_of, origin,
-                     axis);
+  hipLaunchKernelGGL(project_lines_kernel, dim3((n + kLineWaves - 1) / kLineWaves), dim3(64 * kLineWaves), 0, st, cv, X,
+                     nV, n, rows, curve_of, origin, axis);
 }
 
 }  // namespace hexopt
