@@ -1,6 +1,6 @@
 """Benchmark: 1M-hex optimiser iterations/s, FP32+adjoint vs FP64 plain, on one MI355X.
 
-Workload (config 4 shape): the torus R=1, r=0.35 cut by the planes z=+-0.2, triangulated
+Workload: the torus R=1, r=0.35 cut by the planes z=+-0.2, triangulated
 with 40,000 triangles (two tube bands and two flat annuli meeting at four sharp rim
 circles), and a uniform-level octree hex mesh of ~1.0M hexes of its inside. Boundary
 vertices near a rim are LINE vertices on that rim (feature curve); the other boundary
@@ -29,9 +29,21 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "feature-preserving-octree-hex-meshing_amd"))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "cg32_apply_torus1m_hbm_traffic.md")
 R_MAJ, R_MIN, Z_CUT = 1.0, 0.35, 0.2
 WEIGHT_OPT = 0.001          # feature_alignment's opt.weight_opt = weight_opt * 0.001
 LAMDA0, LAMDA_BOUND = 1.0, 1e8
+
+
+def measured_traffic(nH):
+    """HBM bytes per launch of cg32_apply_kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a device copy of known size), when that summary
+    was taken on a mesh of this many hexes; None otherwise."""
+    try:
+        kv = dict(l.split(":", 1) for l in open(TRAFFIC_PROFILE) if l.startswith(("traffic_bytes_per_launch:", "hexes:")))
+        return int(kv["traffic_bytes_per_launch"]) if int(kv["hexes"]) == nH else None
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def build_workload(target):
@@ -155,6 +167,7 @@ def main():
     # each) z and the previous direction read, the new direction and L p written
     bytes_per_hex32 = 192 + 16 + 4 * 12 * rV
     achieved = bytes_per_hex32 * nH / (r32["ms_apply"] * 1e-3) / 1e9
+    traffic = measured_traffic(nH)
     value = args.steps / r32["secs"]
     diag = float(np.linalg.norm(w["SV"].max(axis=0) - w["SV"].min(axis=0)))
     out = {
@@ -188,9 +201,10 @@ def main():
         "max_boundary_distance_diff_over_bbox_diag": abs(r32["maxdist"] - r64["maxdist"]) / diag,
         "element_evals_per_s": round(value * nH, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "cg32_apply_kernel (fused FP32 CG iteration, operator half)",
-                     "bytes_per_hex": round(bytes_per_hex32, 2)},
+                     "bytes_per_hex": round(bytes_per_hex32, 2), "algorithmic_bytes": round(bytes_per_hex32 * nH),
+                     "traffic_source": "profiles/cg32_apply_torus1m_hbm_traffic.md" if traffic else None},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(max(args.hexes // 8, 20000))
