@@ -16,6 +16,13 @@ struct SurfaceTree {
 };
 
 bool build_surface_tree(int nv, const double* V, int nf, const int* F, SurfaceTree& T, const char** err);
+
+// Node records for the closest-point walk, kNodeRec doubles (128 B) per node, so that one
+// node visit is one dependent fetch. Inner node: its left child's box, then its right
+// child's box (12 doubles). Leaf: its face's corners a, b, c (9 doubles). In both, the int32
+// words 24..26 (doubles 12..13) hold left, right, prim; the rest is zero.
+constexpr int kNodeRec = 16;
+void pack_node_records(const SurfaceTree& T, const double* V, const int* F, std::vector<double>& rec);
 void surface_normals(int nv, const double* V, int nf, const int* F, std::vector<double>& FN, std::vector<double>& VN,
                      std::vector<double>& FEN);
 
@@ -30,6 +37,7 @@ struct SurfaceDev {
   const double* FN;   // nf*3
   const double* VN;   // nv*3
   const double* FEN;  // nf*9
+  const double* rec;  // nnodes*kNodeRec (pack_node_records)
 };
 
 constexpr int kBvhBlock = 128;

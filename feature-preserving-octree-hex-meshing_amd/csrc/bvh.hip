@@ -33,7 +33,7 @@ __device__ __forceinline__ double box_ext_dist(const double* b, const double p[3
 __device__ __forceinline__ double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
 // Ericson's closest point on a triangle (point_simplex_squared_distance.cpp:44-109)
-__device__ void closest_on_tri(const double p[3], const double a[3], const double b[3], const double c[3], double r[3]) {
+__device__ __forceinline__ void closest_on_tri(const double p[3], const double a[3], const double b[3], const double c[3], double r[3]) {
   double ab[3], ac[3], ap[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) { ab[k] = b[k] - a[k]; ac[k] = c[k] - a[k]; ap[k] = p[k] - a[k]; }
@@ -113,21 +113,25 @@ __global__ __launch_bounds__(kBvhBlock) void closest_kernel(SurfaceDev s, int nq
   while (top > 0) {
     unsigned f = stk[(top - 1) * kBvhBlock];
     const int n = fr_node(f);
-    const int pr = s.prim[n];
+    // the node's record in one go: children's boxes (inner) or the face's corners (leaf)
+    const double2* r2 = reinterpret_cast<const double2*>(s.rec + (size_t)n * kNodeRec);
+    double x[12];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) { const double2 v = r2[j]; x[2 * j] = v.x; x[2 * j + 1] = v.y; }
+    const int4 w = reinterpret_cast<const int4*>(r2)[6];
+    const int pr = w.z;
     if (pr >= 0) {  // leaf_squared_distance + set_min
-      double a[3], b[3], c[3], r[3];
-      load3(s.V, s.F[(size_t)pr * 3], a);
-      load3(s.V, s.F[(size_t)pr * 3 + 1], b);
-      load3(s.V, s.F[(size_t)pr * 3 + 2], c);
+      const double a[3] = {x[0], x[1], x[2]}, b[3] = {x[3], x[4], x[5]}, c[3] = {x[6], x[7], x[8]};
+      double r[3];
       closest_on_tri(p, a, b, c, r);
       const double d = (p[0] - r[0]) * (p[0] - r[0]) + (p[1] - r[1]) * (p[1] - r[1]) + (p[2] - r[2]) * (p[2] - r[2]);
       if (d < best) { best = d; bf = pr; bc[0] = r[0]; bc[1] = r[1]; bc[2] = r[2]; }
       --top;
       continue;
     }
-    const int L = s.left[n], R = s.right[n];
-    const double* bl = s.box + (size_t)L * 6;
-    const double* br = s.box + (size_t)R * 6;
+    const int L = w.x, R = w.y;
+    const double* bl = x;
+    const double* br = x + 6;
     unsigned state = fr_state(f), seen = fr_seen(f);
     int child = -1;
     while (child < 0 && state < 4) {
@@ -211,12 +215,17 @@ __global__ __launch_bounds__(kBvhBlock) void closest_kernel(SurfaceDev s, int nq
   } else {
     bool found = false;
     const double* cv[3] = {A, B, Cc};
-    for (int v = 0; v < 3 && !found; ++v) {
+    // loops unrolled so that the corner picks are compile-time (no scratch for A, B, Cc)
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      if (found) continue;
       const double* w = cv[v];
       double dd = sqrt((bc[0] - w[0]) * (bc[0] - w[0]) + (bc[1] - w[1]) * (bc[1] - w[1]) + (bc[2] - w[2]) * (bc[2] - w[2]));
       if (dd < eps) { found = true; load3(s.VN, s.F[(size_t)bf * 3 + v], nn); }
     }
-    for (int e = 0; e < 3 && !found; ++e) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      if (found) continue;
       const double* s0 = cv[(e + 1) % 3];
       const double* s1 = cv[(e + 2) % 3];
       double dv[3] = {s1[0] - s0[0], s1[1] - s0[1], s1[2] - s0[2]};

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -107,6 +108,23 @@ bool build_surface_tree(int nv, const double* V, int nf, const int* F, SurfaceTr
   }
   T.depth = maxd + 1;
   return true;
+}
+
+void pack_node_records(const SurfaceTree& T, const double* V, const int* F, std::vector<double>& rec) {
+  const size_t nn = T.prim.size();
+  rec.assign(nn * kNodeRec, 0.0);
+  for (size_t i = 0; i < nn; ++i) {
+    double* r = &rec[i * kNodeRec];
+    if (T.prim[i] >= 0) {
+      for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d) r[c * 3 + d] = V[(size_t)F[(size_t)T.prim[i] * 3 + c] * 3 + d];
+    } else {
+      for (int d = 0; d < 6; ++d) r[d] = T.box[(size_t)T.left[i] * 6 + d];
+      for (int d = 0; d < 6; ++d) r[6 + d] = T.box[(size_t)T.right[i] * 6 + d];
+    }
+    const int32_t w[4] = {T.left[i], T.right[i], T.prim[i], 0};
+    std::memcpy(r + 12, w, sizeof(w));
+  }
 }
 
 void surface_normals(int nv, const double* V, int nf, const int* F, std::vector<double>& FN, std::vector<double>& VN,

@@ -54,8 +54,8 @@ struct hexopt_ctx {
   MeshLayout L;
   std::string err;
   // layout on the device
-  DevBuf d_conn, d_tile_hex, d_tile_col, d_tile_lv, d_lv_gid, d_lv_slot, d_lv_first, d_lidx, d_sv_gid, d_sv_off,
-      d_sv_slots;
+  DevBuf d_conn, d_tile_hex, d_tile_col, d_tile_lv, d_lv_gid, d_lv_slot, d_lv_first, d_lidx, d_tile_corner, d_sv_gid,
+      d_sv_off, d_sv_slots;
   DevBuf d_sidx, d_iso;  // vertex -> fixup index (-1: written by its tile); vertices without hexes
   DevBuf d_torder;       // plane queries sorted by vertex (Morton) order
   DevBuf d_cgsave;       // CG state of the last direction solve, read back with the line search
@@ -191,6 +191,7 @@ hexopt_ctx* hexopt_create(int nV, const double* V, int nH, const int32_t* hexes,
   rc |= upload(c, c->d_lv_slot, L.lv_slot);
   rc |= upload(c, c->d_lv_first, L.lv_first);
   rc |= upload(c, c->d_lidx, L.lidx);
+  rc |= upload(c, c->d_tile_corner, L.tile_corner);
   rc |= upload(c, c->d_sv_gid, L.sv_gid);
   rc |= upload(c, c->d_sv_off, L.sv_off);
   rc |= upload(c, c->d_sv_slots, L.sv_slots);
@@ -216,7 +217,8 @@ hexopt_ctx* hexopt_create(int nV, const double* V, int nH, const int32_t* hexes,
   c->tm = TileMesh{L.nV, L.nH, L.nTiles, L.nShared, L.nSlots, c->d_conn.as<int>(), c->d_tile_hex.as<int>(),
                    c->d_tile_col.as<uint16_t>(), c->d_tile_lv.as<int>(), c->d_lv_gid.as<int>(),
                    c->d_lv_slot.as<int>(), c->d_lv_first.as<uint8_t>(), c->d_lidx.as<uint16_t>(),
-                   c->d_sv_gid.as<int>(), c->d_sv_off.as<int>(), c->d_sv_slots.as<int>()};
+                   c->d_sv_gid.as<int>(), c->d_sv_off.as<int>(), c->d_sv_slots.as<int>(),
+                   c->d_tile_corner.as<uint8_t>()};
   hexopt_constraints none{};
   if (hexopt_set_constraints(c, &none) != HEXOPT_OK) {
     g_create_error = c->err;
@@ -810,7 +812,7 @@ struct hexopt_surface {
   SurfaceTree T;
   int nv = 0, nf = 0;
   std::vector<double> FN, VN, FEN;
-  DevBuf dV, dF, dbox, dl, dr, dp, dFN, dVN, dFEN, dstat, dP, dS, dI, dC, dN;
+  DevBuf dV, dF, dbox, dl, dr, dp, dFN, dVN, dFEN, drec, dstat, dP, dS, dI, dC, dN;
   SurfaceDev view{};
   // occupancy: column bins (built on first use), cell and flag staging
   bool bins_ready = false;
@@ -870,6 +872,9 @@ hexopt_surface* hexopt_surface_create(int nv, const double* V, int nf, const int
   rc |= sup(s, s->dFN, s->FN.data(), s->FN.size());
   rc |= sup(s, s->dVN, s->VN.data(), s->VN.size());
   rc |= sup(s, s->dFEN, s->FEN.data(), s->FEN.size());
+  std::vector<double> rec;
+  pack_node_records(s->T, V, F, rec);
+  rc |= sup(s, s->drec, rec.data(), rec.size());
   if (rc || s->dstat.ensure(sizeof(int)) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) {
     g_surface_error = "device upload failed";
     hexopt_surface_destroy(s);
@@ -877,7 +882,7 @@ hexopt_surface* hexopt_surface_create(int nv, const double* V, int nf, const int
   }
   s->view = SurfaceDev{nv, nf, (int)s->T.prim.size(), s->T.depth, s->dV.as<double>(), s->dF.as<int>(),
                        s->dbox.as<double>(), s->dl.as<int>(), s->dr.as<int>(), s->dp.as<int>(), s->dFN.as<double>(),
-                       s->dVN.as<double>(), s->dFEN.as<double>()};
+                       s->dVN.as<double>(), s->dFEN.as<double>(), s->drec.as<double>()};
   return s;
 }
 
@@ -1356,6 +1361,8 @@ int hexopt_synchronize(hexopt_ctx* c) {
 int hexopt_layout_info(hexopt_ctx* c, int32_t* o) {
   if (!c || !o) return HEXOPT_EINVAL;
   o[0] = c->L.nTiles; o[1] = c->L.nShared; o[2] = c->L.nSlots; o[3] = c->L.max_colours; o[4] = c->ref_mode;
+  o[5] = 0;
+  for (uint8_t f : c->L.tile_corner) o[5] += f;
   return HEXOPT_OK;
 }
 

@@ -20,6 +20,7 @@ struct TileMesh {
   const int* sv_gid;
   const int* sv_off;
   const int* sv_slots;
+  const uint8_t* tile_corner;
 };
 
 enum RefMode { REF_CUBE = 0, REF_GENERAL = 1 };

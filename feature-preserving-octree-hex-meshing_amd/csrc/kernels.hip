@@ -152,6 +152,70 @@ __device__ __forceinline__ void gather8(const P* x, int nV, const int* conn, int
     for (int d = 0; d < 3; ++d) p[k][d] = (double)x[(size_t)d * nV + vid[k]];
 }
 
+// Operator blocks K: 48 values per hex (the 6 symmetric components of each corner tet,
+// tet-major), grouped in 16-byte vectors so that one load brings 4 floats or 2 doubles:
+// value m of hex h sits at element (m / W * nH + h) * W + m % W, W = 16 / sizeof(T).
+template <class T>
+__device__ __forceinline__ size_t k_at(int m, int h, int nH) {
+  constexpr int W = 16 / (int)sizeof(T);
+  return ((size_t)(m / W) * nH + h) * W + m % W;
+}
+
+// the 12 values of corner tets 2q and 2q+1
+template <class CT>
+__device__ __forceinline__ void k_pair(const float* K, int nH, int h, int q, CT k[12]) {
+  const float4* K4 = reinterpret_cast<const float4*>(K);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float4 v = K4[(size_t)(3 * q + i) * nH + h];
+    k[4 * i] = (CT)v.x; k[4 * i + 1] = (CT)v.y; k[4 * i + 2] = (CT)v.z; k[4 * i + 3] = (CT)v.w;
+  }
+}
+template <class CT>
+__device__ __forceinline__ void k_pair(const double* K, int nH, int h, int q, CT k[12]) {
+  const double2* K2 = reinterpret_cast<const double2*>(K);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const double2 v = K2[(size_t)(6 * q + i) * nH + h];
+    k[2 * i] = (CT)v.x; k[2 * i + 1] = (CT)v.y;
+  }
+}
+
+// Adds the tile's per-hex corner values c[8][NC] into its LDS accumulator (NC planes of
+// kMaxLV), without atomics and in a fixed order per vertex, so the sums are deterministic.
+// Tiles where no vertex sits at the same corner index of two hexes (tile_corner, true for
+// every tile of a consistently oriented grid) add corner k of all hexes in phase k: 8
+// phases with every thread busy. Other tiles add hex colour by colour. The caller has made
+// the accumulator's initial values visible (a barrier); the last phase ends in one.
+template <class T, int NC>
+__device__ __forceinline__ void scatter_tile(const TileMesh& tm, int tile, int nh, bool active, const int li[8],
+                                             const T c[8][NC], T* acc) {
+  const int tid = threadIdx.x;
+  if (tm.tile_corner[tile]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (active) {
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[j * kMaxLV + li[k]] += c[k][j];
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  const uint16_t* off = tm.tile_col + (size_t)tile * (kMaxCol + 1);
+  for (int col = 0; col < kMaxCol; ++col) {
+    const int a = off[col], b = off[col + 1];
+    if (a >= nh) break;
+    if (active && tid >= a && tid < b) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[j * kMaxLV + li[k]] += c[k][j];
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Tiled scatter skeleton. Op provides:
 //   __device__ double hex(int h, T out[8][NC])        per-hex corner contributions; returns
@@ -186,18 +250,7 @@ __global__ __launch_bounds__(kTileHex) void tile_scatter_kernel(TileMesh tm, Op 
     if (tid == 0) hex_partial[tile] = sh;
   }
   __syncthreads();
-  const uint16_t* off = tm.tile_col + (size_t)tile * (kMaxCol + 1);
-  for (int col = 0; col < kMaxCol; ++col) {
-    const int a = off[col], b = off[col + 1];
-    if (a >= nh) break;
-    if (active && tid >= a && tid < b) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-#pragma unroll
-        for (int j = 0; j < NC; ++j) acc[j * kMaxLV + li[k]] += c[k][j];
-    }
-    __syncthreads();
-  }
+  scatter_tile<T, NC>(tm, tile, nh, active, li, c, acc);
   double part = 0.0;
   for (int lv = tid; lv < nlv; lv += kTileHex) {
     const int g = tm.lv_gid[lv0 + lv], s = tm.lv_slot[lv0 + lv];
@@ -437,7 +490,7 @@ struct LocalStepOp {
       if (ro.mode == REF_CUBE) { double s = ro.invs[(size_t)c * nH + h]; s2 = s * s; }
       const double ks[6] = {Kt[0][0], Kt[1][1], Kt[2][2], Kt[0][1], Kt[0][2], Kt[1][2]};
 #pragma unroll
-      for (int j = 0; j < 6; ++j) K[((size_t)c * 6 + j) * nH + h] = ks[j] * s2;
+      for (int j = 0; j < 6; ++j) K[k_at<double>(c * 6 + j, h, nH)] = ks[j] * s2;
       // rhs: (W^T M W R) scattered through B
       double Z[3][3];
       {
@@ -501,13 +554,14 @@ struct LocalStepOp {
 // values q of the direction, with the per-tet K = invs^2 M W W^T (6 symmetric components).
 template <class KT, class CT>
 __device__ __forceinline__ void hex_forces(const RefOp& ro, const KT* K, int nH, int h, const CT q[8][3], CT o[8][3]) {
+  CT kp[12];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     CT E[3][3], B[3][3], Y[3][3];
     tet_edges(q, c, E);
-    const KT* kk = K + (size_t)c * 6 * nH + h;
-    const CT k00 = (CT)kk[0], k11 = (CT)kk[(size_t)nH], k22 = (CT)kk[2 * (size_t)nH], k01 = (CT)kk[3 * (size_t)nH],
-             k02 = (CT)kk[4 * (size_t)nH], k12 = (CT)kk[5 * (size_t)nH];
+    if ((c & 1) == 0) k_pair(K, nH, h, c >> 1, kp);
+    const CT* kk = kp + 6 * (c & 1);
+    const CT k00 = kk[0], k11 = kk[1], k22 = kk[2], k01 = kk[3], k02 = kk[4], k12 = kk[5];
     if (ro.mode == REF_CUBE) {
       // invs^2 is folded into K: forces on the tet edges are K E (columns = edges)
 #pragma unroll
@@ -622,7 +676,7 @@ struct AdjLocalOp32 {
       if (ro.mode == REF_CUBE) { float s = (float)ro.invs[(size_t)c * nH + h]; s2 = s * s; }
       const float ks[6] = {Kt[0][0], Kt[1][1], Kt[2][2], Kt[0][1], Kt[0][2], Kt[1][2]};
 #pragma unroll
-      for (int j = 0; j < 6; ++j) K[((size_t)c * 6 + j) * nH + h] = ks[j] * s2;
+      for (int j = 0; j < 6; ++j) K[k_at<float>(c * 6 + j, h, nH)] = ks[j] * s2;
       const float hm = -0.5f * (float)M;
 #pragma unroll
       for (int i = 0; i < 3; ++i)
@@ -976,18 +1030,7 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
       for (int d = 0; d < 3; ++d) e += q[k][d] * c[k][d];
     part = (double)e;
   }
-  const uint16_t* off = tm.tile_col + (size_t)tile * (kMaxCol + 1);
-  for (int col = 0; col < kMaxCol; ++col) {
-    const int a = off[col], b = off[col + 1];
-    if (a >= nh) break;
-    if (active && tid >= a && tid < b) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) acc[d * kMaxLV + li[k]] += c[k][d];
-    }
-    __syncthreads();
-  }
+  scatter_tile<float, 3>(tm, tile, nh, active, li, c, acc);
   for (int lv = tid; lv < nlv; lv += kTileHex) {
     const int g = tm.lv_gid[lv0 + lv], s = tm.lv_slot[lv0 + lv];
     const bool owned = s < 0;

@@ -159,6 +159,17 @@ bool build_layout(int nV, const double* V, int nH, const int* H, MeshLayout& L, 
         L.lidx[(size_t)k * nH + i] = (uint16_t)local[v];
       }
     if ((int)lv.size() > kMaxLV) { *err = "tile local vertex overflow"; return false; }
+    {
+      std::vector<uint8_t> seen(lv.size(), 0);
+      uint8_t ok = 1;
+      for (int i = L.tile_hex[t]; i < L.tile_hex[t + 1]; ++i)
+        for (int k = 0; k < 8; ++k) {
+          uint8_t& m = seen[L.lidx[(size_t)k * nH + i]];
+          if (m & (1u << k)) ok = 0;
+          m |= (uint8_t)(1u << k);
+        }
+      L.tile_corner.push_back(ok);
+    }
     for (int v : lv) {
       L.lv_gid.push_back(v);
       if (ntiles_of_v[v] > 1) {

@@ -278,9 +278,9 @@ class HexOpt:
         self._chk(lib().hexopt_synchronize(C.c_void_p(self.h)), "synchronize")
 
     def layout_info(self):
-        out = np.zeros(5, dtype=np.int32)
+        out = np.zeros(6, dtype=np.int32)
         self._chk(lib().hexopt_layout_info(C.c_void_p(self.h), _ptr(out, C.c_int32)), "layout_info")
-        return dict(zip(["tiles", "shared", "slots", "max_colours", "ref_mode"], out.tolist()))
+        return dict(zip(["tiles", "shared", "slots", "max_colours", "ref_mode", "corner_tiles"], out.tolist()))
 
 
 def surface_normals(V, F):

@@ -53,6 +53,37 @@ def invert_hexes(V, H, frac, seed=0):
     return out, np.sort(pick)
 
 
+def cube_rotations():
+    """The 24 proper rotations of the cube as corner permutations: hex H[:, perm] is the same
+    hex, positively oriented, with its corners numbered from another corner and axis."""
+    C = 2 * HEX_CORNERS - 1
+    key = {tuple(c): k for k, c in enumerate(C.tolist())}
+    out = []
+    for p in ([0, 1, 2], [1, 2, 0], [2, 0, 1], [0, 2, 1], [2, 1, 0], [1, 0, 2]):
+        for sx in (1, -1):
+            for sy in (1, -1):
+                for sz in (1, -1):
+                    R = np.zeros((3, 3), dtype=np.int64)
+                    R[0, p[0]], R[1, p[1]], R[2, p[2]] = sx, sy, sz
+                    if round(np.linalg.det(R)) != 1:
+                        continue
+                    out.append([key[tuple((R @ c).tolist())] for c in C])
+    return np.array(out, dtype=np.int64)
+
+
+def rotate_hex_corners(H, frac, seed=0):
+    """Renumber the corners of a seeded fraction of hexes by random proper cube rotations
+    (other than the identity). The mesh, each hex's orientation and its 8 corner tets are
+    unchanged; only which corner is called 0 differs."""
+    rng = np.random.default_rng(seed)
+    rots = cube_rotations()[1:]
+    out = H.copy()
+    pick = rng.choice(H.shape[0], size=int(round(frac * H.shape[0])), replace=False)
+    which = rng.integers(0, rots.shape[0], size=pick.size)
+    out[pick] = np.take_along_axis(H[pick], rots[which], axis=1)
+    return out
+
+
 def icosphere(level):
     """Unit icosphere with 20*4^level triangles (outward orientation)."""
     p = (1.0 + 5 ** 0.5) / 2.0

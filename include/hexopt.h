@@ -238,8 +238,10 @@ int hexopt_refresh_lines(hexopt_ctx* ctx, hexopt_curves* cv, const int32_t* curv
 /* Blocks until the context's stream is idle. */
 int hexopt_synchronize(hexopt_ctx* ctx);
 
-/* Layout facts for tests/benchmarks: [nTiles, nShared, nSlots, max_colours, ref_mode]. */
-int hexopt_layout_info(hexopt_ctx* ctx, int32_t* out5);
+/* Layout facts for tests/benchmarks: [nTiles, nShared, nSlots, max_colours, ref_mode,
+ * corner_tiles]. corner_tiles counts the tiles whose scatter runs by corner index (no vertex
+ * at the same corner index of two of the tile's hexes); the others run by hex colour. */
+int hexopt_layout_info(hexopt_ctx* ctx, int32_t* out6);
 
 #ifdef __cplusplus
 }

@@ -72,6 +72,21 @@ def test_many_tiles_tracks_fp64():
     assert E32 < E0 and E64 < E0 and abs(E32 - E64) < 2e-2 * (E0 - E64)
 
 
+def test_colour_scatter_tiles_track_fp64():
+    """Half the hexes numbered from rotated corners, so most tiles of the fused FP32 CG
+    operator scatter by hex colour instead of by corner index."""
+    from pyhexopt import meshgen as mg
+    n = 14
+    V, H = mg.grid_hex(n)
+    V = mg.jitter(V, ~mg.grid_boundary_mask(n), 0.15 * 2.0 / n, seed=8)
+    H = mg.rotate_hex_corners(H, 0.5, seed=8)
+    c = {"V": V, "H": H}
+    h64, h32 = make(c, kw={}), make(c, kw={})
+    info = h32.layout_info()
+    assert info["corner_tiles"] < info["tiles"] // 2
+    _compare_first_iteration(h64, h32, 1.0)
+
+
 def test_opt32_bitwise_deterministic():
     c = load("slim_grid6_features")
     wo = float(c["weight_opt"])

@@ -132,6 +132,34 @@ def test_slim_larger_grid_vs_oracle(seed):
     assert rel(log[:, 0], logo[:, 0]) < 1e-3
 
 
+def test_colour_scatter_tiles():
+    """The same jittered grid with half its hexes numbered from rotated corners
+    (meshgen.rotate_hex_corners: same hexes, same 8 corner tets). The consistent numbering
+    scatters every tile by corner index; the mixed one makes most tiles scatter by hex
+    colour. Both must assemble the oracle's system to 1e-10, and each other's to 1e-12."""
+    from pyhexopt import capi, meshgen as mg
+    n = 12
+    V, H = mg.grid_hex(n)
+    V = mg.jitter(V, ~mg.grid_boundary_mask(n), 0.15 * 2.0 / n, seed=5)
+    Hr = mg.rotate_hex_corners(H, 0.5, seed=5)
+    p = np.random.default_rng(5).normal(size=3 * V.shape[0])
+    out = []
+    for HH in (H, Hr):
+        h = capi.HexOpt(V, HH)
+        info = h.layout_info()
+        h.reference()
+        rhs, invd, Lp = h.debug_system(p)
+        _, rhs_o, Lp_o, dg_o = O.linear_system(V, HH, O.grad_ref(V, HH), V, dense=False, p=p)
+        assert rel(rhs, rhs_o) < 1e-10 and rel(1.0 / invd, dg_o) < 1e-10 and rel(Lp, Lp_o) < 1e-10
+        out.append((info, (rhs, invd, Lp)))
+        h.close()
+    (i0, s0), (i1, s1) = out
+    assert i0["tiles"] > 4 and i0["corner_tiles"] == i0["tiles"]
+    assert i1["corner_tiles"] < i1["tiles"] // 2
+    for a, b in zip(s0, s1):
+        assert rel(b, a) < 1e-12
+
+
 def test_scaled_jacobian_matches_reference():
     c = load("slim_grid5_inverted")
     from pyhexopt import capi
