@@ -608,20 +608,20 @@ static int ensure_state32(hexopt_ctx* c) {
 static int cg32(hexopt_ctx* c, const SoftDev& sd, double prox, const float* b, float* x, int iters, int* it_out,
                 double* err_out) {
   const int nV = c->L.nV, nL = c->nL;
-  const int n = 3 * nV + nL;
   if (c->L.nTiles == 0) return fail(c, HEXOPT_EINVAL, "mesh without hexes");
   hipStream_t st = c->stream;
   RefOp ro = refop(c);
   float* P[2] = {c->f_p.as<float>(), c->f_p2.as<float>()};
   float* z = c->f_z.as<float>();
   float* r = c->f_r.as<float>();
-  launch_cg32_init(n, b, c->f_invd.as<float>(), x, r, z, P[0], c->d_cg.p, c->d_part[0].as<double>(),
-                   c->d_part[1].as<double>(), 1e-8, st);
+  launch_cg32_init(nV, nL, sd, prox, b, c->f_invd.as<float>(), x, r, z, P[0], c->d_cg.p, c->d_part[0].as<double>(),
+                   c->d_part[1].as<double>(), c->d_part[3].as<double>(), 1e-8, st);
   for (int k = 0; k < iters; ++k)
     launch_cg32_iteration(c->tm, ro, c->f_K.as<float>(), prox, sd, c->d_sidx.as<int>(), c->d_iso.as<int>(), c->n_iso, z,
                           P[k & 1], P[(k + 1) & 1], c->f_tmp.as<float>(), c->f_halo.as<float>(), c->f_invd.as<float>(),
-                          x, r, z, c->d_cg.p, c->d_part[2].as<double>(), c->d_part[3].as<double>(),
-                          c->d_part[4].as<double>(), c->d_part[5].as<double>(), st);
+                          x, r, z, c->d_cg.p, c->d_part[2].as<double>(), c->d_part[4].as<double>(),
+                          c->d_part[5].as<double>(), c->d_part[3].as<double>(), c->d_part[6].as<double>(),
+                          c->d_part[7].as<double>(), st);
   HIPCHK(c, hipGetLastError());
   if (it_out || err_out) {
     std::vector<unsigned char> cgs(cg_state_bytes());
@@ -768,9 +768,8 @@ int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch
                      c->d_halo.as<double>(), c->d_part[2].as<double>(), nullptr, nullptr, st);
         break;
       case 1:
-        launch_cg32_apply(c->tm, ro, c->f_K.as<float>(), 1e-5, sd, c->d_sidx.as<int>(), c->f_z.as<float>(),
-                          c->f_p.as<float>(), c->f_p2.as<float>(), c->f_tmp.as<float>(), c->f_halo.as<float>(),
-                          c->d_cg.p, c->d_part[2].as<double>(), st);
+        launch_cg32_apply(c->tm, ro, c->f_K.as<float>(), c->f_z.as<float>(), c->f_p.as<float>(), c->f_p2.as<float>(),
+                          c->f_tmp.as<float>(), c->f_halo.as<float>(), c->d_cg.p, c->d_part[2].as<double>(), st);
         break;
       case 2:
         launch_local_step(tm, ro, c->d_cur.as<double>(), c->d_K.as<double>(), HEXOPT_SYMMETRIC_DIRICHLET, 1.0, 1.0,
@@ -786,9 +785,9 @@ int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch
   if (which == 1) {
     // a fresh CG state (not converged) on the current FP32 system, so the pass runs in full
     HIPCHK(c, hipMemsetAsync(c->d_cg.p, 0, cg_state_bytes(), st));
-    launch_cg32_init(3 * c->L.nV + c->nL, c->f_rhs.as<float>(), c->f_invd.as<float>(), c->f_d.as<float>(),
+    launch_cg32_init(c->L.nV, c->nL, sd, 1e-5, c->f_rhs.as<float>(), c->f_invd.as<float>(), c->f_d.as<float>(),
                      c->f_r.as<float>(), c->f_z.as<float>(), c->f_p.as<float>(), c->d_cg.p, c->d_part[0].as<double>(),
-                     c->d_part[1].as<double>(), 1e-8, st);
+                     c->d_part[1].as<double>(), c->d_part[3].as<double>(), 1e-8, st);
   }
   one();  // warm
   HIPCHK(c, hipEventRecord(e0, st));

@@ -96,16 +96,18 @@ void launch_cg_zero_if32(int n, void* state, float* x, int nb, hipStream_t st);
 // launch_cg32_iteration is one CG iteration reading z and p_prev, writing p_new and the
 // updated x, r, z (zout may alias z). sidx[v] is the fixup index of vertex v (-1 when its
 // tile writes it); iso lists the vertices without hexes.
-void launch_cg32_init(int n, const float* b, const float* invd, float* x, float* r, float* z, float* pprev, void* state,
-                      double* part_b, double* part_rz, double tol, hipStream_t st);
-// The operator pass of one iteration alone (feature lines ignored), for timing.
-void launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
-                       const int* sidx, const float* z, const float* pprev, float* pnew, float* out, float* halo,
-                       const void* state, double* part_tile, hipStream_t st);
+void launch_cg32_init(int nV, int nL, const SoftDev& sd, double prox, const float* b, const float* invd, float* x,
+                      float* r, float* z, float* pprev, void* state, double* part_b, double* part_rz, double* part_q,
+                      double tol, hipStream_t st);
+// The operator pass of one iteration alone (the element part E p and the p.E p partials),
+// for timing. halo holds 4 floats per slot.
+void launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const float* z, const float* pprev,
+                       float* pnew, float* out, float* halo, const void* state, double* part_tile, hipStream_t st);
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
                            const int* sidx, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
-                           double* part_tile, double* part_line, double* part_rr, double* part_rz, hipStream_t st);
+                           double* part_tile, double* part_rr, double* part_rz, double* part_qz, double* part_bzq,
+                           double* part_qq, hipStream_t st);
 void launch_residual64(int n, const float* b, const double* Ld, float* r32, double* part, int nb, double* out,
                        hipStream_t st);
 void launch_f2d(int n, const float* a, double* b, hipStream_t st);

@@ -47,6 +47,30 @@ __device__ __forceinline__ T block_sum(T v, T* red) {
   return s;
 }
 
+// N sums at once, each as block_sum computes it, behind one pair of barriers; red holds
+// N * 16 values
+template <int N>
+__device__ __forceinline__ void block_sum_n(double v[N], double* red) {
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) red[k * 16 + w] = v[k];
+  }
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double t = 0;
+    for (int i = 0; i < nw; ++i) t += red[k * 16 + i];
+    v[k] = t;
+  }
+}
+
 __device__ __forceinline__ double block_min(double v, double* red) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
@@ -550,6 +574,22 @@ struct LocalStepOp {
 
 // CG operator L p = A^T M A p + prox p + soft terms; finish() also returns p.(Lp).
 // KT: stored K precision, PT: vector precision, CT: arithmetic, OT: output precision.
+// Cube reference: invs^2 is folded into K, so the forces on the edges of corner tet c are
+// K E (columns = edges), added at the edge ends and taken off the tet's corner.
+template <class CT>
+__device__ __forceinline__ void tet_forces_cube(const CT E[3][3], const CT kk[6], int c, CT o[8][3]) {
+  const CT k00 = kk[0], k11 = kk[1], k22 = kk[2], k01 = kk[3], k02 = kk[4], k12 = kk[5];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const CT f0 = k00 * E[0][k] + k01 * E[1][k] + k02 * E[2][k];
+    const CT f1 = k01 * E[0][k] + k11 * E[1][k] + k12 * E[2][k];
+    const CT f2 = k02 * E[0][k] + k12 * E[1][k] + k22 * E[2][k];
+    const int vk = kHexTet[c][k + 1], v0 = kHexTet[c][0];
+    o[vk][0] += f0; o[vk][1] += f1; o[vk][2] += f2;
+    o[v0][0] -= f0; o[v0][1] -= f1; o[v0][2] -= f2;
+  }
+}
+
 // Element part of the CG operator for hex h: corner forces o (added) from the corner
 // values q of the direction, with the per-tet K = invs^2 M W W^T (6 symmetric components).
 template <class KT, class CT>
@@ -563,16 +603,7 @@ __device__ __forceinline__ void hex_forces(const RefOp& ro, const KT* K, int nH,
     const CT* kk = kp + 6 * (c & 1);
     const CT k00 = kk[0], k11 = kk[1], k22 = kk[2], k01 = kk[3], k02 = kk[4], k12 = kk[5];
     if (ro.mode == REF_CUBE) {
-      // invs^2 is folded into K: forces on the tet edges are K E (columns = edges)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const CT f0 = k00 * E[0][k] + k01 * E[1][k] + k02 * E[2][k];
-        const CT f1 = k01 * E[0][k] + k11 * E[1][k] + k12 * E[2][k];
-        const CT f2 = k02 * E[0][k] + k12 * E[1][k] + k22 * E[2][k];
-        const int vk = kHexTet[c][k + 1], v0 = kHexTet[c][0];
-        o[vk][0] += f0; o[vk][1] += f1; o[vk][2] += f2;
-        o[v0][0] -= f0; o[v0][1] -= f1; o[v0][2] -= f2;
-      }
+      tet_forces_cube(E, kk, c, o);
     } else {
       tet_B(ro, nH, h, c, B);
       CT J[3][3];
@@ -771,6 +802,7 @@ __global__ void line_apply_kernel(SoftDev sd, int nV, const PT* p, const PT* pa,
 struct CGState {
   double rhsNorm2, threshold, absNew, alpha, beta, rn2, tol_error;
   int done, iters, zero_rhs, pad;
+  double qz, bzq, qq;  // FP32 CG: z.R z, z.R q, q.R q of the next direction z + beta q
 };
 
 // r = rhs - tmp, partial |rhs|^2, |r|^2
@@ -793,6 +825,18 @@ __device__ double sum_parts(const double* p, int n, double* red) {
   double s = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
   return block_sum(s, red);
+}
+
+// sum_parts of N arrays of n partials in one pass; red holds N * 16 values
+template <int N>
+__device__ void sum_parts_n(const double* const (&p)[N], int n, double* red, double out[N]) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) out[k] = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] += p[k][i];
+  }
+  block_sum_n<N>(out, red);
 }
 
 // Eigen ConjugateGradient.h:44-62 (zero rhs, early exit) and the first direction.
@@ -899,21 +943,27 @@ __global__ void cg_zero_if_kernel(int n, const CGState* st, T* x) {
 
 // ---------------------------------------------------------------------------------------
 // FP32 CG with the vector work fused into the operator pass (the FP32 optimiser's global
-// step, Eigen's algorithm with FP64 dot products). Per CG iteration:
+// step, Eigen's algorithm with FP64 dot products). L = E + R: E the element operator, R the
+// vertex-local rows (prox, the folded constraint block S, and each line unknown's coupling
+// to its vertex). Per CG iteration:
 //   cg32_apply   per tile: p = z + beta p_prev staged in LDS (p double-buffered, each vertex
 //                written once: by its own tile or by the first tile of its halo slots),
-//                L p of the tile's elements, and a per-tile p.Lp partial from the element
-//                quadratic forms p_e.(K_e p_e) plus the vertex rows
-//   cg32_lines   a_L rows of L p and their p.Lp partials (feature lines only)
-//   cg32_alpha   one block: vertices without hexes, then alpha = r.z / p.Lp
-//   cg32_update  L p of halo vertices from their slots, x += alpha p, r -= alpha L p,
-//                z = r / diag, FP64 partials of |r|^2 and r.z
-//   cg_beta      one block: convergence test and beta
+//                E p of the tile's elements, and a per-tile partial of p.E p from the element
+//                quadratic forms p_e.(K_e p_e)
+//   cg32_lines   p and the a_L rows of L p for the line unknowns (feature lines only)
+//   cg32_alpha   one block: p of the vertices without hexes, then
+//                alpha = r.z / (p.E p + p.R p). p.R p needs no pass of its own: with
+//                p = z + beta q (q the previous direction) it is
+//                z.R z + 2 beta z.R q + beta^2 q.R q, three sums the previous update left
+//   cg32_update  per vertex: E p (its tile's value or the sum of its halo slots) + R p,
+//                x += alpha p, r -= alpha L p, z = r / diag for the vertex and its line
+//                unknown; FP64 partials of |r|^2, r.z and of z.R z, z.R p, p.R p
+//   cg32_beta    one block: convergence test, beta, and the three R sums
 // Every reduction is two-level in a fixed order; kernel boundaries order the levels (an
 // in-kernel last-block reduction would need a device-scope fence per block, which on this
 // multi-XCD part writes back L2 each time).
 
-// rows of L p beyond the elements at vertex g: prox p + S p - lamL t a
+// rows of R p at vertex g: prox p + S p - lamL t a
 __device__ __forceinline__ void vertex_rows(const SoftDev& sd, double prox, int ci, int li, const double pv[3], double pa,
                                             double r[3]) {
 #pragma unroll
@@ -928,6 +978,27 @@ __device__ __forceinline__ void vertex_rows(const SoftDev& sd, double prox, int 
       for (int d = 0; d < 3; ++d) r[d] -= sd.lamL * sd.lt[(size_t)li * 3 + d] * pa;
     }
   }
+}
+
+// R as a symmetric bilinear form over vertex g and its line unknown a (the a_L row
+// lamL (|t|^2 p_a - t.p_v) included): x.R y = prox x.y + x^T S y
+// + lamL (|t|^2 x_a y_a - x_a t.y - y_a t.x)
+__device__ __forceinline__ double vertex_form(const SoftDev& sd, double prox, int ci, int li, const double x[3],
+                                              double xa, const double y[3], double ya) {
+  double f = prox * (x[0] * y[0] + x[1] * y[1] + x[2] * y[2]);
+  if (ci >= 0) {
+    const double* S = sd.S + (size_t)ci * 6;
+    f += x[0] * (S[0] * y[0] + S[3] * y[1] + S[4] * y[2]) + x[1] * (S[3] * y[0] + S[1] * y[1] + S[5] * y[2]) +
+         x[2] * (S[4] * y[0] + S[5] * y[1] + S[2] * y[2]);
+    if (li >= 0) {
+      const double* t = sd.lt + (size_t)li * 3;
+      const double tt = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+      const double tx = t[0] * x[0] + t[1] * x[1] + t[2] * x[2];
+      const double ty = t[0] * y[0] + t[1] * y[1] + t[2] * y[2];
+      f += sd.lamL * (tt * xa * ya - xa * ty - ya * tx);
+    }
+  }
+  return f;
 }
 
 // x = 0, r = b, z = b / diag, p_prev = 0; partials of |b|^2 and r.z
@@ -952,11 +1023,31 @@ __global__ __launch_bounds__(256) void cg32_init_kernel(int n, const float* __re
   if (threadIdx.x == 0) { part_b[blockIdx.x] = a; part_rz[blockIdx.x] = c; }
 }
 
+// partials of z.R z for the first direction (p_prev = 0, so z.R q = q.R q = 0)
+__global__ __launch_bounds__(256) void cg32_form0_kernel(SoftDev sd, double prox, int nV, const float* __restrict__ z,
+                                                         double* part) {
+  __shared__ double red[16];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  double f = 0.0;
+  if (g < nV) {
+    const size_t nA = 3 * (size_t)nV;
+    const double zv[3] = {(double)z[g], (double)z[(size_t)nV + g], (double)z[2 * (size_t)nV + g]};
+    const int ci = sd.cidx ? sd.cidx[g] : -1;
+    const int ln = ci >= 0 ? sd.cline[ci] : -1;
+    const double za = ln >= 0 ? (double)z[nA + ln] : 0.0;
+    f = vertex_form(sd, prox, ci, ln, zv, za, zv, za);
+  }
+  const double a = block_sum(f, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
 // ConjugateGradient.h:44-62 with x0 = 0 (so r0 = b and |r0|^2 = |b|^2)
-__global__ void cg32_init_fin_kernel(CGState* st, const double* part_b, const double* part_rz, int nb, double tol) {
+__global__ void cg32_init_fin_kernel(CGState* st, const double* part_b, const double* part_rz, int nb,
+                                     const double* part_q, int nq, double tol) {
   __shared__ double red[16];
   const double b2 = sum_parts(part_b, nb, red);
   const double rz = sum_parts(part_rz, nb, red);
+  const double qz = sum_parts(part_q, nq, red);
   if (threadIdx.x == 0) {
     st->rhsNorm2 = b2;
     st->iters = 0;
@@ -964,6 +1055,9 @@ __global__ void cg32_init_fin_kernel(CGState* st, const double* part_b, const do
     st->zero_rhs = 0;
     st->beta = 0;
     st->tol_error = 0;
+    st->qz = qz;
+    st->bzq = 0;
+    st->qq = 0;
     if (b2 == 0) { st->done = 1; st->zero_rhs = 1; return; }
     st->threshold = tol * tol * b2;
     st->rn2 = b2;
@@ -972,11 +1066,11 @@ __global__ void cg32_init_fin_kernel(CGState* st, const double* part_b, const do
   }
 }
 
+template <bool CUBE>
 __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp ro, const float* __restrict__ K,
                                                               const float* __restrict__ z,
                                                               const float* __restrict__ pprev, float* __restrict__ pnew,
-                                                              float* __restrict__ out, float* __restrict__ halo,
-                                                              double prox, SoftDev sd, const int* __restrict__ sidx,
+                                                              float* __restrict__ out, float4* __restrict__ halo,
                                                               const CGState* st, double* partial) {
   __shared__ float sp[3 * kMaxLV];
   __shared__ float acc[3 * kMaxLV];
@@ -985,13 +1079,25 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int nV = tm.nV, nH = tm.nH;
-  const size_t nA = 3 * (size_t)nV;
   const int h0 = tm.tile_hex[tile], nh = tm.tile_hex[tile + 1] - h0;
   const int lv0 = tm.tile_lv[tile], nlv = tm.tile_lv[tile + 1] - lv0;
   const float beta = (float)st->beta;
+  // loads that need nothing from the staging go out first: the hex's corner ids and, for
+  // the cube reference, its 48 operator values
+  const bool active = tid < nh;
+  int li[8];
+  float kr[CUBE ? 48 : 1];
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) li[k] = tm.lidx[(size_t)k * nH + h0 + tid];
+    if constexpr (CUBE) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) k_pair(K, nH, h0 + tid, q, kr + 12 * q);
+    }
+  }
   // stage p of the tile's vertices
   for (int lv = tid; lv < nlv; lv += kTileHex) {
-    const int g = tm.lv_gid[lv0 + lv], s = tm.lv_slot[lv0 + lv];
+    const int g = tm.lv_gid[lv0 + lv];
     float pv[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -1006,23 +1112,28 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
   }
   __syncthreads();
   float c[8][3];
-  int li[8];
   double part = 0.0;
 #pragma unroll
   for (int k = 0; k < 8; ++k)
 #pragma unroll
     for (int d = 0; d < 3; ++d) c[k][d] = 0.0f;
-  const bool active = tid < nh;
   if (active) {
     const int h = h0 + tid;
     float q[8][3];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      li[k] = tm.lidx[(size_t)k * nH + h];
+    for (int k = 0; k < 8; ++k)
 #pragma unroll
       for (int d = 0; d < 3; ++d) q[k][d] = sp[d * kMaxLV + li[k]];
+    if constexpr (CUBE) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        float E[3][3];
+        tet_edges(q, t, E);
+        tet_forces_cube(E, kr + 6 * t, t, c);
+      }
+    } else {
+      hex_forces<float, float>(ro, K, nH, h, q, c);
     }
-    hex_forces<float, float>(ro, K, nH, h, q, c);
     float e = 0.0f;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
@@ -1031,160 +1142,162 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
     part = (double)e;
   }
   scatter_tile<float, 3>(tm, tile, nh, active, li, c, acc);
+  // E p: straight to the vertex when the tile is its only one, else to the tile's halo slot
   for (int lv = tid; lv < nlv; lv += kTileHex) {
     const int g = tm.lv_gid[lv0 + lv], s = tm.lv_slot[lv0 + lv];
-    const bool owned = s < 0;
-    const bool counted = tm.lv_first[lv0 + lv] != 0;
-    if (!owned) {
+    if (s < 0) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) halo[(size_t)d * tm.nSlots + s] = acc[d * kMaxLV + lv];
-    }
-    if (!counted) continue;
-    double pv[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) pv[d] = (double)sp[d * kMaxLV + lv];
-    const int ci = sd.cidx ? sd.cidx[g] : -1;
-    const int ln = ci >= 0 ? sd.cline[ci] : -1;
-    const double pa = ln >= 0 ? (double)(z[nA + ln] + beta * pprev[nA + ln]) : 0.0;
-    double vr[3];
-    vertex_rows(sd, prox, ci, ln, pv, pa, vr);
-    part += pv[0] * vr[0] + pv[1] * vr[1] + pv[2] * vr[2];
-    if (owned) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) out[(size_t)d * nV + g] = (float)((double)acc[d * kMaxLV + lv] + vr[d]);
+      for (int d = 0; d < 3; ++d) out[(size_t)d * nV + g] = acc[d * kMaxLV + lv];
+    } else {
+      halo[s] = make_float4(acc[lv], acc[kMaxLV + lv], acc[2 * kMaxLV + lv], 0.0f);
     }
   }
   const double bs = block_sum(part, red);
   if (tid == 0) partial[tile] = bs;
 }
 
-// Vertices without hexes (their p and vertex rows; no tile touches them), then
-// alpha = absNew / p.Lp over the tile, line and isolated-vertex partials.
-__global__ __launch_bounds__(1024) void cg32_alpha_kernel(CGState* st, const double* part_tile, int ntiles,
-                                                          const double* part_line, int nline, SoftDev sd, double prox,
-                                                          int nV, const int* __restrict__ iso, int niso,
+// p of the vertices without hexes (no tile stages them), then
+// alpha = absNew / (p.E p + z.R z + 2 beta z.R q + beta^2 q.R q)
+__global__ __launch_bounds__(1024) void cg32_alpha_kernel(CGState* st, const double* part_tile, int ntiles, int nV,
+                                                          const int* __restrict__ iso, int niso,
                                                           const float* __restrict__ z,
                                                           const float* __restrict__ pprev, float* __restrict__ pnew) {
   __shared__ double red[16];
   if (st->done) return;
-  const size_t nA = 3 * (size_t)nV;
   const float beta = (float)st->beta;
-  double pi = 0.0;
   for (int k = threadIdx.x; k < niso; k += blockDim.x) {
     const int g = iso[k];
-    double pv[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const float v = z[(size_t)d * nV + g] + beta * pprev[(size_t)d * nV + g];
-      pnew[(size_t)d * nV + g] = v;
-      pv[d] = (double)v;
-    }
-    const int ci = sd.cidx ? sd.cidx[g] : -1;
-    const int ln = ci >= 0 ? sd.cline[ci] : -1;
-    const double pa = ln >= 0 ? (double)(z[nA + ln] + beta * pprev[nA + ln]) : 0.0;
-    double vr[3];
-    vertex_rows(sd, prox, ci, ln, pv, pa, vr);
-    pi += pv[0] * vr[0] + pv[1] * vr[1] + pv[2] * vr[2];
+    for (int d = 0; d < 3; ++d) pnew[(size_t)d * nV + g] = z[(size_t)d * nV + g] + beta * pprev[(size_t)d * nV + g];
   }
-  const double a = sum_parts(part_tile, ntiles, red);
-  const double b = nline > 0 ? sum_parts(part_line, nline, red) : 0.0;
-  const double c = block_sum(pi, red);
-  if (threadIdx.x == 0) st->alpha = st->absNew / (a + b + c);
+  const double e = sum_parts(part_tile, ntiles, red);
+  if (threadIdx.x == 0) {
+    const double b = (double)beta;
+    st->alpha = st->absNew / (e + st->qz + 2.0 * b * st->bzq + b * b * st->qq);
+  }
 }
 
-// a_L rows: p_a = z_a + beta p_prev_a, (L p)_a = lamL (|t|^2 p_a - t . p_v); sets alpha
+// line unknowns: p_a = z_a + beta p_prev_a and the a_L rows (L p)_a = lamL (|t|^2 p_a - t.p_v)
 __global__ __launch_bounds__(256) void cg32_lines_kernel(SoftDev sd, int nV, const float* __restrict__ z,
                                                          const float* __restrict__ pprev, float* __restrict__ pnew,
-                                                         float* __restrict__ out, const CGState* st, double* partial) {
-  __shared__ double red[16];
+                                                         float* __restrict__ out, const CGState* st) {
   if (st->done) return;
   const size_t nA = 3 * (size_t)nV;
   const float beta = (float)st->beta;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double part = 0.0;
-  if (i < sd.nL) {
-    const int v = sd.lv[i];
-    const float pa = z[nA + i] + beta * pprev[nA + i];
-    pnew[nA + i] = pa;
-    const double* t = sd.lt + (size_t)i * 3;
-    double tp = 0, tt = 0;
+  if (i >= sd.nL) return;
+  const int v = sd.lv[i];
+  const float pa = z[nA + i] + beta * pprev[nA + i];
+  pnew[nA + i] = pa;
+  const double* t = sd.lt + (size_t)i * 3;
+  double tp = 0, tt = 0;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double pv = (double)(z[(size_t)d * nV + v] + beta * pprev[(size_t)d * nV + v]);
-      tp += t[d] * pv;
-      tt += t[d] * t[d];
-    }
-    const double r = sd.lamL * (tt * (double)pa - tp);
-    out[nA + i] = (float)r;
-    part = (double)pa * r;
+  for (int d = 0; d < 3; ++d) {
+    const double pv = (double)(z[(size_t)d * nV + v] + beta * pprev[(size_t)d * nV + v]);
+    tp += t[d] * pv;
+    tt += t[d] * t[d];
   }
-  const double bs = block_sum(part, red);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  out[nA + i] = (float)(sd.lamL * (tt * (double)pa - tp));
 }
 
-// vertices (3 unknowns each) then line unknowns
+// one thread per vertex, which also updates its line unknown (each line unknown belongs to
+// exactly one vertex)
 __global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev sd, double prox,
-                                                          const int* __restrict__ sidx, const float* __restrict__ halo,
+                                                          const int* __restrict__ sidx, const float4* __restrict__ halo,
                                                           const float* __restrict__ p, const float* __restrict__ out,
                                                           const float* __restrict__ invd, float* __restrict__ x,
                                                           float* __restrict__ r, float* __restrict__ z,
-                                                          const CGState* st, double* part_rr, double* part_rz) {
-  __shared__ double red[16];
+                                                          const CGState* st, double* part_rr, double* part_rz,
+                                                          double* part_qz, double* part_bzq, double* part_qq) {
+  __shared__ double red[5 * 16];
   if (st->done) return;
   const int nV = tm.nV;
   const size_t nA = 3 * (size_t)nV;
   const float alpha = (float)st->alpha;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double srr = 0, srz = 0;
-  if (i < nV) {
-    const int g = i;
-    float lp[3];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  double srr = 0, srz = 0, fz = 0, fzq = 0, fq = 0;
+  if (g < nV) {
+    double a[3];
     const int si = sidx[g];
     if (si < 0) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) lp[d] = out[(size_t)d * nV + g];
+      for (int d = 0; d < 3; ++d) a[d] = (double)out[(size_t)d * nV + g];
     } else {
-      double a[3] = {0, 0, 0};
+      a[0] = a[1] = a[2] = 0.0;
       for (int k = tm.sv_off[si]; k < tm.sv_off[si + 1]; ++k) {
-        const int s = tm.sv_slots[k];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) a[d] += (double)halo[(size_t)d * tm.nSlots + s];
+        const float4 hv = halo[tm.sv_slots[k]];
+        a[0] += (double)hv.x; a[1] += (double)hv.y; a[2] += (double)hv.z;
       }
-      double pv[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) pv[d] = (double)p[(size_t)d * nV + g];
-      const int ci = sd.cidx ? sd.cidx[g] : -1;
-      const int ln = ci >= 0 ? sd.cline[ci] : -1;
-      const double pa = ln >= 0 ? (double)p[nA + ln] : 0.0;
-      double vr[3];
-      vertex_rows(sd, prox, ci, ln, pv, pa, vr);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) lp[d] = (float)(a[d] + vr[d]);
     }
+    double pv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pv[d] = (double)p[(size_t)d * nV + g];
+    const int ci = sd.cidx ? sd.cidx[g] : -1;
+    const int ln = ci >= 0 ? sd.cline[ci] : -1;
+    const double pa = ln >= 0 ? (double)p[nA + ln] : 0.0;
+    double vr[3];
+    vertex_rows(sd, prox, ci, ln, pv, pa, vr);
+    double zn[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const size_t j = (size_t)d * nV + g;
       x[j] += alpha * p[j];
-      const float rv = r[j] - alpha * lp[d];
+      const float rv = r[j] - alpha * (float)(a[d] + vr[d]);
       r[j] = rv;
       const float zv = invd[j] * rv;
       z[j] = zv;
+      zn[d] = (double)zv;
       srr += (double)rv * (double)rv;
       srz += (double)rv * (double)zv;
     }
-  } else if (i < nV + sd.nL) {
-    const size_t j = nA + (size_t)(i - nV);
-    x[j] += alpha * p[j];
-    const float rv = r[j] - alpha * out[j];
-    r[j] = rv;
-    const float zv = invd[j] * rv;
-    z[j] = zv;
-    srr += (double)rv * (double)rv;
-    srz += (double)rv * (double)zv;
+    double za = 0.0;
+    if (ln >= 0) {
+      const size_t j = nA + ln;
+      x[j] += alpha * p[j];
+      const float rv = r[j] - alpha * out[j];
+      r[j] = rv;
+      const float zv = invd[j] * rv;
+      z[j] = zv;
+      za = (double)zv;
+      srr += (double)rv * (double)rv;
+      srz += (double)rv * (double)zv;
+    }
+    // the next direction is z + beta p: its R sums
+    fz = vertex_form(sd, prox, ci, ln, zn, za, zn, za);
+    fzq = vertex_form(sd, prox, ci, ln, zn, za, pv, pa);
+    fq = vertex_form(sd, prox, ci, ln, pv, pa, pv, pa);
   }
-  const double a = block_sum(srr, red);
-  const double b = block_sum(srz, red);
-  if (threadIdx.x == 0) { part_rr[blockIdx.x] = a; part_rz[blockIdx.x] = b; }
+  double v[5] = {srr, srz, fz, fzq, fq};
+  block_sum_n<5>(v, red);
+  if (threadIdx.x == 0) {
+    part_rr[blockIdx.x] = v[0]; part_rz[blockIdx.x] = v[1];
+    part_qz[blockIdx.x] = v[2]; part_bzq[blockIdx.x] = v[3]; part_qq[blockIdx.x] = v[4];
+  }
+}
+
+// cg_beta_kernel plus the three R sums of the next direction
+__global__ void cg32_beta_kernel(CGState* st, const double* part_rr, const double* part_rz, const double* part_qz,
+                                 const double* part_bzq, const double* part_qq, int nb) {
+  __shared__ double red[5 * 16];
+  const double* const parts[5] = {part_rr, part_rz, part_qz, part_bzq, part_qq};
+  double v[5];
+  sum_parts_n<5>(parts, nb, red, v);
+  const double rr = v[0], rz = v[1], qz = v[2], bzq = v[3], qq = v[4];
+  if (threadIdx.x == 0 && !st->done) {
+    st->rn2 = rr;
+    if (rr < st->threshold) {
+      st->done = 1;
+    } else {
+      const double absOld = st->absNew;
+      st->absNew = rz;
+      st->beta = rz / absOld;
+      st->iters += 1;
+      st->qz = qz;
+      st->bzq = bzq;
+      st->qq = qq;
+    }
+    st->tol_error = sqrt(rr / st->rhsNorm2);
+  }
 }
 
 // residual of the direction system in FP64: r64 = b - L d (L d given in FP64), partial |r64|^2
@@ -1715,39 +1828,48 @@ void launch_f2d(int n, const float* a, double* b, hipStream_t st) {
   hipLaunchKernelGGL(f2d_kernel, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, st, n, a, b);
 }
 
-void launch_cg32_init(int n, const float* b, const float* invd, float* x, float* r, float* z, float* pprev, void* state,
-                      double* part_b, double* part_rz, double tol, hipStream_t st) {
+void launch_cg32_init(int nV, int nL, const SoftDev& sd, double prox, const float* b, const float* invd, float* x,
+                      float* r, float* z, float* pprev, void* state, double* part_b, double* part_rz, double* part_q,
+                      double tol, hipStream_t st) {
+  const int n = 3 * nV + nL;
   const int nb = std::max(1, std::min((n + 255) / 256, 2048));
+  const int nq = std::max(1, (nV + 255) / 256);
   hipLaunchKernelGGL(cg32_init_kernel, dim3(nb), dim3(256), 0, st, n, b, invd, x, r, z, pprev, part_b, part_rz);
+  hipLaunchKernelGGL(cg32_form0_kernel, dim3(nq), dim3(256), 0, st, sd, prox, nV, (const float*)z, part_q);
   hipLaunchKernelGGL(cg32_init_fin_kernel, dim3(1), dim3(1024), 0, st, (CGState*)state, (const double*)part_b,
-                     (const double*)part_rz, nb, tol);
+                     (const double*)part_rz, nb, (const double*)part_q, nq, tol);
 }
 
-void launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
-                       const int* sidx, const float* z, const float* pprev, float* pnew, float* out, float* halo,
-                       const void* state, double* part_tile, hipStream_t st) {
-  hipLaunchKernelGGL(cg32_apply_kernel, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out, halo,
-                     prox, sd, sidx, (const CGState*)state, part_tile);
+void launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const float* z, const float* pprev,
+                       float* pnew, float* out, float* halo, const void* state, double* part_tile, hipStream_t st) {
+  float4* h4 = reinterpret_cast<float4*>(halo);
+  if (ro.mode == REF_CUBE)
+    hipLaunchKernelGGL(cg32_apply_kernel<true>, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out,
+                       h4, (const CGState*)state, part_tile);
+  else
+    hipLaunchKernelGGL(cg32_apply_kernel<false>, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out,
+                       h4, (const CGState*)state, part_tile);
 }
 
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
                            const int* sidx, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
-                           double* part_tile, double* part_line, double* part_rr, double* part_rz, hipStream_t st) {
+                           double* part_tile, double* part_rr, double* part_rz, double* part_qz, double* part_bzq,
+                           double* part_qq, hipStream_t st) {
   CGState* cs = (CGState*)state;
   const int nL = sd.nL;
-  const int nline = (nL + 255) / 256;
-  launch_cg32_apply(tm, ro, K, prox, sd, sidx, z, pprev, pnew, out, halo, cs, part_tile, st);
+  launch_cg32_apply(tm, ro, K, z, pprev, pnew, out, halo, cs, part_tile, st);
   if (nL > 0)
-    hipLaunchKernelGGL(cg32_lines_kernel, dim3(nline), dim3(256), 0, st, sd, tm.nV, z, pprev, pnew, out,
-                       (const CGState*)cs, part_line);
-  hipLaunchKernelGGL(cg32_alpha_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_tile, tm.nTiles,
-                     (const double*)part_line, nL > 0 ? nline : 0, sd, prox, tm.nV, iso, niso, z, pprev, pnew);
-  const int nbu = (tm.nV + nL + 255) / 256;
-  hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, sidx, (const float*)halo,
-                     (const float*)pnew, (const float*)out, invd, x, r, zout, (const CGState*)cs, part_rr, part_rz);
-  hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_rr, (const double*)part_rz,
-                     nbu);
+    hipLaunchKernelGGL(cg32_lines_kernel, dim3((nL + 255) / 256), dim3(256), 0, st, sd, tm.nV, z, pprev, pnew, out,
+                       (const CGState*)cs);
+  hipLaunchKernelGGL(cg32_alpha_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_tile, tm.nTiles, tm.nV, iso,
+                     niso, z, pprev, pnew);
+  const int nbu = (tm.nV + 255) / 256;
+  hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, sidx,
+                     (const float4*)halo, (const float*)pnew, (const float*)out, invd, x, r, zout,
+                     (const CGState*)cs, part_rr, part_rz, part_qz, part_bzq, part_qq);
+  hipLaunchKernelGGL(cg32_beta_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_rr, (const double*)part_rz,
+                     (const double*)part_qz, (const double*)part_bzq, (const double*)part_qq, nbu);
 }
 
 void launch_max_step(const double* cur, const double* dst, int nV, const int* conn, int nH, double* part,

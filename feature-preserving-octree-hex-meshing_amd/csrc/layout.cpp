@@ -129,6 +129,31 @@ bool build_layout(int nV, const double* V, int nH, const int* H, MeshLayout& L, 
       off[c] = (uint16_t)n;
     }
   }
+  // vertices renumbered tile by tile: each takes the next id in the first tile that holds it
+  // (in Morton order within the tile), and vertices in no hex come last (Morton order). A
+  // tile's vertices are then one contiguous run of ids plus the halo vertices of earlier
+  // tiles, so its vertex loads and stores coalesce.
+  {
+    std::vector<int> nid(nV, -1);
+    int next = 0;
+    std::vector<int> fresh;
+    for (int t = 0; t < L.nTiles; ++t) {
+      fresh.clear();
+      for (int i = L.tile_hex[t]; i < L.tile_hex[t + 1]; ++i)
+        for (int k = 0; k < 8; ++k) {
+          const int v = H[(size_t)L.hperm[i] * 8 + k];
+          if (nid[v] == -1) { nid[v] = -2; fresh.push_back(v); }
+        }
+      std::sort(fresh.begin(), fresh.end(), [&](int a, int c) { return L.vinv[a] < L.vinv[c]; });
+      for (int v : fresh) nid[v] = next++;
+    }
+    for (int i = 0; i < nV; ++i)
+      if (nid[L.vperm[i]] == -1) nid[L.vperm[i]] = next++;
+    for (int v = 0; v < nV; ++v) {
+      L.vinv[v] = nid[v];
+      L.vperm[nid[v]] = v;
+    }
+  }
   // connectivity in new numbering, corner-major
   L.conn.resize((size_t)8 * nH);
   for (int i = 0; i < nH; ++i)
@@ -151,14 +176,18 @@ bool build_layout(int nV, const double* V, int nH, const int* H, MeshLayout& L, 
   int slot_next = 0;
   std::vector<int> local(nV, -1);
   for (int t = 0; t < L.nTiles; ++t) {
+    // the tile's vertices in ascending id order
     std::vector<int> lv;
     for (int i = L.tile_hex[t]; i < L.tile_hex[t + 1]; ++i)
       for (int k = 0; k < 8; ++k) {
         int v = L.conn[(size_t)k * nH + i];
-        if (local[v] < 0) { local[v] = (int)lv.size(); lv.push_back(v); }
-        L.lidx[(size_t)k * nH + i] = (uint16_t)local[v];
+        if (local[v] < 0) { local[v] = 0; lv.push_back(v); }
       }
     if ((int)lv.size() > kMaxLV) { *err = "tile local vertex overflow"; return false; }
+    std::sort(lv.begin(), lv.end());
+    for (size_t j = 0; j < lv.size(); ++j) local[lv[j]] = (int)j;
+    for (int i = L.tile_hex[t]; i < L.tile_hex[t + 1]; ++i)
+      for (int k = 0; k < 8; ++k) L.lidx[(size_t)k * nH + i] = (uint16_t)local[L.conn[(size_t)k * nH + i]];
     {
       std::vector<uint8_t> seen(lv.size(), 0);
       uint8_t ok = 1;
