@@ -56,7 +56,7 @@ struct hexopt_ctx {
   // layout on the device
   DevBuf d_conn, d_tile_hex, d_tile_col, d_tile_lv, d_lv_gid, d_lv_slot, d_lv_first, d_lidx, d_tile_corner, d_sv_gid,
       d_sv_off, d_sv_slots;
-  DevBuf d_sidx, d_iso;  // vertex -> fixup index (-1: written by its tile); vertices without hexes
+  DevBuf d_vslot, d_iso;  // per vertex its run of halo slots ((-1, -1): its tile writes it); vertices without hexes
   DevBuf d_torder;       // plane queries sorted by vertex (Morton) order
   DevBuf d_cgsave;       // CG state of the last direction solve, read back with the line search
   int n_iso = 0;
@@ -69,7 +69,7 @@ struct hexopt_ctx {
   // SLIM state
   DevBuf d_K, d_rhs, d_invd, d_x, d_r, d_z, d_p, d_tmp, d_halo, d_cg, d_part[8], d_scal;
   // constraints
-  DevBuf d_cidx, d_S, d_b, d_cline, d_bids, d_bpos, d_cids, d_cpos, d_tids, d_tn, d_td, d_lv, d_lt, d_lo, d_rhs_a,
+  DevBuf d_cidx, d_cl, d_S, d_b, d_cline, d_bids, d_bpos, d_cids, d_cpos, d_tids, d_tn, d_td, d_lv, d_lt, d_lo, d_rhs_a,
       d_invd_a, d_aL;
   SoftDev sd{};
   int nL = 0;
@@ -196,13 +196,16 @@ hexopt_ctx* hexopt_create(int nV, const double* V, int nH, const int32_t* hexes,
   rc |= upload(c, c->d_sv_off, L.sv_off);
   rc |= upload(c, c->d_sv_slots, L.sv_slots);
   {
-    std::vector<int> sidx(L.nV, -1), iso;
+    // vslot[v]: the run of halo slots of vertex v (slots are numbered vertex by vertex), or
+    // (-1, -1) when its tile writes it
+    std::vector<int> iso, vslot((size_t)2 * L.nV, -1);
     for (int i = 0; i < L.nShared; ++i) {
-      sidx[L.sv_gid[i]] = i;
+      vslot[(size_t)2 * L.sv_gid[i]] = L.sv_off[i];
+      vslot[(size_t)2 * L.sv_gid[i] + 1] = L.sv_off[i + 1];
       if (L.sv_off[i + 1] == L.sv_off[i]) iso.push_back(L.sv_gid[i]);
     }
+    rc |= upload(c, c->d_vslot, vslot);
     c->n_iso = (int)iso.size();
-    rc |= upload(c, c->d_sidx, sidx);
     rc |= upload(c, c->d_iso, iso);
   }
   std::vector<double> soa;
@@ -395,6 +398,14 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
   rc |= upload(c, c->d_S, S);
   rc |= upload(c, c->d_b, b);
   rc |= upload(c, c->d_cline, cline);
+  {
+    std::vector<int> cl((size_t)2 * nV, -1);
+    for (int v = 0; v < nV; ++v) {
+      cl[(size_t)2 * v] = cidx[v];
+      cl[(size_t)2 * v + 1] = cidx[v] >= 0 ? cline[cidx[v]] : -1;
+    }
+    rc |= upload(c, c->d_cl, cl);
+  }
   rc |= upload(c, c->d_bids, bids);
   rc |= upload(c, c->d_bpos, vec(cs->bc, (size_t)cs->nb * 3));
   rc |= upload(c, c->d_cids, cids);
@@ -418,6 +429,7 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
   if (rc != HEXOPT_OK) return HEXOPT_EHIP;
   SoftDev& s = c->sd;
   s.cidx = c->d_cidx.as<int>(); s.S = c->d_S.as<double>(); s.b = c->d_b.as<double>(); s.cline = c->d_cline.as<int>();
+  s.cl = c->d_cl.as<int>();
   s.nb = cs->nb; s.nC = cs->nC; s.nT = cs->nT; s.nL = cs->nL;
   s.b_ids = c->d_bids.as<int>(); s.b_pos = c->d_bpos.as<double>();
   s.c_ids = c->d_cids.as<int>(); s.c_pos = c->d_cpos.as<double>();
@@ -617,7 +629,7 @@ static int cg32(hexopt_ctx* c, const SoftDev& sd, double prox, const float* b, f
   launch_cg32_init(nV, nL, sd, prox, b, c->f_invd.as<float>(), x, r, z, P[0], c->d_cg.p, c->d_part[0].as<double>(),
                    c->d_part[1].as<double>(), c->d_part[3].as<double>(), 1e-8, st);
   for (int k = 0; k < iters; ++k)
-    launch_cg32_iteration(c->tm, ro, c->f_K.as<float>(), prox, sd, c->d_sidx.as<int>(), c->d_iso.as<int>(), c->n_iso, z,
+    launch_cg32_iteration(c->tm, ro, c->f_K.as<float>(), prox, sd, c->d_vslot.as<int>(), c->d_iso.as<int>(), c->n_iso, z,
                           P[k & 1], P[(k + 1) & 1], c->f_tmp.as<float>(), c->f_halo.as<float>(), c->f_invd.as<float>(),
                           x, r, z, c->d_cg.p, c->d_part[2].as<double>(), c->d_part[4].as<double>(),
                           c->d_part[5].as<double>(), c->d_part[3].as<double>(), c->d_part[6].as<double>(),

@@ -39,6 +39,7 @@ struct SoftDev {
   const double* S;     // rows*6: sum of constraint Hessian halves (xx yy zz xy xz yz)
   const double* b;     // rows*3: constraint rhs
   const int* cline;    // rows: -1 or feature-line index
+  const int* cl;       // nV*2: (row in S/b or -1, feature-line index or -1) per vertex
   // entry lists used for the energy (the reference's own expression per entry)
   int nb, nC, nT, nL;
   const int* b_ids; const double* b_pos;
@@ -94,8 +95,8 @@ void launch_cg_zero_if32(int n, void* state, float* x, int nb, hipStream_t st);
 // Fused FP32 CG (Eigen's algorithm, FP64 dot products) on the CG operator of launch_apply32:
 // launch_cg32_init sets x = 0, r = b, z = b / diag, p_prev = 0 and the state; each
 // launch_cg32_iteration is one CG iteration reading z and p_prev, writing p_new and the
-// updated x, r, z (zout may alias z). sidx[v] is the fixup index of vertex v (-1 when its
-// tile writes it); iso lists the vertices without hexes.
+// updated x, r, z (zout may alias z). vslot[2v], vslot[2v+1] bound the halo slots of vertex
+// v (-1, -1 when its tile writes it); iso lists the vertices without hexes.
 void launch_cg32_init(int nV, int nL, const SoftDev& sd, double prox, const float* b, const float* invd, float* x,
                       float* r, float* z, float* pprev, void* state, double* part_b, double* part_rz, double* part_q,
                       double tol, hipStream_t st);
@@ -104,7 +105,7 @@ void launch_cg32_init(int nV, int nL, const SoftDev& sd, double prox, const floa
 void launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const float* z, const float* pprev,
                        float* pnew, float* out, float* halo, const void* state, double* part_tile, hipStream_t st);
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
-                           const int* sidx, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
+                           const int* vslot, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
                            double* part_tile, double* part_rr, double* part_rz, double* part_qz, double* part_bzq,
                            double* part_qq, hipStream_t st);

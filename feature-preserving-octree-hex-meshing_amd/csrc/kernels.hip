@@ -240,6 +240,39 @@ __device__ __forceinline__ void scatter_tile(const TileMesh& tm, int tile, int n
   }
 }
 
+// scatter_tile for 3 values per vertex kept as float4 in LDS (one 16-byte read-modify-write
+// per corner)
+__device__ __forceinline__ void scatter_tile4(const TileMesh& tm, int tile, int nh, bool active, const int li[8],
+                                              const float c[8][3], float4* acc) {
+  const int tid = threadIdx.x;
+  if (tm.tile_corner[tile]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (active) {
+        float4 v = acc[li[k]];
+        v.x += c[k][0]; v.y += c[k][1]; v.z += c[k][2];
+        acc[li[k]] = v;
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  const uint16_t* off = tm.tile_col + (size_t)tile * (kMaxCol + 1);
+  for (int col = 0; col < kMaxCol; ++col) {
+    const int a = off[col], b = off[col + 1];
+    if (a >= nh) break;
+    if (active && tid >= a && tid < b) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        float4 v = acc[li[k]];
+        v.x += c[k][0]; v.y += c[k][1]; v.z += c[k][2];
+        acc[li[k]] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Tiled scatter skeleton. Op provides:
 //   __device__ double hex(int h, T out[8][NC])        per-hex corner contributions; returns
@@ -1156,6 +1189,141 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
   if (tid == 0) partial[tile] = bs;
 }
 
+// Everything a tile pass loads before its first use, fetched one tile ahead by the
+// persistent operator pass below: the tile's ranges, the thread's local vertices (id, halo
+// slot, first-writer flag; at most kMaxLV / kTileHex = 4 per thread), its hex's corner ids
+// and 48 operator values.
+struct PipeTile {
+  int h0, nh, lv0, nlv;
+  int g[kMaxLV / kTileHex], s[kMaxLV / kTileHex];
+  unsigned first;  // bit j: local vertex tid + j * kTileHex writes p
+  int li[8];
+  float kr[48];
+};
+
+__device__ __forceinline__ void pipe_fetch(const TileMesh& tm, const float* __restrict__ K, int t, PipeTile& P) {
+  const int tid = threadIdx.x;
+  P.h0 = tm.tile_hex[t];
+  P.nh = tm.tile_hex[t + 1] - P.h0;
+  P.lv0 = tm.tile_lv[t];
+  P.nlv = tm.tile_lv[t + 1] - P.lv0;
+  P.first = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxLV / kTileHex; ++j) {
+    const int lv = tid + j * kTileHex;
+    if (lv < P.nlv) {
+      P.g[j] = tm.lv_gid[P.lv0 + lv];
+      P.s[j] = tm.lv_slot[P.lv0 + lv];
+      P.first |= tm.lv_first[P.lv0 + lv] ? (1u << j) : 0u;
+    }
+  }
+  if (tid < P.nh) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) P.li[k] = tm.lidx[(size_t)k * tm.nH + P.h0 + tid];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) k_pair(K, tm.nH, P.h0 + tid, q, P.kr + 12 * q);
+  }
+}
+
+// The cube-reference operator pass of cg32_apply_kernel as a persistent kernel: the 8 XCDs
+// take contiguous eighths of the tiles (as xcd_tile deals them), and within an XCD block r
+// takes tiles r, r + B, r + 2B, ... of that eighth (B blocks on the XCD). While a block
+// scatters and writes one tile, the loads of its next tile are in flight.
+__global__ __launch_bounds__(kTileHex) void cg32_apply_pipe_kernel(TileMesh tm, const float* __restrict__ K,
+                                                                   const float* __restrict__ z,
+                                                                   const float* __restrict__ pprev,
+                                                                   float* __restrict__ pnew, float* __restrict__ out,
+                                                                   float4* __restrict__ halo, const CGState* st,
+                                                                   double* partial) {
+  __shared__ float4 sp[kMaxLV];
+  __shared__ float4 acc[kMaxLV];
+  __shared__ double red[16];
+  if (st->done) return;
+  const int tid = threadIdx.x;
+  const int nV = tm.nV, nT = tm.nTiles;
+  const float beta = (float)st->beta;
+  const int q8 = nT / 8, r8 = nT % 8;
+  const int x = blockIdx.x % 8, rank = blockIdx.x / 8;
+  const int nb = ((int)gridDim.x + 7 - x) / 8;  // blocks on XCD x
+  const int tend = (x + 1) * q8 + (x + 1 < r8 ? x + 1 : r8);
+  int tile = x * q8 + (x < r8 ? x : r8) + rank;
+  PipeTile P;
+  if (tile < tend) pipe_fetch(tm, K, tile, P);
+  while (tile < tend) {
+    // stage p of the tile's vertices
+#pragma unroll
+    for (int j = 0; j < kMaxLV / kTileHex; ++j) {
+      const int lv = tid + j * kTileHex;
+      if (lv < P.nlv) {
+        const int g = P.g[j];
+        float pv[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) pv[d] = z[(size_t)d * nV + g] + beta * pprev[(size_t)d * nV + g];
+        sp[lv] = make_float4(pv[0], pv[1], pv[2], 0.0f);
+        acc[lv] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (P.first & (1u << j)) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) pnew[(size_t)d * nV + g] = pv[d];
+        }
+      }
+    }
+    __syncthreads();
+    const bool active = tid < P.nh;
+    float c[8][3];
+    double part = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) c[k][d] = 0.0f;
+    if (active) {
+      float q[8][3];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const float4 v = sp[P.li[k]];
+        q[k][0] = v.x; q[k][1] = v.y; q[k][2] = v.z;
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        float E[3][3];
+        tet_edges(q, t, E);
+        tet_forces_cube(E, P.kr + 6 * t, t, c);
+      }
+      float e = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) e += q[k][d] * c[k][d];
+      part = (double)e;
+    }
+    // this tile's indices, then the next tile's loads
+    const int cur = tile, nh = P.nh, nlv = P.nlv;
+    int li[8], g[kMaxLV / kTileHex], sl[kMaxLV / kTileHex];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) li[k] = P.li[k];
+#pragma unroll
+    for (int j = 0; j < kMaxLV / kTileHex; ++j) { g[j] = P.g[j]; sl[j] = P.s[j]; }
+    tile += nb;
+    if (tile < tend) pipe_fetch(tm, K, tile, P);
+    scatter_tile4(tm, cur, nh, active, li, c, acc);
+#pragma unroll
+    for (int j = 0; j < kMaxLV / kTileHex; ++j) {
+      const int lv = tid + j * kTileHex;
+      if (lv < nlv) {
+        const float4 v = acc[lv];
+        if (sl[j] < 0) {
+          out[g[j]] = v.x;
+          out[(size_t)nV + g[j]] = v.y;
+          out[2 * (size_t)nV + g[j]] = v.z;
+        } else {
+          halo[sl[j]] = v;
+        }
+      }
+    }
+    const double bs = block_sum(part, red);
+    if (tid == 0) partial[cur] = bs;
+  }
+}
+
 // p of the vertices without hexes (no tile stages them), then
 // alpha = absNew / (p.E p + z.R z + 2 beta z.R q + beta^2 q.R q)
 __global__ __launch_bounds__(1024) void cg32_alpha_kernel(CGState* st, const double* part_tile, int ntiles, int nV,
@@ -1203,7 +1371,7 @@ __global__ __launch_bounds__(256) void cg32_lines_kernel(SoftDev sd, int nV, con
 // one thread per vertex, which also updates its line unknown (each line unknown belongs to
 // exactly one vertex)
 __global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev sd, double prox,
-                                                          const int* __restrict__ sidx, const float4* __restrict__ halo,
+                                                          const int2* __restrict__ vslot, const float4* __restrict__ halo,
                                                           const float* __restrict__ p, const float* __restrict__ out,
                                                           const float* __restrict__ invd, float* __restrict__ x,
                                                           float* __restrict__ r, float* __restrict__ z,
@@ -1218,22 +1386,22 @@ __global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev s
   double srr = 0, srz = 0, fz = 0, fzq = 0, fq = 0;
   if (g < nV) {
     double a[3];
-    const int si = sidx[g];
-    if (si < 0) {
+    const int2 run = vslot[g];
+    if (run.x < 0) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) a[d] = (double)out[(size_t)d * nV + g];
     } else {
       a[0] = a[1] = a[2] = 0.0;
-      for (int k = tm.sv_off[si]; k < tm.sv_off[si + 1]; ++k) {
-        const float4 hv = halo[tm.sv_slots[k]];
+      for (int k = run.x; k < run.y; ++k) {
+        const float4 hv = halo[k];
         a[0] += (double)hv.x; a[1] += (double)hv.y; a[2] += (double)hv.z;
       }
     }
     double pv[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) pv[d] = (double)p[(size_t)d * nV + g];
-    const int ci = sd.cidx ? sd.cidx[g] : -1;
-    const int ln = ci >= 0 ? sd.cline[ci] : -1;
+    const int2 cl = sd.cl ? reinterpret_cast<const int2*>(sd.cl)[g] : make_int2(-1, -1);
+    const int ci = cl.x, ln = cl.y;
     const double pa = ln >= 0 ? (double)p[nA + ln] : 0.0;
     double vr[3];
     vertex_rows(sd, prox, ci, ln, pv, pa, vr);
@@ -1843,16 +2011,28 @@ void launch_cg32_init(int nV, int nL, const SoftDev& sd, double prox, const floa
 void launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const float* z, const float* pprev,
                        float* pnew, float* out, float* halo, const void* state, double* part_tile, hipStream_t st) {
   float4* h4 = reinterpret_cast<float4*>(halo);
-  if (ro.mode == REF_CUBE)
-    hipLaunchKernelGGL(cg32_apply_kernel<true>, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out,
-                       h4, (const CGState*)state, part_tile);
-  else
+  if (ro.mode == REF_CUBE) {
+    // persistent: as many blocks as fit on the device at once, at most one per tile
+    static int fit[64] = {0};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev >= 0 && dev < 64 && fit[dev] == 0) {
+      int per_cu = 0, cus = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg32_apply_pipe_kernel, kTileHex, 0) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        per_cu = cus = 0;
+      fit[dev] = std::max(8, per_cu * cus);
+    }
+    const int grid = std::min(tm.nTiles, (dev >= 0 && dev < 64) ? fit[dev] : 8);
+    hipLaunchKernelGGL(cg32_apply_pipe_kernel, dim3(grid), dim3(kTileHex), 0, st, tm, K, z, pprev, pnew, out, h4,
+                       (const CGState*)state, part_tile);
+  } else
     hipLaunchKernelGGL(cg32_apply_kernel<false>, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out,
                        h4, (const CGState*)state, part_tile);
 }
 
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
-                           const int* sidx, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
+                           const int* vslot, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
                            double* part_tile, double* part_rr, double* part_rz, double* part_qz, double* part_bzq,
                            double* part_qq, hipStream_t st) {
@@ -1865,7 +2045,7 @@ void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, 
   hipLaunchKernelGGL(cg32_alpha_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_tile, tm.nTiles, tm.nV, iso,
                      niso, z, pprev, pnew);
   const int nbu = (tm.nV + 255) / 256;
-  hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, sidx,
+  hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, (const int2*)vslot,
                      (const float4*)halo, (const float*)pnew, (const float*)out, invd, x, r, zout,
                      (const CGState*)cs, part_rr, part_rz, part_qz, part_bzq, part_qq);
   hipLaunchKernelGGL(cg32_beta_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_rr, (const double*)part_rz,

@@ -173,7 +173,6 @@ bool build_layout(int nV, const double* V, int nH, const int* H, MeshLayout& L, 
   for (int v = 0; v < nV; ++v) vslot_start[v + 1] = vslot_start[v] + (ntiles_of_v[v] > 1 ? ntiles_of_v[v] : 0);
   L.nSlots = vslot_start[nV];
   L.sv_slots.assign(L.nSlots, -1);
-  int slot_next = 0;
   std::vector<int> local(nV, -1);
   for (int t = 0; t < L.nTiles; ++t) {
     // the tile's vertices in ascending id order
@@ -202,7 +201,9 @@ bool build_layout(int nV, const double* V, int nH, const int* H, MeshLayout& L, 
     for (int v : lv) {
       L.lv_gid.push_back(v);
       if (ntiles_of_v[v] > 1) {
-        int s = slot_next++;
+        // slots are numbered vertex by vertex (each vertex's in tile order), so the fold of a
+        // vertex reads one contiguous run
+        const int s = vslot_start[v] + slot_count[v];
         L.lv_slot.push_back(s);
         L.lv_first.push_back(slot_count[v] == 0 ? 1 : 0);
         L.sv_slots[vslot_start[v] + slot_count[v]++] = s;

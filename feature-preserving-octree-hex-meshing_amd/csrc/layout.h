@@ -37,7 +37,8 @@ struct MeshLayout {
                                      // phase k of the tile's scatter
   std::vector<int> sv_gid;      // vertices finished by the fixup pass (shared or isolated)
   std::vector<int> sv_off;      // nShared+1 ranges into sv_slots
-  std::vector<int> sv_slots;    // halo slots, ascending tile order
+  std::vector<int> sv_slots;    // halo slots, ascending tile order; slots are numbered vertex by
+                                // vertex, so sv_slots[k] == k
   int max_colours = 0;
 };
 
