@@ -780,8 +780,9 @@ int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch
                      c->d_halo.as<double>(), c->d_part[2].as<double>(), nullptr, nullptr, st);
         break;
       case 1:
-        launch_cg32_apply(c->tm, ro, c->f_K.as<float>(), c->f_z.as<float>(), c->f_p.as<float>(), c->f_p2.as<float>(),
-                          c->f_tmp.as<float>(), c->f_halo.as<float>(), c->d_cg.p, c->d_part[2].as<double>(), st);
+        (void)launch_cg32_apply(c->tm, ro, c->f_K.as<float>(), sd, c->d_iso.as<int>(), c->n_iso, c->f_z.as<float>(),
+                                c->f_p.as<float>(), c->f_p2.as<float>(), c->f_tmp.as<float>(), c->f_halo.as<float>(),
+                                c->d_cg.p, c->d_part[2].as<double>(), st);
         break;
       case 2:
         launch_local_step(tm, ro, c->d_cur.as<double>(), c->d_K.as<double>(), HEXOPT_SYMMETRIC_DIRICHLET, 1.0, 1.0,

@@ -100,10 +100,13 @@ void launch_cg_zero_if32(int n, void* state, float* x, int nb, hipStream_t st);
 void launch_cg32_init(int nV, int nL, const SoftDev& sd, double prox, const float* b, const float* invd, float* x,
                       float* r, float* z, float* pprev, void* state, double* part_b, double* part_rz, double* part_q,
                       double tol, hipStream_t st);
-// The operator pass of one iteration alone (the element part E p and the p.E p partials),
-// for timing. halo holds 4 floats per slot.
-void launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const float* z, const float* pprev,
-                       float* pnew, float* out, float* halo, const void* state, double* part_tile, hipStream_t st);
+// The operator pass of one iteration (the element part E p, the p.E p partials, p and the
+// a_L rows of the line unknowns). halo holds 4 floats per slot. Returns the number of
+// partials, positive when the pass also set p of the vertices without hexes (iso) and
+// negative when that is left to the caller.
+int launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const SoftDev& sd, const int* iso, int niso,
+                      const float* z, const float* pprev, float* pnew, float* out, float* halo, const void* state,
+                      double* part, hipStream_t st);
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
                            const int* vslot, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,

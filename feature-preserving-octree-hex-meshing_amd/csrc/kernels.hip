@@ -436,7 +436,7 @@ __device__ void general_B(const double x[4][3], const double r[4][3], double vol
 
 // Pass 1: per hex, cube-reference operator if the hex qualifies (compute_referenceMesh with
 // cube=true, |vol| > 1e-20, H_flag true); counts the hexes that need the general operator.
-__global__ void ref_cube_kernel(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube,
+__global__ __launch_bounds__(256) void ref_cube_kernel(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube,
                                 double* invs, int* n_general) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nH) return;
@@ -453,7 +453,7 @@ __global__ void ref_cube_kernel(const double* X, int nV, const int* conn, int nH
 }
 
 // Pass 2 (only when some hex needs it): the general operator B for every tet.
-__global__ void ref_general_kernel(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube,
+__global__ __launch_bounds__(256) void ref_general_kernel(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube,
                                    double* Bout) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nH) return;
@@ -1189,6 +1189,25 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
   if (tid == 0) partial[tile] = bs;
 }
 
+// line unknown i: p_a = z_a + beta p_prev_a and the a_L row (L p)_a = lamL (|t|^2 p_a - t.p_v)
+__device__ __forceinline__ void line_row(const SoftDev& sd, int nV, const float* __restrict__ z,
+                                         const float* __restrict__ pprev, float* __restrict__ pnew,
+                                         float* __restrict__ out, float beta, int i) {
+  const size_t nA = 3 * (size_t)nV;
+  const int v = sd.lv[i];
+  const float pa = z[nA + i] + beta * pprev[nA + i];
+  pnew[nA + i] = pa;
+  const double* t = sd.lt + (size_t)i * 3;
+  double tp = 0, tt = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double pv = (double)(z[(size_t)d * nV + v] + beta * pprev[(size_t)d * nV + v]);
+    tp += t[d] * pv;
+    tt += t[d] * t[d];
+  }
+  out[nA + i] = (float)(sd.lamL * (tt * (double)pa - tp));
+}
+
 // Everything a tile pass loads before its first use, fetched one tile ahead by the
 // persistent operator pass below: the tile's ranges, the thread's local vertices (id, halo
 // slot, first-writer flag; at most kMaxLV / kTileHex = 4 per thread), its hex's corner ids
@@ -1233,8 +1252,9 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_pipe_kernel(TileMesh tm, 
                                                                    const float* __restrict__ z,
                                                                    const float* __restrict__ pprev,
                                                                    float* __restrict__ pnew, float* __restrict__ out,
-                                                                   float4* __restrict__ halo, const CGState* st,
-                                                                   double* partial) {
+                                                                   float4* __restrict__ halo, SoftDev sd,
+                                                                   const int* __restrict__ iso, int niso,
+                                                                   const CGState* st, double* partial) {
   __shared__ float4 sp[kMaxLV];
   __shared__ float4 acc[kMaxLV];
   __shared__ double red[16];
@@ -1249,6 +1269,16 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_pipe_kernel(TileMesh tm, 
   int tile = x * q8 + (x < r8 ? x : r8) + rank;
   PipeTile P;
   if (tile < tend) pipe_fetch(tm, K, tile, P);
+  // the work outside the tiles, spread over all blocks: line unknowns, and p of the
+  // vertices without hexes
+  for (int i = blockIdx.x * kTileHex + tid; i < sd.nL; i += gridDim.x * kTileHex)
+    line_row(sd, nV, z, pprev, pnew, out, beta, i);
+  for (int i = blockIdx.x * kTileHex + tid; i < niso; i += gridDim.x * kTileHex) {
+    const int g = iso[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pnew[(size_t)d * nV + g] = z[(size_t)d * nV + g] + beta * pprev[(size_t)d * nV + g];
+  }
+  double part = 0.0;  // this thread's element quadratic forms, over the block's tiles in order
   while (tile < tend) {
     // stage p of the tile's vertices
 #pragma unroll
@@ -1270,7 +1300,6 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_pipe_kernel(TileMesh tm, 
     __syncthreads();
     const bool active = tid < P.nh;
     float c[8][3];
-    double part = 0.0;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
 #pragma unroll
@@ -1293,7 +1322,7 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_pipe_kernel(TileMesh tm, 
       for (int k = 0; k < 8; ++k)
 #pragma unroll
         for (int d = 0; d < 3; ++d) e += q[k][d] * c[k][d];
-      part = (double)e;
+      part += (double)e;
     }
     // this tile's indices, then the next tile's loads
     const int cur = tile, nh = P.nh, nlv = P.nlv;
@@ -1319,12 +1348,14 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_pipe_kernel(TileMesh tm, 
         }
       }
     }
-    const double bs = block_sum(part, red);
-    if (tid == 0) partial[cur] = bs;
+    // the next tile's staging overwrites sp and acc
+    __syncthreads();
   }
+  const double bs = block_sum(part, red);
+  if (tid == 0) partial[blockIdx.x] = bs;
 }
 
-// p of the vertices without hexes (no tile stages them), then
+// p of the vertices without hexes when the operator pass left them (niso > 0), then
 // alpha = absNew / (p.E p + z.R z + 2 beta z.R q + beta^2 q.R q)
 __global__ __launch_bounds__(1024) void cg32_alpha_kernel(CGState* st, const double* part_tile, int ntiles, int nV,
                                                           const int* __restrict__ iso, int niso,
@@ -1345,27 +1376,13 @@ __global__ __launch_bounds__(1024) void cg32_alpha_kernel(CGState* st, const dou
   }
 }
 
-// line unknowns: p_a = z_a + beta p_prev_a and the a_L rows (L p)_a = lamL (|t|^2 p_a - t.p_v)
+// line unknowns (the operator pass of general references leaves them to this kernel)
 __global__ __launch_bounds__(256) void cg32_lines_kernel(SoftDev sd, int nV, const float* __restrict__ z,
                                                          const float* __restrict__ pprev, float* __restrict__ pnew,
                                                          float* __restrict__ out, const CGState* st) {
   if (st->done) return;
-  const size_t nA = 3 * (size_t)nV;
-  const float beta = (float)st->beta;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= sd.nL) return;
-  const int v = sd.lv[i];
-  const float pa = z[nA + i] + beta * pprev[nA + i];
-  pnew[nA + i] = pa;
-  const double* t = sd.lt + (size_t)i * 3;
-  double tp = 0, tt = 0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double pv = (double)(z[(size_t)d * nV + v] + beta * pprev[(size_t)d * nV + v]);
-    tp += t[d] * pv;
-    tt += t[d] * t[d];
-  }
-  out[nA + i] = (float)(sd.lamL * (tt * (double)pa - tp));
+  if (i < sd.nL) line_row(sd, nV, z, pprev, pnew, out, (float)st->beta, i);
 }
 
 // one thread per vertex, which also updates its line unknown (each line unknown belongs to
@@ -1497,7 +1514,7 @@ __global__ void sum_fin_kernel(const double* part, int n, double* out) {
 
 // ---------------------------------------------------------------------------------------
 // Flip-avoiding step bound (flip_avoiding_line_search.cpp:24-298), FP64.
-__device__ int solve_p3(double x[3], double a, double b, double c) {
+__device__ __forceinline__ int solve_p3(double x[3], double a, double b, double c) {
   double a2 = a * a;
   double q = (a2 - 3 * b) / 9;
   double r = (a * (2 * a2 - 9 * b) + 27 * c) / 54;
@@ -1525,7 +1542,7 @@ __device__ int solve_p3(double x[3], double a, double b, double c) {
   return 1;
 }
 
-__device__ double smallest_pos_quad_zero(double a, double b, double c) {
+__device__ __forceinline__ double smallest_pos_quad_zero(double a, double b, double c) {
   double t1, t2;
   if (fabs(a) > 1.0e-10) {
     double delta_in = b * b - 4 * a * c;
@@ -1564,7 +1581,7 @@ __device__ __forceinline__ double dd3(const double a[3], const double b[3], cons
 // there; it reports INFINITY without solving, which leaves min(1, 0.8 t_min) unchanged since
 // 0.8 kRootCut > 1. NaN coefficients fail the test and take the exact path.
 constexpr double kRootCut = 1.3;
-__device__ double tet_min_root(const double x[4][3], const double d[4][3]) {
+__device__ __forceinline__ double tet_min_root(const double x[4][3], const double d[4][3]) {
   double e0[3], e1[3], e2[3], f0[3], f1[3], f2[3];
   for (int k = 0; k < 3; ++k) {
     e0[k] = x[1][k] - x[0][k]; e1[k] = x[2][k] - x[0][k]; e2[k] = x[3][k] - x[0][k];
@@ -1614,7 +1631,8 @@ __device__ __forceinline__ double line_dir(double a, DT q) {
 
 // direction d = dst - cur, taken per vertex on the fly
 template <bool DIFF, class DT>
-__global__ void max_step_kernel(const double* cur, const DT* dst, int nV, const int* conn, int nH, double* part) {
+__global__ __launch_bounds__(256) void max_step_kernel(const double* cur, const DT* dst, int nV, const int* conn,
+                                                       int nH, double* part) {
   __shared__ double red[16];
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   double m = INFINITY;
@@ -1650,7 +1668,7 @@ __global__ void min_fin_kernel(const double* part, int n, double* out) {
 // ---------------------------------------------------------------------------------------
 // Energy of X = cur + step (dst - cur) (compute_energy_with_jacobians); per-block partials.
 template <class T, bool DIFF, class DT>
-__global__ void energy_kernel(RefOp ro, const double* cur, const DT* dst, double step, int nV, const int* conn,
+__global__ __launch_bounds__(256) void energy_kernel(RefOp ro, const double* cur, const DT* dst, double step, int nV, const int* conn,
                               int nH, int energy, double M, double exp_f, double* part) {
   __shared__ double red[16];
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1827,7 +1845,7 @@ struct HvpOp {
 // ---------------------------------------------------------------------------------------
 // scaled_jacobian (global_functions.cpp:2309-2358, a_jacobian :2422-2442): per-hex min of
 // the 8 corner scaled Jacobians; partial min and sum.
-__global__ void scaled_jacobian_kernel(const double* X, int nV, const int* conn, int nH, double* per_hex,
+__global__ __launch_bounds__(256) void scaled_jacobian_kernel(const double* X, int nV, const int* conn, int nH, double* per_hex,
                                        double* part_min, double* part_sum) {
   __shared__ double red[16];
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2008,27 +2026,40 @@ void launch_cg32_init(int nV, int nL, const SoftDev& sd, double prox, const floa
                      (const double*)part_rz, nb, (const double*)part_q, nq, tol);
 }
 
-void launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const float* z, const float* pprev,
-                       float* pnew, float* out, float* halo, const void* state, double* part_tile, hipStream_t st) {
+namespace {
+// blocks of the persistent operator pass: as many as fit on the device at once
+int pipe_grid(int ntiles) {
+  static int fit[64] = {0};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev < 0 || dev >= 64) return std::min(ntiles, 8);
+  if (fit[dev] == 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg32_apply_pipe_kernel, kTileHex, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      per_cu = cus = 0;
+    fit[dev] = std::max(8, per_cu * cus);
+  }
+  return std::min(ntiles, fit[dev]);
+}
+}  // namespace
+
+int launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const SoftDev& sd, const int* iso, int niso,
+                      const float* z, const float* pprev, float* pnew, float* out, float* halo, const void* state,
+                      double* part, hipStream_t st) {
   float4* h4 = reinterpret_cast<float4*>(halo);
   if (ro.mode == REF_CUBE) {
-    // persistent: as many blocks as fit on the device at once, at most one per tile
-    static int fit[64] = {0};
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    if (dev >= 0 && dev < 64 && fit[dev] == 0) {
-      int per_cu = 0, cus = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg32_apply_pipe_kernel, kTileHex, 0) != hipSuccess ||
-          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        per_cu = cus = 0;
-      fit[dev] = std::max(8, per_cu * cus);
-    }
-    const int grid = std::min(tm.nTiles, (dev >= 0 && dev < 64) ? fit[dev] : 8);
-    hipLaunchKernelGGL(cg32_apply_pipe_kernel, dim3(grid), dim3(kTileHex), 0, st, tm, K, z, pprev, pnew, out, h4,
-                       (const CGState*)state, part_tile);
-  } else
-    hipLaunchKernelGGL(cg32_apply_kernel<false>, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out,
-                       h4, (const CGState*)state, part_tile);
+    const int grid = pipe_grid(tm.nTiles);
+    hipLaunchKernelGGL(cg32_apply_pipe_kernel, dim3(grid), dim3(kTileHex), 0, st, tm, K, z, pprev, pnew, out, h4, sd,
+                       iso, niso, (const CGState*)state, part);
+    return grid;
+  }
+  hipLaunchKernelGGL(cg32_apply_kernel<false>, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out,
+                     h4, (const CGState*)state, part);
+  if (sd.nL > 0)
+    hipLaunchKernelGGL(cg32_lines_kernel, dim3((sd.nL + 255) / 256), dim3(256), 0, st, sd, tm.nV, z, pprev, pnew, out,
+                       (const CGState*)state);
+  return -tm.nTiles;
 }
 
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
@@ -2037,13 +2068,11 @@ void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, 
                            double* part_tile, double* part_rr, double* part_rz, double* part_qz, double* part_bzq,
                            double* part_qq, hipStream_t st) {
   CGState* cs = (CGState*)state;
-  const int nL = sd.nL;
-  launch_cg32_apply(tm, ro, K, z, pprev, pnew, out, halo, cs, part_tile, st);
-  if (nL > 0)
-    hipLaunchKernelGGL(cg32_lines_kernel, dim3((nL + 255) / 256), dim3(256), 0, st, sd, tm.nV, z, pprev, pnew, out,
-                       (const CGState*)cs);
-  hipLaunchKernelGGL(cg32_alpha_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_tile, tm.nTiles, tm.nV, iso,
-                     niso, z, pprev, pnew);
+  // a positive count: the operator pass did the lines and the vertices without hexes
+  const int np = launch_cg32_apply(tm, ro, K, sd, iso, niso, z, pprev, pnew, out, halo, cs, part_tile, st);
+  const bool done_iso = np > 0;
+  hipLaunchKernelGGL(cg32_alpha_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_tile, done_iso ? np : -np,
+                     tm.nV, iso, done_iso ? 0 : niso, z, pprev, pnew);
   const int nbu = (tm.nV + 255) / 256;
   hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, (const int2*)vslot,
                      (const float4*)halo, (const float*)pnew, (const float*)out, invd, x, r, zout,
