@@ -709,22 +709,24 @@ struct AdjLocalOp32 {
       for (int d = 0; d < 3; ++d) p[k][d] = (float)pd[k][d];
     double e = 0.0;
     int inv = 0;
+    // FP64 orientation test of the corner tets first, so the FP64 corners are dead before the
+    // FP32 loop: sign of det J = det(E B). For a cube reference B = invs * (a permutation
+    // with determinant +1 for every corner) and invs > 0, so the sign is that of det E.
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      double E64[3][3];
+      tet_edges(pd, c, E64);
+      if (ro.mode == REF_CUBE) {
+        inv += det3(E64) < 0.0;
+      } else {
+        double B64[3][3], J64[3][3];
+        tet_B(ro, nH, h, c, B64);
+        matmul3(E64, B64, J64);
+        inv += det3(J64) < 0.0;
+      }
+    }
 #pragma unroll 1
     for (int c = 0; c < 8; ++c) {
-      {  // FP64 orientation test of the corner tet: sign of det J = det(E B). For a cube
-         // reference B = invs * (a permutation with determinant +1 for every corner) and
-         // invs > 0, so the sign is that of det E.
-        double E64[3][3];
-        tet_edges(pd, c, E64);
-        if (ro.mode == REF_CUBE) {
-          inv += det3(E64) < 0.0;
-        } else {
-          double B64[3][3], J64[3][3];
-          tet_B(ro, nH, h, c, B64);
-          matmul3(E64, B64, J64);
-          inv += det3(J64) < 0.0;
-        }
-      }
       float B[3][3], E[3][3], J[3][3], W[3][3], R[3][3], P[3][3];
       tet_B(ro, nH, h, c, B);
       tet_edges(p, c, E);
