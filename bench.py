@@ -29,14 +29,14 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "feature-preserving-octree-hex-meshing_amd"))
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "cg32_apply_torus1m_hbm_traffic.md")
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "cg32_apply_pipe_torus1m_hbm_traffic.md")
 R_MAJ, R_MIN, Z_CUT = 1.0, 0.35, 0.2
 WEIGHT_OPT = 0.001          # feature_alignment's opt.weight_opt = weight_opt * 0.001
 LAMDA0, LAMDA_BOUND = 1.0, 1e8
 
 
 def measured_traffic(nH):
-    """HBM bytes per launch of cg32_apply_kernel from the committed rocprofv3 FETCH_SIZE /
+    """HBM bytes per launch of cg32_apply_pipe_kernel from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (calibrated against a device copy of known size), when that summary
     was taken on a mesh of this many hexes; None otherwise."""
     try:
@@ -202,9 +202,9 @@ def main():
         "element_evals_per_s": round(value * nH, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "cg32_apply_kernel (fused FP32 CG iteration, operator half)",
+                     "kernel": "cg32_apply_pipe_kernel (persistent FP32 CG operator pass)",
                      "bytes_per_hex": round(bytes_per_hex32, 2), "algorithmic_bytes": round(bytes_per_hex32 * nH),
-                     "traffic_source": "profiles/cg32_apply_torus1m_hbm_traffic.md" if traffic else None},
+                     "traffic_source": "profiles/cg32_apply_pipe_torus1m_hbm_traffic.md" if traffic else None},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(max(args.hexes // 8, 20000))
