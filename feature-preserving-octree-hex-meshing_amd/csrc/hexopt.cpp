@@ -55,8 +55,8 @@ struct hexopt_ctx {
   std::string err;
   // layout on the device
   DevBuf d_conn, d_tile_hex, d_tile_col, d_tile_lv, d_lv_gid, d_lv_slot, d_lv_first, d_lidx, d_tile_corner, d_sv_gid,
-      d_sv_off, d_sv_slots;
-  DevBuf d_vslot, d_iso;  // per vertex its run of halo slots ((-1, -1): its tile writes it); vertices without hexes
+      d_sv_off, d_sv_slots, d_lv_dst, d_v_dst;
+  DevBuf d_iso;  // vertices without hexes
   DevBuf d_torder;       // plane queries sorted by vertex (Morton) order
   DevBuf d_cgsave;       // CG state of the last direction solve, read back with the line search
   int n_iso = 0;
@@ -192,19 +192,15 @@ hexopt_ctx* hexopt_create(int nV, const double* V, int nH, const int32_t* hexes,
   rc |= upload(c, c->d_lv_first, L.lv_first);
   rc |= upload(c, c->d_lidx, L.lidx);
   rc |= upload(c, c->d_tile_corner, L.tile_corner);
+  rc |= upload(c, c->d_lv_dst, L.lv_dst);
+  rc |= upload(c, c->d_v_dst, L.v_dst);
   rc |= upload(c, c->d_sv_gid, L.sv_gid);
   rc |= upload(c, c->d_sv_off, L.sv_off);
   rc |= upload(c, c->d_sv_slots, L.sv_slots);
   {
-    // vslot[v]: the run of halo slots of vertex v (slots are numbered vertex by vertex), or
-    // (-1, -1) when its tile writes it
-    std::vector<int> iso, vslot((size_t)2 * L.nV, -1);
-    for (int i = 0; i < L.nShared; ++i) {
-      vslot[(size_t)2 * L.sv_gid[i]] = L.sv_off[i];
-      vslot[(size_t)2 * L.sv_gid[i] + 1] = L.sv_off[i + 1];
+    std::vector<int> iso;
+    for (int i = 0; i < L.nShared; ++i)
       if (L.sv_off[i + 1] == L.sv_off[i]) iso.push_back(L.sv_gid[i]);
-    }
-    rc |= upload(c, c->d_vslot, vslot);
     c->n_iso = (int)iso.size();
     rc |= upload(c, c->d_iso, iso);
   }
@@ -221,7 +217,7 @@ hexopt_ctx* hexopt_create(int nV, const double* V, int nH, const int32_t* hexes,
                    c->d_tile_col.as<uint16_t>(), c->d_tile_lv.as<int>(), c->d_lv_gid.as<int>(),
                    c->d_lv_slot.as<int>(), c->d_lv_first.as<uint8_t>(), c->d_lidx.as<uint16_t>(),
                    c->d_sv_gid.as<int>(), c->d_sv_off.as<int>(), c->d_sv_slots.as<int>(),
-                   c->d_tile_corner.as<uint8_t>()};
+                   c->d_tile_corner.as<uint8_t>(), c->d_lv_dst.as<int>(), c->d_v_dst.as<int>()};
   hexopt_constraints none{};
   if (hexopt_set_constraints(c, &none) != HEXOPT_OK) {
     g_create_error = c->err;
@@ -607,7 +603,9 @@ static int ensure_state32(hexopt_ctx* c) {
   HIPCHK(c, c->f_K.ensure((size_t)48 * nH * sizeof(float)));
   for (DevBuf* b : {&c->f_rhs, &c->f_invd, &c->f_d, &c->f_r, &c->f_z, &c->f_p, &c->f_p2, &c->f_tmp, &c->f_e})
     HIPCHK(c, b->ensure(n * sizeof(float)));
-  HIPCHK(c, c->f_halo.ensure((size_t)6 * (c->L.nSlots + 1) * sizeof(float)));
+  // the FP32 local pass keeps 6 floats per halo slot, the FP32 CG a float4 per (tile, vertex) pair
+  const size_t nh32 = std::max((size_t)6 * (c->L.nSlots + 1), (size_t)4 * (c->L.lv_gid.size() + 1));
+  HIPCHK(c, c->f_halo.ensure(nh32 * sizeof(float)));
   HIPCHK(c, c->d_Ld64.ensure(n * sizeof(double)));
   HIPCHK(c, c->d_ninv.ensure(sizeof(int)));
   HIPCHK(c, c->d_cgsave.ensure(cg_state_bytes()));
@@ -629,7 +627,7 @@ static int cg32(hexopt_ctx* c, const SoftDev& sd, double prox, const float* b, f
   launch_cg32_init(nV, nL, sd, prox, b, c->f_invd.as<float>(), x, r, z, P[0], c->d_cg.p, c->d_part[0].as<double>(),
                    c->d_part[1].as<double>(), c->d_part[3].as<double>(), 1e-8, st);
   for (int k = 0; k < iters; ++k)
-    launch_cg32_iteration(c->tm, ro, c->f_K.as<float>(), prox, sd, c->d_vslot.as<int>(), c->d_iso.as<int>(), c->n_iso, z,
+    launch_cg32_iteration(c->tm, ro, c->f_K.as<float>(), prox, sd, c->d_iso.as<int>(), c->n_iso, z,
                           P[k & 1], P[(k + 1) & 1], c->f_tmp.as<float>(), c->f_halo.as<float>(), c->f_invd.as<float>(),
                           x, r, z, c->d_cg.p, c->d_part[2].as<double>(), c->d_part[4].as<double>(),
                           c->d_part[5].as<double>(), c->d_part[3].as<double>(), c->d_part[6].as<double>(),

@@ -21,6 +21,8 @@ struct TileMesh {
   const int* sv_off;
   const int* sv_slots;
   const uint8_t* tile_corner;
+  const int* lv_dst;  // per local vertex: place among all (tile, vertex) pairs, vertex by vertex
+  const int* v_dst;   // nV+1: each vertex's run of places
 };
 
 enum RefMode { REF_CUBE = 0, REF_GENERAL = 1 };
@@ -95,20 +97,21 @@ void launch_cg_zero_if32(int n, void* state, float* x, int nb, hipStream_t st);
 // Fused FP32 CG (Eigen's algorithm, FP64 dot products) on the CG operator of launch_apply32:
 // launch_cg32_init sets x = 0, r = b, z = b / diag, p_prev = 0 and the state; each
 // launch_cg32_iteration is one CG iteration reading z and p_prev, writing p_new and the
-// updated x, r, z (zout may alias z). vslot[2v], vslot[2v+1] bound the halo slots of vertex
-// v (-1, -1 when its tile writes it); iso lists the vertices without hexes.
+// updated x, r, z (zout may alias z). Every tile writes E p of each of its vertices as a float4
+// at lv_dst (halo holds one float4 per (tile, vertex) pair); the update sums vertex v's run
+// v_dst[v] .. v_dst[v+1]. iso lists the vertices without hexes.
 void launch_cg32_init(int nV, int nL, const SoftDev& sd, double prox, const float* b, const float* invd, float* x,
                       float* r, float* z, float* pprev, void* state, double* part_b, double* part_rz, double* part_q,
                       double tol, hipStream_t st);
 // The operator pass of one iteration (the element part E p, the p.E p partials, p and the
-// a_L rows of the line unknowns). halo holds 4 floats per slot. Returns the number of
+// a_L rows of the line unknowns). halo holds a float4 per (tile, vertex) pair. Returns the number of
 // partials, positive when the pass also set p of the vertices without hexes (iso) and
 // negative when that is left to the caller.
 int launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const SoftDev& sd, const int* iso, int niso,
                       const float* z, const float* pprev, float* pnew, float* out, float* halo, const void* state,
                       double* part, hipStream_t st);
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
-                           const int* vslot, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
+                           const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
                            double* part_tile, double* part_rr, double* part_rz, double* part_qz, double* part_bzq,
                            double* part_qq, hipStream_t st);

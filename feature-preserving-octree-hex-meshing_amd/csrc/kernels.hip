@@ -1177,16 +1177,9 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_kernel(TileMesh tm, RefOp
     part = (double)e;
   }
   scatter_tile<float, 3>(tm, tile, nh, active, li, c, acc);
-  // E p: straight to the vertex when the tile is its only one, else to the tile's halo slot
-  for (int lv = tid; lv < nlv; lv += kTileHex) {
-    const int g = tm.lv_gid[lv0 + lv], s = tm.lv_slot[lv0 + lv];
-    if (s < 0) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) out[(size_t)d * nV + g] = acc[d * kMaxLV + lv];
-    } else {
-      halo[s] = make_float4(acc[lv], acc[kMaxLV + lv], acc[2 * kMaxLV + lv], 0.0f);
-    }
-  }
+  // E p of each local vertex to its place; the update sums each vertex's run
+  for (int lv = tid; lv < nlv; lv += kTileHex)
+    halo[tm.lv_dst[lv0 + lv]] = make_float4(acc[lv], acc[kMaxLV + lv], acc[2 * kMaxLV + lv], 0.0f);
   const double bs = block_sum(part, red);
   if (tid == 0) partial[tile] = bs;
 }
@@ -1211,9 +1204,9 @@ __device__ __forceinline__ void line_row(const SoftDev& sd, int nV, const float*
 }
 
 // Everything a tile pass loads before its first use, fetched one tile ahead by the
-// persistent operator pass below: the tile's ranges, the thread's local vertices (id, halo
-// slot, first-writer flag; at most kMaxLV / kTileHex = 4 per thread), its hex's corner ids
-// and 48 operator values.
+// persistent operator pass below: the tile's ranges, the thread's local vertices (id, place
+// of its E p, first-writer flag; at most kMaxLV / kTileHex = 4 per thread), its hex's corner
+// ids and 48 operator values.
 struct PipeTile {
   int h0, nh, lv0, nlv;
   int g[kMaxLV / kTileHex], s[kMaxLV / kTileHex];
@@ -1234,7 +1227,7 @@ __device__ __forceinline__ void pipe_fetch(const TileMesh& tm, const float* __re
     const int lv = tid + j * kTileHex;
     if (lv < P.nlv) {
       P.g[j] = tm.lv_gid[P.lv0 + lv];
-      P.s[j] = tm.lv_slot[P.lv0 + lv];
+      P.s[j] = tm.lv_dst[P.lv0 + lv];
       P.first |= tm.lv_first[P.lv0 + lv] ? (1u << j) : 0u;
     }
   }
@@ -1340,14 +1333,7 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_pipe_kernel(TileMesh tm, 
     for (int j = 0; j < kMaxLV / kTileHex; ++j) {
       const int lv = tid + j * kTileHex;
       if (lv < nlv) {
-        const float4 v = acc[lv];
-        if (sl[j] < 0) {
-          out[g[j]] = v.x;
-          out[(size_t)nV + g[j]] = v.y;
-          out[2 * (size_t)nV + g[j]] = v.z;
-        } else {
-          halo[sl[j]] = v;
-        }
+        halo[sl[j]] = acc[lv];
       }
     }
     // the next tile's staging overwrites sp and acc
@@ -1390,7 +1376,7 @@ __global__ __launch_bounds__(256) void cg32_lines_kernel(SoftDev sd, int nV, con
 // one thread per vertex, which also updates its line unknown (each line unknown belongs to
 // exactly one vertex)
 __global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev sd, double prox,
-                                                          const int2* __restrict__ vslot, const float4* __restrict__ halo,
+                                                          const float4* __restrict__ halo,
                                                           const float* __restrict__ p, const float* __restrict__ out,
                                                           const float* __restrict__ invd, float* __restrict__ x,
                                                           float* __restrict__ r, float* __restrict__ z,
@@ -1404,17 +1390,12 @@ __global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev s
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   double srr = 0, srz = 0, fz = 0, fzq = 0, fq = 0;
   if (g < nV) {
-    double a[3];
-    const int2 run = vslot[g];
-    if (run.x < 0) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) a[d] = (double)out[(size_t)d * nV + g];
-    } else {
-      a[0] = a[1] = a[2] = 0.0;
-      for (int k = run.x; k < run.y; ++k) {
-        const float4 hv = halo[k];
-        a[0] += (double)hv.x; a[1] += (double)hv.y; a[2] += (double)hv.z;
-      }
+    // E p: the sum of the vertex's run (one place per tile holding it, in tile order)
+    double a[3] = {0.0, 0.0, 0.0};
+    const int k0 = tm.v_dst[g], k1 = tm.v_dst[g + 1];
+    for (int k = k0; k < k1; ++k) {
+      const float4 hv = halo[k];
+      a[0] += (double)hv.x; a[1] += (double)hv.y; a[2] += (double)hv.z;
     }
     double pv[3];
 #pragma unroll
@@ -2065,7 +2046,7 @@ int launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const
 }
 
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
-                           const int* vslot, const int* iso, int niso, const float* z, const float* pprev, float* pnew,
+                           const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
                            double* part_tile, double* part_rr, double* part_rz, double* part_qz, double* part_bzq,
                            double* part_qq, hipStream_t st) {
@@ -2076,8 +2057,7 @@ void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, 
   hipLaunchKernelGGL(cg32_alpha_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_tile, done_iso ? np : -np,
                      tm.nV, iso, done_iso ? 0 : niso, z, pprev, pnew);
   const int nbu = (tm.nV + 255) / 256;
-  hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, (const int2*)vslot,
-                     (const float4*)halo, (const float*)pnew, (const float*)out, invd, x, r, zout,
+  hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, (const float4*)halo, (const float*)pnew, (const float*)out, invd, x, r, zout,
                      (const CGState*)cs, part_rr, part_rz, part_qz, part_bzq, part_qq);
   hipLaunchKernelGGL(cg32_beta_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_rr, (const double*)part_rz,
                      (const double*)part_qz, (const double*)part_bzq, (const double*)part_qq, nbu);

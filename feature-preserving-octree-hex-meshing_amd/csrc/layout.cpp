@@ -171,6 +171,9 @@ bool build_layout(int nV, const double* V, int nH, const int* H, MeshLayout& L, 
   std::vector<int> slot_count(nV, 0);
   std::vector<int> vslot_start(nV + 1, 0);
   for (int v = 0; v < nV; ++v) vslot_start[v + 1] = vslot_start[v] + (ntiles_of_v[v] > 1 ? ntiles_of_v[v] : 0);
+  L.v_dst.assign(nV + 1, 0);
+  for (int v = 0; v < nV; ++v) L.v_dst[v + 1] = L.v_dst[v] + ntiles_of_v[v];
+  std::vector<int> dst_count(nV, 0);
   L.nSlots = vslot_start[nV];
   L.sv_slots.assign(L.nSlots, -1);
   std::vector<int> local(nV, -1);
@@ -200,6 +203,7 @@ bool build_layout(int nV, const double* V, int nH, const int* H, MeshLayout& L, 
     }
     for (int v : lv) {
       L.lv_gid.push_back(v);
+      L.lv_dst.push_back(L.v_dst[v] + dst_count[v]++);
       if (ntiles_of_v[v] > 1) {
         // slots are numbered vertex by vertex (each vertex's in tile order), so the fold of a
         // vertex reads one contiguous run

@@ -32,6 +32,9 @@ struct MeshLayout {
   std::vector<uint8_t> lv_first;  // 1: the vertex is this tile's alone, or this tile holds its
                                   //    first halo slot (the one tile that writes per-vertex data)
   std::vector<uint16_t> lidx;   // [8][nH] local vertex of each corner within its tile
+  std::vector<int> lv_dst;      // per local vertex: its place among all (tile, vertex) pairs,
+                                // numbered vertex by vertex (each vertex's in tile order)
+  std::vector<int> v_dst;       // nV+1: vertex v's run of places in that numbering
   std::vector<uint8_t> tile_corner;  // 1: no vertex of the tile sits at the same corner index of
                                      // two of its hexes, so corner k of every hex can be added in
                                      // phase k of the tile's scatter
