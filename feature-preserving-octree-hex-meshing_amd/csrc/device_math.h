@@ -37,6 +37,14 @@ template <class T> __device__ __forceinline__ T det3(const T a[3][3]) {
 template <class T> __device__ __forceinline__ T dsqrt(T x) { return sqrt(x); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
 
+// Reciprocal and reciprocal square root for the Jacobi rotations: exact division and sqrt
+// in FP64; in FP32 the hardware's 1-ulp approximations, well inside the FP32 SVD's own
+// rounding (its orthogonality test stops at 1e-6).
+template <class T> __device__ __forceinline__ T rot_rcp(T x) { return T(1) / x; }
+template <> __device__ __forceinline__ float rot_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <class T> __device__ __forceinline__ T rot_rsqrt(T x) { return T(1) / sqrt(x); }
+template <> __device__ __forceinline__ float rot_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // One-sided Jacobi SVD of a 3x3 matrix A = U diag(s) V^T, s >= 0 descending, U, V
 // orthogonal (the contract of Eigen::JacobiSVD that igl::polar_svd relies on). Column pairs
 // of B = A V are rotated until orthogonal; the column norms are the singular values.
@@ -62,12 +70,21 @@ __device__ void svd3(const T A[3][3], T U[3][3], T s[3], T V[3][3]) {
       T beta = B[0][q] * B[0][q] + B[1][q] * B[1][q] + B[2][q] * B[2][q];
       T gamma = B[0][p] * B[0][q] + B[1][p] * B[1][q] + B[2][p] * B[2][q];
       if (gamma == T(0)) continue;
-      T conv = fabs(gamma) / dsqrt(alpha * beta);
+      T conv, zeta, t, c;
+      if (sizeof(T) == 8) {
+        conv = fabs(gamma) / dsqrt(alpha * beta);
+        zeta = (beta - alpha) / (T(2) * gamma);
+        t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + dsqrt(T(1) + zeta * zeta));
+        c = T(1) / dsqrt(T(1) + t * t);
+      } else {
+        conv = fabs(gamma) * rot_rsqrt(alpha * beta);
+        zeta = (beta - alpha) * rot_rcp(T(2) * gamma);
+        t = (zeta >= T(0) ? T(1) : T(-1)) * rot_rcp(fabs(zeta) + dsqrt(T(1) + zeta * zeta));
+        c = rot_rsqrt(T(1) + t * t);
+      }
       if (!(conv > tol)) continue;
       off = fmax(off, conv);
-      T zeta = (beta - alpha) / (T(2) * gamma);
-      T t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + dsqrt(T(1) + zeta * zeta));
-      T c = T(1) / dsqrt(T(1) + t * t), sn = c * t;
+      const T sn = c * t;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         T bp = B[i][p], bq = B[i][q];
