@@ -73,6 +73,11 @@ struct hexopt_ctx {
       d_invd_a, d_aL;
   SoftDev sd{};
   int nL = 0;
+  // padding glue (hexopt_glue_setup): groups of copies, pinned copies, the original mesh
+  bool glue = false;
+  int nO = 0;
+  double lamG = 0;
+  DevBuf d_g_of, d_g_off, d_g_ids, d_gsum, d_gfixed, d_Vo, d_Ho, d_xfix, d_gmean;
   std::vector<double> h_lt;  // nL*3 (caller copy for rhs/diag of the a_L rows)
   // adjoint scratch
   DevBuf d_X, d_G, d_v;
@@ -139,17 +144,22 @@ int ensure_state(hexopt_ctx* c) {
   return HEXOPT_OK;
 }
 
+SoftDev soft_with(hexopt_ctx* c, const hexopt_params* prm);
+
 int energy_now(hexopt_ctx* c, const hexopt_params* prm, const double* dst, double step, const double* aL, double* eq,
                double* es) {
   const int nsb = nblocks(c->sd.nb + c->sd.nC + c->sd.nT + c->sd.nL);
   double* out = c->d_scal.as<double>();
   launch_energy(refop(c), c->d_cur.as<double>(), dst, step, c->L.nV, c->tm.conn, c->L.nH, prm->energy, prm->weight_opt,
                 prm->exp_factor, c->sd, aL, c->d_part[0].as<double>(), c->d_part[1].as<double>(), nsb, out, c->stream);
-  double h[2];
-  HIPCHK(c, hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  double h[3] = {0, 0, 0};
+  if (c->glue)
+    launch_glue_energy(soft_with(c, prm), c->L.nV, c->d_cur.as<double>(), dst, step, c->d_part[2].as<double>(), out + 2,
+                       c->stream);
+  HIPCHK(c, hipMemcpyAsync(h, out, (c->glue ? 3 : 2) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *eq = h[0];
-  *es = h[1];
+  *es = h[1] + h[2];
   return HEXOPT_OK;
 }
 
@@ -157,6 +167,15 @@ SoftDev soft_with(hexopt_ctx* c, const hexopt_params* prm) {
   SoftDev s = c->sd;
   s.lamP = prm->lamda_projection;
   if (!prm->projection) s.nb = 0;
+  if (c->glue) {
+    s.nG = c->nO;
+    s.g_of = c->d_g_of.as<int>();
+    s.g_off = c->d_g_off.as<int>();
+    s.g_ids = c->d_g_ids.as<int>();
+    s.g_sum = c->d_gsum.as<double>();
+    s.lamG = c->lamG;
+    s.fixed = c->d_gfixed.as<uint8_t>();
+  }
   return s;
 }
 
@@ -450,6 +469,8 @@ int hexopt_slim_solve(hexopt_ctx* c, const hexopt_params* prm, int iters, hexopt
   if (prm->energy < 0 || prm->energy > 5) return fail(c, HEXOPT_EINVAL, "unknown energy");
   int rc = check_projection(c, prm);
   if (rc != HEXOPT_OK) return rc;
+  if (c->glue && (c->sd.nC || c->sd.nT || c->nL || prm->projection))
+    return fail(c, HEXOPT_EUNSUPPORTED, "the glue takes no feature or projection constraints (slim_m.cpp:647)");
   HIPCHK(c, hipSetDevice(c->device));
   if ((rc = ensure_state(c)) != HEXOPT_OK) return rc;
   const int nV = c->L.nV, nH = c->L.nH, nL = c->nL;
@@ -479,18 +500,33 @@ int hexopt_slim_solve(hexopt_ctx* c, const hexopt_params* prm, int iters, hexopt
                                hipMemcpyDeviceToDevice, st));
       HIPCHK(c, hipMemsetAsync(x + 3 * (size_t)nV, 0, nL * sizeof(double), st));
     }
-    // CG warm start at the current positions (a_L guess 0)
-    HIPCHK(c, hipMemcpyAsync(x, cur, 3 * (size_t)nV * sizeof(double), hipMemcpyDeviceToDevice, st));
     double* pt = c->d_part[2].as<double>();
     double* pf = c->d_part[3].as<double>();
     double* pl2 = c->d_part[4].as<double>();
     const int nfix = (c->L.nShared + 255) / 256, nline = (nL + 255) / 256;
+    double* gsum = c->glue ? c->d_gsum.as<double>() : nullptr;
+    if (c->glue) {
+      // known_value_post (slim_m.cpp:403-475): the pinned rows leave the system. Its right-hand
+      // side is rhs - L x_pinned on the free rows, and the CG runs on the free rows from the
+      // free part of the current positions (pinned rows of r, z and p stay 0)
+      double* xfix = c->d_xfix.as<double>();
+      launch_glue_split(nV, c->d_invd.as<double>(), cur, xfix, x, st);
+      launch_group_sum(sd, nV, xfix, gsum, st);
+      launch_apply(c->tm, ro, c->d_K.as<double>(), xfix, c->d_tmp.as<double>(), prox, sd, c->d_halo.as<double>(), pt,
+                   pf, pl2, st);
+      launch_glue_rhs(nV, c->d_invd.as<double>(), c->d_rhs.as<double>(), c->d_tmp.as<double>(), st);
+      launch_group_sum(sd, nV, x, gsum, st);
+    } else {
+      // CG warm start at the current positions (a_L guess 0)
+      HIPCHK(c, hipMemcpyAsync(x, cur, 3 * (size_t)nV * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
     launch_apply(c->tm, ro, c->d_K.as<double>(), x, c->d_tmp.as<double>(), prox, sd, c->d_halo.as<double>(), pt, pf,
                  pl2, st);
     launch_cg_init(n, c->d_rhs.as<double>(), c->d_tmp.as<double>(), c->d_r.as<double>(), c->d_part[0].as<double>(),
                    c->d_part[1].as<double>(), nb, c->d_cg.p, c->d_invd.as<double>(), c->d_p.as<double>(),
                    c->d_part[5].as<double>(), 1e-8, st);
     for (int k = 0; k < 20; ++k) {
+      if (c->glue) launch_group_sum(sd, nV, c->d_p.as<double>(), gsum, st);
       launch_apply(c->tm, ro, c->d_K.as<double>(), c->d_p.as<double>(), c->d_tmp.as<double>(), prox, sd,
                    c->d_halo.as<double>(), pt, pf, pl2, st);
       launch_cg_alpha(c->d_cg.p, pt, c->L.nTiles, pf, nfix, pl2, nline, st);
@@ -499,6 +535,7 @@ int hexopt_slim_solve(hexopt_ctx* c, const hexopt_params* prm, int iters, hexopt
                        c->d_part[7].as<double>(), nb, c->d_p.as<double>(), st);
     }
     launch_cg_zero_if(n, c->d_cg.p, x, nb, st);
+    if (c->glue) launch_glue_restore(nV, c->d_invd.as<double>(), cur, x, st);
     if (nL > 0 && !prm->projection)
       HIPCHK(c, hipMemcpyAsync(aL, x + 3 * (size_t)nV, nL * sizeof(double), hipMemcpyDeviceToDevice, st));
     // flip-avoiding step bound
@@ -554,6 +591,114 @@ int hexopt_slim_solve(hexopt_ctx* c, const hexopt_params* prm, int iters, hexopt
   return HEXOPT_OK;
 }
 
+int hexopt_glue_setup(hexopt_ctx* c, int nO, const double* V_orig, const int32_t* H_orig, const int32_t* orig_of,
+                      const uint8_t* fixed_orig) {
+  if (!c || nO <= 0 || !V_orig || !H_orig || !orig_of || !fixed_orig) return HEXOPT_EINVAL;
+  const MeshLayout& L = c->L;
+  const int nV = L.nV, nH = L.nH;
+  if (nH == 0) return fail(c, HEXOPT_EINVAL, "mesh without hexes");
+  for (int v = 0; v < nV; ++v)
+    if (orig_of[v] < 0 || orig_of[v] >= nO) return fail(c, HEXOPT_EINVAL, "orig_of out of range");
+  for (long i = 0; i < 8L * nH; ++i)
+    if (H_orig[i] < 0 || H_orig[i] >= nO) return fail(c, HEXOPT_EINVAL, "H_orig out of range");
+  // hex h of the context must be hex h of the original mesh, corner by corner
+  for (int h = 0; h < nH; ++h)
+    for (int k = 0; k < 8; ++k)
+      if (orig_of[L.vperm[L.conn[(size_t)k * nH + h]]] != H_orig[(size_t)L.hperm[h] * 8 + k])
+        return fail(c, HEXOPT_EINVAL, "exploded hex corners do not copy the original hex's corners");
+  // groups: the copies of each original vertex, in the caller's order
+  std::vector<int> cnt(nO + 1, 0), g_of(nV), g_ids(nV);
+  std::vector<uint8_t> fx(nV);
+  for (int v = 0; v < nV; ++v) cnt[orig_of[v] + 1]++;
+  for (int o = 0; o < nO; ++o) cnt[o + 1] += cnt[o];
+  std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+  for (int v = 0; v < nV; ++v) {
+    const int o = orig_of[v];
+    g_ids[pos[o]++] = L.vinv[v];
+    g_of[L.vinv[v]] = o;
+    fx[L.vinv[v]] = fixed_orig[o] ? 1 : 0;
+  }
+  std::vector<double> vo((size_t)3 * nO);
+  for (int o = 0; o < nO; ++o)
+    for (int d = 0; d < 3; ++d) vo[(size_t)d * nO + o] = V_orig[(size_t)o * 3 + d];
+  std::vector<int> ho((size_t)8 * nH);
+  for (int h = 0; h < nH; ++h)
+    for (int k = 0; k < 8; ++k) ho[(size_t)k * nH + h] = H_orig[(size_t)L.hperm[h] * 8 + k];
+  int rc = HEXOPT_OK;
+  rc |= upload(c, c->d_g_of, g_of);
+  rc |= upload(c, c->d_g_off, cnt);
+  rc |= upload(c, c->d_g_ids, g_ids);
+  rc |= upload(c, c->d_gfixed, fx);
+  rc |= upload(c, c->d_Vo, vo);
+  rc |= upload(c, c->d_Ho, ho);
+  if (rc != HEXOPT_OK) return HEXOPT_EHIP;
+  HIPCHK(c, c->d_gsum.ensure((size_t)3 * nO * sizeof(double) + 8));
+  HIPCHK(c, c->d_gmean.ensure((size_t)3 * nO * sizeof(double) + 8));
+  HIPCHK(c, c->d_xfix.ensure((size_t)3 * nV * sizeof(double) + 8));
+  // glueInternalNodes' reference (optimization.cpp:578-582): compute_referenceMesh of the
+  // original mesh with cube = false and every H_flag set; grad_tet_ref scales it by the
+  // corner-tet volumes of the exploded start positions (REST)
+  HIPCHK(c, c->d_B.ensure((size_t)72 * nH * sizeof(double) + 8));
+  launch_ref_general(c->d_Vo.as<double>(), nO, c->d_Ho.as<int>(), nH, nullptr, 0, c->d_B.as<double>(), c->stream,
+                     c->d_rest.as<double>(), nV, c->tm.conn);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ref_mode = REF_GENERAL;
+  c->glue = true;
+  c->nO = nO;
+  c->lamG = 1e4;
+  return HEXOPT_OK;
+}
+
+int hexopt_glue_solve(hexopt_ctx* c, const hexopt_params* prm, int max_iters, hexopt_glue_log* log, int* iters_out,
+                      double* V_out, double* min_sj) {
+  if (!c || !prm || max_iters < 0) return HEXOPT_EINVAL;
+  if (!c->glue) return fail(c, HEXOPT_EINVAL, "hexopt_glue_setup has not been called");
+  const int nH = c->L.nH, nO = c->nO;
+  // slim_solve2's gluing branch (slim_m.cpp:1191-1268): lamda_glue from its lower bound
+  const double LMIN = 1e+4, LMAX = 1e+16;
+  c->lamG = LMIN;
+  SoftDev sd = soft_with(c, prm);
+  double mj = -1, aj = 0;
+  int it = 0;
+  int rc = HEXOPT_OK;
+  while (it < max_iters) {
+    hexopt_iter_log lg{};
+    double E = 0, Eq = 0, Es = 0;
+    const double lam = c->lamG;
+    if ((rc = hexopt_slim_solve(c, prm, 1, &lg, &E, &Eq, &Es, nullptr)) != HEXOPT_OK) return rc;
+    ++it;
+    // the glued mesh: each original vertex at the mean of its copies
+    launch_group_mean(sd, c->L.nV, c->d_cur.as<double>(), c->d_gmean.as<double>(), c->stream);
+    c->lamG = std::min(LMIN * std::max(Eq / Es * c->lamG, 1.0), LMAX);
+    double* out = c->d_scal.as<double>();
+    launch_scaled_jacobian(c->d_gmean.as<double>(), nO, c->d_Ho.as<int>(), nH, nullptr, c->d_part[0].as<double>(),
+                           c->d_part[1].as<double>(), out, c->stream);
+    double h[2];
+    HIPCHK(c, hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    mj = h[0];
+    aj = h[1] / nH;
+    if (log) {
+      hexopt_glue_log& g = log[it - 1];
+      g.energy = E; g.e_quality = Eq; g.e_soft = Es; g.lamda_glue = lam; g.min_sj = mj; g.avg_sj = aj;
+      g.step = lg.step; g.cg_iters = lg.cg_iters;
+    }
+    if (mj > 0) break;
+  }
+  if (iters_out) *iters_out = it;
+  if (min_sj) *min_sj = mj;
+  if (V_out) {
+    if (it == 0) launch_group_mean(sd, c->L.nV, c->d_cur.as<double>(), c->d_gmean.as<double>(), c->stream);
+    std::vector<double> m((size_t)3 * nO);
+    HIPCHK(c, hipMemcpyAsync(m.data(), c->d_gmean.p, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int o = 0; o < nO; ++o)
+      for (int d = 0; d < 3; ++d) V_out[(size_t)o * 3 + d] = m[(size_t)d * nO + o];
+  }
+  return HEXOPT_OK;
+}
+
 int hexopt_debug_system(hexopt_ctx* c, const hexopt_params* prm, const double* p_in, double* rhs_out,
                         double* invd_out, double* Lp_out) {
   if (!c || !prm || !p_in) return HEXOPT_EINVAL;
@@ -579,6 +724,7 @@ int hexopt_debug_system(hexopt_ctx* c, const hexopt_params* prm, const double* p
     for (int d = 0; d < 3; ++d) h[(size_t)d * nV + i] = p_in[(size_t)d * nV + c->L.vperm[i]];
   for (int i = 0; i < nL; ++i) h[3 * (size_t)nV + i] = p_in[3 * (size_t)nV + i];
   HIPCHK(c, hipMemcpyAsync(c->d_p.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+  if (c->glue) launch_group_sum(sd, nV, c->d_p.as<double>(), c->d_gsum.as<double>(), st);
   launch_apply(c->tm, refop(c), c->d_K.as<double>(), c->d_p.as<double>(), c->d_tmp.as<double>(), prox, sd,
                c->d_halo.as<double>(), c->d_part[2].as<double>(), c->d_part[3].as<double>(), c->d_part[4].as<double>(),
                st);
@@ -652,6 +798,7 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
   if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");
   if (prm->energy != HEXOPT_SYMMETRIC_DIRICHLET)
     return fail(c, HEXOPT_EUNSUPPORTED, "the FP32 adjoint optimiser supports SYMMETRIC_DIRICHLET");
+  if (c->glue) return fail(c, HEXOPT_EUNSUPPORTED, "the FP32 adjoint optimiser does not run the glue");
   if (prm->projection && c->sd.nb > 0) return fail(c, HEXOPT_EUNSUPPORTED, "projection constraints");
   if (opt->cg_iters < 1 || opt->refine_iters < 0 || opt->residual_every < 0)
     return fail(c, HEXOPT_EINVAL, "bad iteration counts");

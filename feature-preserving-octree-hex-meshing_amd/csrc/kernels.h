@@ -49,12 +49,36 @@ struct SoftDev {
   const int* t_ids; const double* t_n; const double* t_d;
   const int* lv; const double* lt; const double* lo;
   double lamP, lamC, lamT, lamL;
+  // padding glue (exploded mesh): groups of copies of one original vertex, CSR g_off/g_ids
+  // (copies in the caller's order), g_of[v] the group of vertex v; g_sum holds 3 sums per
+  // group of the vector being applied; rows of vertices with fixed[v] are left out of solves
+  int nG;
+  const int* g_of;
+  const int* g_off;
+  const int* g_ids;
+  const double* g_sum;
+  double lamG;
+  const uint8_t* fixed;
 };
 
 void launch_ref_cube(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube, double* invs,
                      int* n_general, hipStream_t st);
 void launch_ref_general(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube, double* B,
-                        hipStream_t st);
+                        hipStream_t st, const double* Xvol = nullptr, int nVvol = 0, const int* connvol = nullptr);
+// Glue (padding untangle): per-group sums of a SoA vector x (nV stride) into sd.g_sum layout
+// gs[3g + d]; the glue energy lamG sum_j |y_j - mean of the other copies|^2 at y = cur + t (dst
+// - cur) into out[0]; vertex means of the groups (SoA, nG stride); and the masked-solve helpers
+// (fixed rows: invd == 0).
+void launch_group_sum(const SoftDev& sd, int nV, const double* x, double* gs, hipStream_t st);
+void launch_glue_energy(const SoftDev& sd, int nV, const double* cur, const double* dst, double step, double* part,
+                        double* out, hipStream_t st);
+void launch_group_mean(const SoftDev& sd, int nV, const double* x, double* mean, hipStream_t st);
+// xfix = fixed rows of cur (others 0), x = free rows of cur (others 0)
+void launch_glue_split(int nV, const double* invd, const double* cur, double* xfix, double* x, hipStream_t st);
+// rhs = free rows of rhs - tmp (fixed rows 0)
+void launch_glue_rhs(int nV, const double* invd, double* rhs, const double* tmp, hipStream_t st);
+// x = cur on fixed rows
+void launch_glue_restore(int nV, const double* invd, const double* cur, double* x, hipStream_t st);
 void launch_local_step(const TileMesh& tm, const RefOp& ro, const double* X, double* K, int energy, double M,
                        double exp_f, double prox, const SoftDev& sd, double* rhs, double* invd, double* halo,
                        hipStream_t st);

@@ -453,13 +453,25 @@ __global__ __launch_bounds__(256) void ref_cube_kernel(const double* X, int nV, 
 }
 
 // Pass 2 (only when some hex needs it): the general operator B for every tet.
-__global__ __launch_bounds__(256) void ref_general_kernel(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube,
-                                   double* Bout) {
+// The reference shape comes from X (via conn); the corner-tet volumes that scale it come from
+// Xv (via connv) when given, else from X (grad_tet_ref takes them from the mesh being
+// optimised, which for the glue is the exploded mesh, not the one the shape comes from).
+__global__ __launch_bounds__(256) void ref_general_kernel(const double* X, int nV, const int* conn, int nH,
+                                                          const uint8_t* hflag, int cube, double* Bout,
+                                                          const double* Xv, int nVv, const int* connv) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nH) return;
   int vid[8];
-  double p[8][3];
+  double p[8][3], pv[8][3];
   gather8(X, nV, conn, nH, h, vid, p);
+  if (Xv) {
+    gather8(Xv, nVv, connv, nH, h, vid, pv);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) pv[i][d] = p[i][d];
+  }
   double r8[8][3];
   const bool hf = hflag == nullptr || hflag[h] != 0;
   if (!hf) {
@@ -494,7 +506,7 @@ __global__ __launch_bounds__(256) void ref_general_kernel(const double* X, int n
       pick_row<double, 3>(r8, kHexTet[c][j], r[j]);
     }
     double B[3][3];
-    general_B(x, r, corner_volume(p, c), B);
+    general_B(x, r, corner_volume(pv, c), B);
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) Bout[((size_t)c * 9 + i * 3 + j) * nH + h] = B[i][j];
   }
@@ -517,6 +529,20 @@ __device__ __forceinline__ void soft_apply(const SoftDev& sd, int g, const doubl
     for (int d = 0; d < 3; ++d) out[d] -= sd.lamL * sd.lt[(size_t)li * 3 + d] * a;
   }
   (void)nV;
+}
+
+// Glue rows of vertex g (add_soft_constraints' equality block, slim_m.cpp:611-645): with k
+// copies in its group and S their sum (of the vector being applied), the rows of
+// lamG A^T A are lamG k / (k - 1)^2 (k p_g - S); none for a vertex with no other copy.
+__device__ __forceinline__ void glue_rows(const SoftDev& sd, int g, const double pv[3], double r[3]) {
+  if (!sd.g_of) return;
+  const int gr = sd.g_of[g];
+  if (gr < 0) return;
+  const int k = sd.g_off[gr + 1] - sd.g_off[gr];
+  if (k < 2) return;
+  const double c = sd.lamG * (double)k / ((double)(k - 1) * (double)(k - 1));
+#pragma unroll
+  for (int d = 0; d < 3; ++d) r[d] += c * ((double)k * pv[d] - sd.g_sum[(size_t)gr * 3 + d]);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -596,10 +622,18 @@ struct LocalStepOp {
 #pragma unroll
       for (int d = 0; d < 3; ++d) { dg[d] += S[d]; r[d] += b[d]; }
     }
+    if (sd.g_of && sd.g_of[g] >= 0) {  // glue diagonal lamG k / (k - 1); its rhs is 0
+      const int k = sd.g_off[sd.g_of[g] + 1] - sd.g_off[sd.g_of[g]];
+      if (k > 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dg[d] += sd.lamG * (double)k / (double)(k - 1);
+    }
+    // a fixed row leaves the solve: invd 0 marks it for the CG kernels
+    const bool pinned = sd.fixed && sd.fixed[g];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       rhs[(size_t)d * nV + g] = r[d];
-      invd[(size_t)d * nV + g] = dg[d] != 0.0 ? 1.0 / dg[d] : 1.0;
+      invd[(size_t)d * nV + g] = pinned ? 0.0 : (dg[d] != 0.0 ? 1.0 / dg[d] : 1.0);
     }
     return 0.0;
   }
@@ -683,6 +717,7 @@ struct ApplyOp {
         for (int d = 0; d < 3; ++d) r[d] -= sd.lamL * sd.lt[(size_t)li * 3 + d] * a;
       }
     }
+    glue_rows(sd, g, pv, r);
 #pragma unroll
     for (int d = 0; d < 3; ++d) out[(size_t)d * nV + g] = (OT)r[d];
     return pv[0] * r[0] + pv[1] * r[1] + pv[2] * r[2];
@@ -841,12 +876,14 @@ struct CGState {
 };
 
 // r = rhs - tmp, partial |rhs|^2, |r|^2
+// (rows with invd == 0 are fixed and left out: r = 0 there)
 template <class T>
-__global__ void cg_init_kernel(int n, const T* rhs, const T* tmp, T* r, double* part_b, double* part_r) {
+__global__ void cg_init_kernel(int n, const T* rhs, const T* tmp, T* r, const T* invd, double* part_b,
+                               double* part_r) {
   __shared__ double red[16];
   double sb = 0, sr = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    T v = rhs[i] - tmp[i];
+    T v = invd[i] == T(0) ? T(0) : rhs[i] - tmp[i];
     r[i] = v;
     sb += (double)rhs[i] * (double)rhs[i];
     sr += (double)v * (double)v;
@@ -931,7 +968,7 @@ __global__ void cg_update_kernel(int n, const CGState* st, T* x, T* r, T* z, con
     const T alpha = (T)st->alpha;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
       x[i] += alpha * p[i];
-      T rv = r[i] - alpha * tmp[i];
+      T rv = invd[i] == T(0) ? T(0) : r[i] - alpha * tmp[i];  // fixed rows stay out
       r[i] = rv;
       T zv = invd[i] * rv;
       z[i] = zv;
@@ -1861,6 +1898,131 @@ __global__ __launch_bounds__(256) void scaled_jacobian_kernel(const double* X, i
 // ---------------------------------------------------------------------------------------
 // host launch wrappers (kernels.h)
 
+// ---------------------------------------------------------------------------------------
+// Glue (padding untangle). One thread per group of copies, summing in the caller's order.
+__global__ __launch_bounds__(256) void group_sum_kernel(SoftDev sd, int nV, const double* __restrict__ x,
+                                                        double* __restrict__ gs) {
+  const int gr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gr >= sd.nG) return;
+  double s[3] = {0, 0, 0};
+  for (int k = sd.g_off[gr]; k < sd.g_off[gr + 1]; ++k) {
+    const int v = sd.g_ids[k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s[d] += x[(size_t)d * nV + v];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) gs[(size_t)gr * 3 + d] = s[d];
+}
+
+__global__ __launch_bounds__(256) void group_mean_kernel(SoftDev sd, int nV, const double* __restrict__ x,
+                                                         double* __restrict__ mean) {
+  const int gr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gr >= sd.nG) return;
+  const int a = sd.g_off[gr], b = sd.g_off[gr + 1];
+  double s[3] = {0, 0, 0};
+  for (int k = a; k < b; ++k) {
+    const int v = sd.g_ids[k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s[d] += x[(size_t)d * nV + v];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) mean[(size_t)d * sd.nG + gr] = s[d] / (double)(b - a);
+}
+
+// compute_soft_const_energy's glue term (slim_m.cpp:748-766) at y = cur + t (dst - cur): for
+// each copy j, lamG |y_j - (sum of the other copies, in order) / (k - 1)|^2
+__global__ __launch_bounds__(256) void glue_energy_kernel(SoftDev sd, int nV, const double* __restrict__ cur,
+                                                          const double* __restrict__ dst, double step,
+                                                          double* part) {
+  __shared__ double red[16];
+  const int gr = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0;
+  if (gr < sd.nG) {
+    const int a = sd.g_off[gr], b = sd.g_off[gr + 1];
+    if (b - a > 1) {
+      const double coef = 1.0 / (double)(b - a - 1);
+      for (int j = a; j < b; ++j) {
+        const int vj = sd.g_ids[j];
+        double cv[3] = {0, 0, 0}, yj[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double c0 = cur[(size_t)d * nV + vj];
+          yj[d] = dst ? c0 + step * (dst[(size_t)d * nV + vj] - c0) : c0;
+        }
+        for (int k = a; k < b; ++k) {
+          if (k == j) continue;
+          const int vk = sd.g_ids[k];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double c0 = cur[(size_t)d * nV + vk];
+            cv[d] += dst ? c0 + step * (dst[(size_t)d * nV + vk] - c0) : c0;
+          }
+        }
+        double q = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double t = yj[d] - cv[d] * coef;
+          q += t * t;
+        }
+        e += sd.lamG * q;
+      }
+    }
+  }
+  const double s = block_sum(e, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void glue_split_kernel(int n, const double* invd, const double* cur, double* xfix, double* x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const bool f = invd[i] == 0.0;
+    xfix[i] = f ? cur[i] : 0.0;
+    x[i] = f ? 0.0 : cur[i];
+  }
+}
+
+__global__ void glue_rhs_kernel(int n, const double* invd, double* rhs, const double* tmp) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    rhs[i] = invd[i] == 0.0 ? 0.0 : rhs[i] - tmp[i];
+}
+
+__global__ void glue_restore_kernel(int n, const double* invd, const double* cur, double* x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (invd[i] == 0.0) x[i] = cur[i];
+}
+
+void launch_group_sum(const SoftDev& sd, int nV, const double* x, double* gs, hipStream_t st) {
+  if (sd.nG <= 0) return;
+  hipLaunchKernelGGL(group_sum_kernel, dim3((sd.nG + 255) / 256), dim3(256), 0, st, sd, nV, x, gs);
+}
+
+void launch_group_mean(const SoftDev& sd, int nV, const double* x, double* mean, hipStream_t st) {
+  if (sd.nG <= 0) return;
+  hipLaunchKernelGGL(group_mean_kernel, dim3((sd.nG + 255) / 256), dim3(256), 0, st, sd, nV, x, mean);
+}
+
+void launch_glue_energy(const SoftDev& sd, int nV, const double* cur, const double* dst, double step, double* part,
+                        double* out, hipStream_t st) {
+  const int nb = std::max(1, (sd.nG + 255) / 256);
+  hipLaunchKernelGGL(glue_energy_kernel, dim3(nb), dim3(256), 0, st, sd, nV, cur, dst, step, part);
+  hipLaunchKernelGGL(sum_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part, nb, out);
+}
+
+namespace {
+int flat_blocks(int n) { return std::max(1, std::min((n + 255) / 256, 4096)); }
+}  // namespace
+
+void launch_glue_split(int nV, const double* invd, const double* cur, double* xfix, double* x, hipStream_t st) {
+  hipLaunchKernelGGL(glue_split_kernel, dim3(flat_blocks(3 * nV)), dim3(256), 0, st, 3 * nV, invd, cur, xfix, x);
+}
+
+void launch_glue_rhs(int nV, const double* invd, double* rhs, const double* tmp, hipStream_t st) {
+  hipLaunchKernelGGL(glue_rhs_kernel, dim3(flat_blocks(3 * nV)), dim3(256), 0, st, 3 * nV, invd, rhs, tmp);
+}
+
+void launch_glue_restore(int nV, const double* invd, const double* cur, double* x, hipStream_t st) {
+  hipLaunchKernelGGL(glue_restore_kernel, dim3(flat_blocks(3 * nV)), dim3(256), 0, st, 3 * nV, invd, cur, x);
+}
+
 void launch_ref_cube(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube, double* invs,
                      int* n_general, hipStream_t st) {
   if (nH == 0) return;
@@ -1869,9 +2031,10 @@ void launch_ref_cube(const double* X, int nV, const int* conn, int nH, const uin
 }
 
 void launch_ref_general(const double* X, int nV, const int* conn, int nH, const uint8_t* hflag, int cube, double* B,
-                        hipStream_t st) {
+                        hipStream_t st, const double* Xvol, int nVvol, const int* connvol) {
   if (nH == 0) return;
-  hipLaunchKernelGGL(ref_general_kernel, dim3((nH + 255) / 256), dim3(256), 0, st, X, nV, conn, nH, hflag, cube, B);
+  hipLaunchKernelGGL(ref_general_kernel, dim3((nH + 255) / 256), dim3(256), 0, st, X, nV, conn, nH, hflag, cube, B,
+                     Xvol, nVvol, connvol);
 }
 
 void launch_local_step(const TileMesh& tm, const RefOp& ro, const double* X, double* K, int energy, double M,
@@ -1923,7 +2086,7 @@ template <class T>
 void cg_init_t(int n, const T* rhs, const T* tmp, T* r, double* pb, double* pr, int nb, void* state, const T* invd, T* p,
                double* pabs, double tol, hipStream_t st) {
   CGState* S = (CGState*)state;
-  hipLaunchKernelGGL(cg_init_kernel<T>, dim3(nb), dim3(256), 0, st, n, rhs, tmp, r, pb, pr);
+  hipLaunchKernelGGL(cg_init_kernel<T>, dim3(nb), dim3(256), 0, st, n, rhs, tmp, r, invd, pb, pr);
   hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)pb, nb, (const double*)pr, tol);
   hipLaunchKernelGGL(cg_first_dir_kernel<T>, dim3(nb), dim3(256), 0, st, n, (const CGState*)S, invd, (const T*)r, p, pabs);
   hipLaunchKernelGGL(cg_absnew_fin_kernel, dim3(1), dim3(1024), 0, st, S, (const double*)pabs, nb);

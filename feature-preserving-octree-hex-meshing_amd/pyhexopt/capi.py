@@ -22,7 +22,7 @@ EXPORTS = ["hexopt_create", "hexopt_create_error", "hexopt_destroy", "hexopt_las
            "hexopt_compute_surface_normals", "hexopt_surface_closest", "hexopt_project_planes",
            "hexopt_occ_classify_cells", "hexopt_occ_voxel_parity", "hexopt_occ_cell_parity", "hexopt_hex_inside",
            "hexopt_curves_create", "hexopt_curves_error", "hexopt_curves_destroy", "hexopt_project_lines",
-           "hexopt_refresh_lines", "hexopt_metro_hausdorff"]
+           "hexopt_refresh_lines", "hexopt_metro_hausdorff", "hexopt_glue_setup", "hexopt_glue_solve"]
 
 _LIB = None
 
@@ -44,6 +44,12 @@ class Constraints(C.Structure):
 class IterLog(C.Structure):
     _fields_ = [("energy", C.c_double), ("e_quality", C.c_double), ("e_soft", C.c_double),
                 ("step_bound", C.c_double), ("step", C.c_double), ("cg_iters", C.c_int), ("cg_error", C.c_double)]
+
+
+class GlueLog(C.Structure):
+    _fields_ = [("energy", C.c_double), ("e_quality", C.c_double), ("e_soft", C.c_double),
+                ("lamda_glue", C.c_double), ("min_sj", C.c_double), ("avg_sj", C.c_double), ("step", C.c_double),
+                ("cg_iters", C.c_int)]
 
 
 class Opt32Options(C.Structure):
@@ -77,7 +83,7 @@ def lib():
                      "hexopt_surface_normals", "hexopt_compute_surface_normals", "hexopt_surface_closest",
                      "hexopt_project_planes", "hexopt_occ_classify_cells", "hexopt_occ_voxel_parity",
                      "hexopt_occ_cell_parity", "hexopt_hex_inside", "hexopt_project_lines",
-                     "hexopt_refresh_lines", "hexopt_metro_hausdorff"]:
+                     "hexopt_refresh_lines", "hexopt_metro_hausdorff", "hexopt_glue_setup", "hexopt_glue_solve"]:
             getattr(L, name).restype = C.c_int
         L.hexopt_curves_create.restype = C.c_void_p
         L.hexopt_curves_create.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
@@ -184,6 +190,29 @@ class HexOpt:
         log = np.array([[l.energy, l.e_quality, l.e_soft, l.step_bound, l.step, l.cg_iters, l.cg_error]
                         for l in logs[:iters]]).reshape(iters, 7)
         return E.value, Eq.value, Es.value, log, aL[:getattr(self, "nL", 0)]
+
+    def glue_setup(self, V_orig, H_orig, orig_of, fixed_orig):
+        """Exploded-mesh glue (include/hexopt.h): this context holds the exploded mesh."""
+        V_orig, H_orig = _d(V_orig), _i(H_orig)
+        orig_of = _i(orig_of)
+        fx = np.ascontiguousarray(fixed_orig, dtype=np.uint8)
+        self.nO = V_orig.shape[0]
+        self._chk(lib().hexopt_glue_setup(C.c_void_p(self.h), self.nO, _ptr(V_orig), _ptr(H_orig, C.c_int32),
+                                          _ptr(orig_of, C.c_int32), _ptr(fx, C.c_uint8)), "glue_setup")
+
+    def glue_solve(self, max_iters=50, energy=CONFORMAL, weight_opt=1.0):
+        """Returns (glued V, log rows [energy, e_quality, e_soft, lamda_glue, min_sj, avg_sj, step,
+        cg_iters], iterations, min scaled Jacobian)."""
+        prm = self.params(energy, weight_opt)
+        logs = (GlueLog * max(max_iters, 1))()
+        it, mj = C.c_int(), C.c_double()
+        Vg = np.zeros((self.nO, 3))
+        self._chk(lib().hexopt_glue_solve(C.c_void_p(self.h), C.byref(prm), max_iters, logs, C.byref(it), _ptr(Vg),
+                                          C.byref(mj)), "glue_solve")
+        n = it.value
+        log = np.array([[l.energy, l.e_quality, l.e_soft, l.lamda_glue, l.min_sj, l.avg_sj, l.step, l.cg_iters]
+                        for l in logs[:n]]).reshape(n, 8)
+        return Vg, log, n, mj.value
 
     def energy(self, energy=SYMMETRIC_DIRICHLET, weight_opt=1.0, aL=None):
         prm = self.params(energy, weight_opt)

@@ -84,6 +84,39 @@ def rotate_hex_corners(H, frac, seed=0):
     return out
 
 
+def exploded_start(V, H, fixed, target, amax, seed=0):
+    """Start positions of a padding-untangle glue (one copy of each corner per hex, (nH, 8, 3)),
+    in the manner of optimization::rigidDeformCubes (optimization.cpp:457-497): a hex with a
+    pinned vertex starts at its target corners; any other hex starts as the hex's shape in V,
+    turned by a seeded random rotation of up to amax radians and scaled by up to 20% about its
+    centre, placed at the centre of its target corners."""
+    rng = np.random.default_rng(seed)
+    out = np.asarray(V, dtype=np.float64)[H].copy()
+    T = np.asarray(target, dtype=np.float64)
+    hb = np.asarray(fixed, dtype=bool)[H].any(axis=1)
+    for h in range(H.shape[0]):
+        if hb[h]:
+            out[h] = T[H[h]]
+            continue
+        ax = rng.normal(size=3)
+        ax /= np.linalg.norm(ax)
+        K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+        a = rng.uniform(-amax, amax)
+        R = np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * K @ K
+        c = out[h].mean(0)
+        out[h] = (out[h] - c) @ R.T * rng.uniform(0.8, 1.2) + T[H[h]].mean(0)
+    return out
+
+
+def exploded_mesh(H, start):
+    """Exploded mesh of hexes H with per-hex corner copies `start` (nH, 8, 3): vertices
+    8h + i, hexes [8h .. 8h+7], and orig_of[8h + i] = H[h, i]."""
+    nH = H.shape[0]
+    Ve = np.asarray(start, dtype=np.float64).reshape(nH * 8, 3)
+    He = np.arange(nH * 8, dtype=np.int32).reshape(nH, 8)
+    return Ve, He, np.asarray(H, dtype=np.int32).reshape(-1).copy()
+
+
 def icosphere(level):
     """Unit icosphere with 20*4^level triangles (outward orientation)."""
     p = (1.0 + 5 ** 0.5) / 2.0

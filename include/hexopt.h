@@ -96,6 +96,36 @@ int hexopt_set_constraints(hexopt_ctx* ctx, const hexopt_constraints* cs);
 int hexopt_slim_solve(hexopt_ctx* ctx, const hexopt_params* prm, int iters, hexopt_iter_log* log, double* E,
                       double* Eq, double* Es, double* aL);
 
+/* Padding untangle by gluing an exploded mesh (optimization::glueInternalNodes,
+ * optimization.cpp:552-656). ctx holds the exploded mesh: hex h is hex h of the original mesh
+ * (nO vertices, hexes H_orig, nH x 8) with its own copy of each corner; orig_of[v] (caller
+ * numbering, nV entries) is the original vertex that vertex v copies, and corner k of context
+ * hex h must copy corner k of original hex h. The copies of an original vertex flagged in
+ * fixed_orig are pinned where ctx has them (known_value_post, slim_m.cpp:403-475); the others
+ * are glued by lamda_glue |x_j - mean of the other copies|^2 (add_soft_constraints,
+ * slim_m.cpp:611-645). Sets the reference operator: the original mesh's reference shapes
+ * (V_orig, compute_referenceMesh with cube = false) scaled to the volumes of ctx's REST
+ * positions (grad_tet_ref, grad.cpp:372-478). ctx must carry no other constraints. */
+int hexopt_glue_setup(hexopt_ctx* ctx, int nO, const double* V_orig, const int32_t* H_orig, const int32_t* orig_of,
+                      const uint8_t* fixed_orig);
+
+typedef struct {
+  double energy, e_quality, e_soft; /* after the iteration (the glue term is in e_soft) */
+  double lamda_glue;                /* the lamda_glue the iteration ran with */
+  double min_sj, avg_sj;            /* scaled Jacobian of the glued mesh after it */
+  double step;                      /* accepted line-search step */
+  int cg_iters;
+} hexopt_glue_log;
+
+/* slim_solve2's gluing branch (slim_m.cpp:1191-1268): SLIM iterations (energy from prm, the
+ * reference uses CONFORMAL) with lamda_glue starting at 1e4 and set after each iteration to
+ * min(1e4 max(E_quality / E_soft lamda_glue, 1), 1e16), until the glued mesh (each original
+ * vertex at the mean of its copies) has a positive minimum scaled Jacobian or max_iters
+ * (50 in the reference) iterations ran. log: nullable, max_iters entries. V_out: nullable,
+ * nO x 3, the glued mesh. iters, min_sj: nullable. */
+int hexopt_glue_solve(hexopt_ctx* ctx, const hexopt_params* prm, int max_iters, hexopt_glue_log* log, int* iters,
+                      double* V_out, double* min_sj);
+
 /* FP32 + adjoint optimiser (no single counterpart in the reference; it performs the same
  * SLIM iteration as hexopt_slim_solve in its adjoint form). Per iteration: one FP32 pass
  * over the elements gives the energy, -dE/dx / 2 (the hand-written adjoint), the SLIM
@@ -147,7 +177,9 @@ int hexopt_scaled_jacobian(hexopt_ctx* ctx, int which, double* min_j, double* av
 /* Diagnostic view of the global system of the next SLIM iteration at CURRENT (no reference
  * counterpart; used by the parity tests): runs the local step, then returns rhs, the inverse
  * Jacobi diagonal and L p_in. Vectors are flattened like the reference's unknowns
- * [x(0..nV) y(0..nV) z(0..nV) a_L(0..nL)] in the caller's numbering; outputs nullable. */
+ * [x(0..nV) y(0..nV) z(0..nV) a_L(0..nL)] in the caller's numbering; outputs nullable. On a
+ * glue context L carries the glue rows at the current lamda_glue (1e4 after hexopt_glue_setup)
+ * and inv_diag is 0 on pinned rows. */
 int hexopt_debug_system(hexopt_ctx* ctx, const hexopt_params* prm, const double* p_in, double* rhs, double* inv_diag,
                         double* Lp);
 

@@ -13,7 +13,12 @@
 //      lambda_T update.
 //   2  closest point + pseudonormal only (igl::AABB + normals as build_aabb_tree,
 //      grid_hex_meshing.cpp:4231-4248).
+//   4  the padding untangle: optimization::glueInternalNodes' set-up of the exploded mesh
+//      (optimization.cpp:552-628) and slim_solve2's gluing branch (slim_m.cpp:1191-1268),
+//      its loop written out here around the reference's own functions so that every
+//      iteration is logged, then checked against one slim_solve2(data, -1, 1) call.
 #include "slim_m.h"
+#include "igl/flip_avoiding_line_search.h"
 #include "igl/AABB.h"
 #include "igl/per_face_normals.h"
 #include "igl/per_vertex_normals.h"
@@ -26,6 +31,8 @@
 
 void update_weights_and_closest_rotations(SLIMData& s, const Eigen::MatrixXd& V, const Eigen::MatrixXi& F,
                                           Eigen::MatrixXd& uv);
+void solve_weighted_arap(SLIMData& s, const Eigen::MatrixXd& V, const Eigen::MatrixXi& F, Eigen::MatrixXd& uv,
+                         Eigen::VectorXi& soft_b_p, Eigen::MatrixXd& soft_bc_p);
 double compute_energy_total(SLIMData& s, Eigen::MatrixXd& V_new);
 void build_linear_system(SLIMData& s, Eigen::SparseMatrix<double>& L);
 
@@ -79,6 +86,194 @@ struct Surface {
 };
 }  // namespace
 
+// Input: nO, nH, energy, weight_opt, V (nO x 3, the glued target BV), H (nH x 8), fixed
+// (nO, the V_Boundary_flag), cubes (nH x 8 x 3, the exploded start positions).
+int glue_mode(In& in, Out& out) {
+  const int nO = in.get<int32_t>(), nH = in.get<int32_t>(), energy = in.get<int32_t>();
+  const double weight_opt = in.get<double>();
+  auto V = in.arr<double>((size_t)nO * 3);
+  auto H = in.arr<int32_t>((size_t)nH * 8);
+  auto fixed = in.arr<int32_t>(nO);
+  auto cubes = in.arr<double>((size_t)nH * 24);
+  Eigen::MatrixXd BV = rows3(V, nO);
+  Mesh mesh;
+  mesh.type = Mesh_type::Hex;
+  mesh.V = BV.transpose();
+  for (int h = 0; h < nH; ++h) {
+    Hybrid hy; hy.id = h;
+    for (int i = 0; i < 8; ++i) hy.vs.push_back(H[(size_t)h * 8 + i]);
+    mesh.Hs.push_back(hy);
+  }
+  std::vector<bool> V_Boundary_flag(nO);
+  for (int v = 0; v < nO; ++v) V_Boundary_flag[v] = fixed[v] != 0;
+  // glueInternalNodes (optimization.cpp:552-628)
+  Tetralize_Set ts;
+  std::vector<int> nhex(nO, 0);
+  for (int h = 0; h < nH; ++h) for (int i = 0; i < 8; ++i) nhex[H[(size_t)h * 8 + i]]++;
+  std::vector<uint32_t> V_ranges;
+  int32_t nv = 0;
+  for (int v = 0; v < nO; ++v) { nv += nhex[v]; V_ranges.push_back(nv - 1); }
+  ts.V_ranges = V_ranges;
+  ts.V.resize(nv, 3);
+  ts.T.resize(nH * 8, 4);
+  std::vector<int> copy_of((size_t)nH * 8);  // exploded vertex of (hex, corner)
+  Eigen::Vector4i t;
+  for (int h = 0; h < nH; ++h) {
+    const std::vector<uint32_t>& vs = mesh.Hs[h].vs;
+    for (int i = 0; i < 8; ++i) {
+      for (int j = 0; j < 4; ++j) t[j] = V_ranges[vs[hex_tetra_table[i][j]]];
+      ts.T.row(h * 8 + i) = t;
+    }
+    for (int i = 0; i < 8; ++i) {
+      for (int d = 0; d < 3; ++d) ts.V(V_ranges[vs[i]], d) = cubes[((size_t)h * 8 + i) * 3 + d];
+      copy_of[(size_t)h * 8 + i] = V_ranges[vs[i]];
+      V_ranges[vs[i]]--;
+    }
+  }
+  std::vector<bool> H_inout_tag(nH, true);
+  std::vector<uint32_t> Hids;
+  for (int h = 0; h < nH; ++h) Hids.push_back(h);
+  compute_referenceMesh(BV, mesh.Hs, H_inout_tag, Hids, ts.RT, false);
+  ts.mappedV.resize(ts.V.rows(), -1);
+  ts.mappedV_reverse.resize(ts.V.rows(), -1);
+  nv = 0;
+  for (uint32_t i = 0; i < ts.V_ranges.size(); i++) {
+    if (V_Boundary_flag[i]) continue;
+    int start = 0, end = ts.V_ranges[i];
+    if (i != 0) start = ts.V_ranges[i - 1] + 1;
+    for (int j = start; j < end + 1; j++) { ts.mappedV[j] = nv; ts.mappedV_reverse[nv++] = j; }
+  }
+  ts.known_value_post = true;
+  ts.post_index = nv;
+  ts.post_Variables.resize(ts.V.rows() - nv, 3);
+  for (uint32_t i = 0; i < ts.V_ranges.size(); i++) {
+    if (!V_Boundary_flag[i]) continue;
+    int start = 0, end = ts.V_ranges[i];
+    if (i != 0) start = ts.V_ranges[i - 1] + 1;
+    for (int j = start; j < end + 1; j++) {
+      ts.mappedV[j] = nv;
+      ts.mappedV_reverse[nv] = j;
+      ts.post_Variables.row(nv - ts.post_index) = BV.row(i);
+      nv++;
+    }
+  }
+  V_ranges = ts.V_ranges;
+  Eigen::MatrixXd tempV = ts.V;
+  nv = 0;
+  int id = 0;
+  for (uint32_t i = 0; i < ts.V_ranges.size(); i++) if (!V_Boundary_flag[i]) {
+    int start = 0, end = V_ranges[i];
+    if (i != 0) start = V_ranges[i - 1] + 1;
+    nv += end - start + 1;
+    ts.V_ranges[id++] = nv - 1;
+  }
+  for (uint32_t i = 0; i < ts.V_ranges.size(); i++) if (V_Boundary_flag[i]) {
+    int start = 0, end = V_ranges[i];
+    if (i != 0) start = V_ranges[i - 1] + 1;
+    nv += end - start + 1;
+    ts.V_ranges[id++] = nv - 1;
+  }
+  for (int i = 0; i < ts.V.rows(); i++) ts.V.row(i) = tempV.row(ts.mappedV_reverse[i]);
+  for (int i = 0; i < ts.T.rows(); i++) for (int j = 0; j < 4; j++) ts.T(i, j) = ts.mappedV[ts.T(i, j)];
+  ts.UV = ts.V;
+  ts.O_Vranges = V_ranges;
+  ts.global = false;
+  ts.projection = false;
+  ts.lamda_projection = 1e+3;
+  ts.glue = true;
+  ts.record_Sequence = true;
+  ts.known_value_post = true;
+  ts.lamda_glue = 1e4;
+  ts.m = mesh;
+  ts.energy_type = (SLIM_ENERGY)energy;
+  // optimization::slim_m_opt (optimization.cpp:883-900)
+  SLIMData data;
+  data.stop_threshold = 1.e-7;
+  data.soft_const_p = 1e1;
+  data.exp_factor = 5.0;
+  data.weight_opt = weight_opt;
+  slim_precompute(ts.V, ts.T, ts.UV, data, ts.energy_type, ts);
+  std::vector<double> e0 = {data.energy, data.engery_quality, data.engery_soft};
+  SLIMData whole = data;
+  // the first iteration's assembled system (weights at the start point, lamda_glue 1e4):
+  // L and rhs over all unknowns, the sliced right-hand side of the free rows and the CG guess
+  {
+    SLIMData d2 = data;
+    d2.ts.lamda_glue = 1e4;
+    Eigen::MatrixXd uv = d2.V_o;
+    update_weights_and_closest_rotations(d2, d2.V, d2.F, uv);
+    Eigen::SparseMatrix<double> Lm;
+    build_linear_system(d2, Lm);
+    std::vector<double> trip;
+    for (int k = 0; k < Lm.outerSize(); ++k)
+      for (Eigen::SparseMatrix<double>::InnerIterator it(Lm, k); it; ++it) {
+        trip.push_back(it.row()); trip.push_back(it.col()); trip.push_back(it.value());
+      }
+    out.putd("L_triplets", trip);
+    std::vector<double> rh(d2.rhs.data(), d2.rhs.data() + d2.rhs.size());
+    out.putd("rhs", rh);
+  }
+  {
+    std::vector<int> ri((size_t)nH * 8);
+    for (int k = 0; k < nH * 8; ++k) ri[k] = ts.mappedV[copy_of[k]];
+    out.puti("ref_index", ri);
+    std::vector<int> pi = {ts.post_index};
+    out.puti("post_index", pi);
+  }
+  // slim_solve2's gluing branch, one logged iteration at a time
+  const double LMIN = 1e+4, LMAX = 1e+16;
+  data.ts.lamda_glue = LMIN;
+  Mesh_Quality mq;
+  std::vector<double> log;
+  int iter = 0;
+  do {
+    Eigen::MatrixXd mV = data.ts.m.V.transpose();
+    compute_referenceMesh(mV, data.ts.m.Hs, H_inout_tag, Hids, data.ts.RT, false);
+    Eigen::MatrixXd dest_res;
+    data.ts.V = dest_res = data.V_o;
+    update_weights_and_closest_rotations(data, data.V, data.F, dest_res);
+    solve_weighted_arap(data, data.V, data.F, dest_res, data.b, data.bc);
+    std::function<double(Eigen::MatrixXd&)> ce = [&](Eigen::MatrixXd& a) { return compute_energy_total(data, a); };
+    const double lam = data.ts.lamda_glue;
+    data.energy = igl::flip_avoiding_line_search(data.F, data.V_o, dest_res, ce, data.energy * data.mesh_area) /
+                  data.mesh_area;
+    iter++;
+    data.ts.m.V.resize(3, data.ts.O_Vranges.size());
+    Eigen::MatrixXd tv = data.V_o;
+    for (int i = 0; i < data.ts.V.rows(); i++) tv.row(i) = data.V_o.row(data.ts.mappedV[i]);
+    for (uint32_t i = 0; i < data.ts.O_Vranges.size(); i++) {
+      int start = 0, end = data.ts.O_Vranges[i];
+      if (i != 0) start = data.ts.O_Vranges[i - 1] + 1;
+      Eigen::Vector3d v; v.setZero();
+      for (int j = start; j <= end; j++) v += tv.row(j);
+      v /= (end - start + 1);
+      data.ts.m.V.col(i) = v;
+    }
+    data.ts.lamda_glue = std::min(LMIN * std::max(data.engery_quality / data.engery_soft * data.ts.lamda_glue, 1.0), LMAX);
+    scaled_jacobian(data.ts.m, mq);
+    log.push_back(data.energy); log.push_back(data.engery_quality); log.push_back(data.engery_soft);
+    log.push_back(lam); log.push_back(mq.min_Jacobian);
+    if (mq.min_Jacobian > 0) break;
+  } while (iter < 50);
+  slim_solve2(whole, -1, 1, false);
+  double dmax = (whole.V_o - data.V_o).cwiseAbs().maxCoeff();
+  if (dmax != 0) { fprintf(stderr, "glue loop differs from slim_solve2 by %g\n", dmax); return 4; }
+  // exploded positions by (hex, corner), and the glued mesh
+  std::vector<double> UVx((size_t)nH * 24);
+  for (int k = 0; k < nH * 8; ++k)
+    for (int d = 0; d < 3; ++d) UVx[(size_t)k * 3 + d] = data.V_o(ts.mappedV[copy_of[k]], d);
+  std::vector<double> Vg((size_t)nO * 3);
+  for (int i = 0; i < nO; ++i) for (int d = 0; d < 3; ++d) Vg[(size_t)i * 3 + d] = data.ts.m.V(d, i);
+  out.putd("E0", e0);
+  out.putd("log", log);
+  out.putd("UVx", UVx);
+  out.putd("Vglued", Vg);
+  std::vector<double> q = {mq.min_Jacobian, mq.ave_Jacobian, (double)iter, data.ts.lamda_glue};
+  out.putd("SJ", q);
+  fclose(out.f);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc != 3) { fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]); return 2; }
   In in{fopen(argv[1], "rb")};
@@ -109,6 +304,8 @@ int main(int argc, char** argv) {
     fclose(out.f);
     return 0;
   }
+
+  if (mode == 4) return glue_mode(in, out);
 
   int nV = in.get<int32_t>(), nH = in.get<int32_t>();
   int energy = in.get<int32_t>(), n_outer = in.get<int32_t>(), n_inner = in.get<int32_t>();

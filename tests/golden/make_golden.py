@@ -211,10 +211,54 @@ def metro_cases():
     save("metro_torus_hexboundary", V0=SV, F0=SF, V1=HV, F1=HF, **{"ref_" + k: v for k, v in r.items()})
 
 
+def glue_cases():
+    """Padding untangle: glueInternalNodes' exploded-mesh glue and slim_solve2's gluing loop
+    (CONFORMAL, weight_opt 1). Start copies from meshgen.exploded_start."""
+    for name, n, sig, amax, seed in (("glue_grid4_untangle", 4, 0.3, 0.2, 5), ("glue_grid5_tangled", 5, 0.3, 0.3, 6)):
+        V, H = mg.grid_hex(n)
+        bm = mg.grid_boundary_mask(n)
+        Vt = mg.jitter(V, ~bm, sig * 2.0 / n, seed=seed)
+        start = mg.exploded_start(V, H, bm, Vt, amax, seed=seed)
+        r = refio.run_glue(Vt, H, bm.astype(np.int32), start, energy=3, weight_opt=1.0)
+        save(name, V=Vt, H=H, fixed=bm.astype(np.uint8), start=start, energy=3, weight_opt=1.0,
+             ref_cg_order_sensitivity=glue_order_sensitivity(r, start),
+             **{"ref_" + k: v for k, v in r.items()})
+
+
+def glue_system(r, start):
+    """The reference's first known_value_post system (slim_m.cpp:403-446) from its assembled
+    L and rhs: L over the free rows and columns, b = rhs - L[free, pinned] x_pinned, and the
+    CG guess, all in the reference's unknown order."""
+    nv = r["ref_index"].size
+    pidx = int(r["post_index"][0])
+    X = np.zeros((nv, 3))
+    X[r["ref_index"]] = start.reshape(-1, 3)
+    L = dense(r["L_triplets"], 3 * nv)
+    free = np.concatenate([d * nv + np.arange(pidx) for d in range(3)])
+    pinned = np.concatenate([d * nv + np.arange(pidx, nv) for d in range(3)])
+    xb = np.concatenate([X[pidx:, d] for d in range(3)])
+    b = r["rhs"][free] - L[np.ix_(free, pinned)] @ xb
+    x0 = np.concatenate([X[:pidx, d] for d in range(3)])
+    return L[np.ix_(free, free)], b, x0
+
+
+def glue_order_sensitivity(r, start):
+    """Relative change of the reference's first sliced 20-step CG solve when its system is
+    merely permuted (reversed unknown order)."""
+    A, b, x0 = glue_system(r, start)
+    perm = np.arange(b.size)[::-1]
+    xa = eigen_cg(A[np.ix_(perm, perm)], b[perm], x0[perm].copy())[np.argsort(perm)]
+    xb = eigen_cg(A, b, x0.copy())
+    return float(np.abs(xa - xb).max() / np.abs(xb).max())
+
+
 if __name__ == "__main__":
     import sys
     if sys.argv[1:] == ["metro"]:
         metro_cases()
+    elif sys.argv[1:] == ["glue"]:
+        glue_cases()
     else:
         main()
         metro_cases()
+        glue_cases()

@@ -70,6 +70,20 @@ def run_slim(mode, V, H, energy=2, n_outer=1, n_inner=1, cube=True, projection=F
         return read_out(pout)
 
 
+def run_glue(V, H, fixed, cubes, energy=1, weight_opt=1.0):
+    """Mode 4: glueInternalNodes' exploded-mesh set-up and slim_solve2's gluing loop.
+    cubes: (nH, 8, 3) exploded start positions (hex h, corner i)."""
+    with tempfile.TemporaryDirectory() as td:
+        pin, pout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(pin, "wb") as f:
+            _w(f, "i", 4)
+            _w(f, "iii", V.shape[0], H.shape[0], energy)
+            _w(f, "d", weight_opt)
+            _arr(f, V, "<f8"); _arr(f, H, "<i4"); _arr(f, fixed, "<i4"); _arr(f, cubes, "<f8")
+        subprocess.run([REF_BIN, pin, pout], check=True, stdout=subprocess.DEVNULL)
+        return read_out(pout)
+
+
 def run_aabb(V, F, P):
     with tempfile.TemporaryDirectory() as td:
         pin, pout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
