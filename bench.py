@@ -122,6 +122,34 @@ def cpu_baseline(target_small):
                       "on the %d-hex clipped torus (%.1f s), scaled by %d/1e6" % (H.shape[0], dt, H.shape[0])}
 
 
+def glue_measure(target, steps):
+    """The padding untangle (hexopt_glue_setup / hexopt_glue_solve's SLIM iteration, FP64,
+    CONFORMAL) on the exploded clipped-torus mesh: boundary vertices pinned, interior target
+    jittered by 0.3 edge lengths, start copies from meshgen.exploded_start. Fixed iteration
+    count; seconds per iteration and the FP64 element-operator kernel on the exploded mesh."""
+    from pyhexopt import capi, meshgen as mg
+    V, H, bnd = mg.torus_hex_mesh(target, R_MAJ, R_MIN, Z_CUT)
+    fixed = np.zeros(V.shape[0], dtype=bool)
+    fixed[bnd] = True
+    edge = np.linalg.norm(V[H[0, 1]] - V[H[0, 0]])
+    Vt = mg.jitter(V, ~fixed, 0.3 * edge, seed=1)
+    start = mg.exploded_start(V, H, fixed, Vt, 0.2, seed=1)
+    Ve, He, orig = mg.exploded_mesh(H, start)
+    g = capi.HexOpt(Ve, He)
+    g.glue_setup(Vt, H, orig, fixed)
+    g.slim_solve(1, energy=capi.CONFORMAL)
+    g.synchronize()
+    t0 = time.perf_counter()
+    E, Eq, Es, log, _ = g.slim_solve(steps, energy=capi.CONFORMAL)
+    g.synchronize()
+    secs = time.perf_counter() - t0
+    ms_apply = g.time_kernel(0, 20)
+    g.close()
+    return {"hexes": int(H.shape[0]), "exploded_vertices": int(Ve.shape[0]), "iterations": steps,
+            "ms_per_iteration": round(1e3 * secs / steps, 3), "energy": E,
+            "fp64_operator_kernel_us": round(1e3 * ms_apply, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +157,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--hexes", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-glue", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see the design notes on "
@@ -206,6 +235,8 @@ def main():
                      "bytes_per_hex": round(bytes_per_hex32, 2), "algorithmic_bytes": round(bytes_per_hex32 * nH),
                      "traffic_source": "profiles/cg32_apply_pipe_torus1m_hbm_traffic.md" if traffic else None},
     }
+    if not args.no_glue:
+        out["padding_untangle"] = glue_measure(max(args.hexes // 8, 20000), 5)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(max(args.hexes // 8, 20000))
     print(json.dumps(out))
