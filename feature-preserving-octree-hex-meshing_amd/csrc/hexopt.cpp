@@ -146,11 +146,14 @@ int ensure_state(hexopt_ctx* c) {
 
 SoftDev soft_with(hexopt_ctx* c, const hexopt_params* prm);
 
+// per-tet M: weight_opt unless the caller gives tet_mass (the igl::slim variant's weight_opt / 2)
+double tet_mass(const hexopt_params* prm) { return prm->tet_mass > 0 ? prm->tet_mass : prm->weight_opt; }
+
 int energy_now(hexopt_ctx* c, const hexopt_params* prm, const double* dst, double step, const double* aL, double* eq,
                double* es) {
   const int nsb = nblocks(c->sd.nb + c->sd.nC + c->sd.nT + c->sd.nL);
   double* out = c->d_scal.as<double>();
-  launch_energy(refop(c), c->d_cur.as<double>(), dst, step, c->L.nV, c->tm.conn, c->L.nH, prm->energy, prm->weight_opt,
+  launch_energy(refop(c), c->d_cur.as<double>(), dst, step, c->L.nV, c->tm.conn, c->L.nH, prm->energy, tet_mass(prm),
                 prm->exp_factor, c->sd, aL, c->d_part[0].as<double>(), c->d_part[1].as<double>(), nsb, out, c->stream);
   double h[3] = {0, 0, 0};
   if (c->glue)
@@ -475,7 +478,7 @@ int hexopt_slim_solve(hexopt_ctx* c, const hexopt_params* prm, int iters, hexopt
   if ((rc = ensure_state(c)) != HEXOPT_OK) return rc;
   const int nV = c->L.nV, nH = c->L.nH, nL = c->nL;
   const int n = 3 * nV + nL;
-  const double M = prm->weight_opt, prox = 0.00001 * prm->weight_opt, mesh_area = M * 8.0 * nH;
+  const double M = tet_mass(prm), prox = 0.00001 * prm->weight_opt, mesh_area = M * 8.0 * nH;
   const double exp_f = prm->exp_factor;
   hipStream_t st = c->stream;
   SoftDev sd = soft_with(c, prm);
@@ -707,7 +710,7 @@ int hexopt_debug_system(hexopt_ctx* c, const hexopt_params* prm, const double* p
   if (rc) return rc;
   const int nV = c->L.nV, nL = c->nL;
   const size_t n = (size_t)3 * nV + nL;
-  const double M = prm->weight_opt, prox = 0.00001 * prm->weight_opt;
+  const double M = tet_mass(prm), prox = 0.00001 * prm->weight_opt;
   hipStream_t st = c->stream;
   SoftDev sd = soft_with(c, prm);
   launch_local_step(c->tm, refop(c), c->d_cur.as<double>(), c->d_K.as<double>(), prm->energy, M, prm->exp_factor, prox,
@@ -808,7 +811,7 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
   if ((rc = ensure_state32(c))) return rc;
   const int nV = c->L.nV, nH = c->L.nH, nL = c->nL;
   const int n = 3 * nV + nL;
-  const double M = prm->weight_opt, prox = 0.00001 * prm->weight_opt, mesh_area = M * 8.0 * nH;
+  const double M = tet_mass(prm), prox = 0.00001 * prm->weight_opt, mesh_area = M * 8.0 * nH;
   hipStream_t st = c->stream;
   SoftDev sd = soft_with(c, prm);
   RefOp ro = refop(c);
@@ -1469,7 +1472,7 @@ int hexopt_energy_grad(hexopt_ctx* c, const hexopt_params* prm, int precision, c
     else HIPCHK(c, hipMemsetAsync(d_aL, 0, c->nL * sizeof(double), c->stream));
   }
   SoftDev sd = soft_with(c, prm);
-  launch_grad(precision, c->tm, refop(c), c->d_X.p, c->d_G.p, prm->weight_opt, sd, d_aL, c->d_halo.p, c->stream);
+  launch_grad(precision, c->tm, refop(c), c->d_X.p, c->d_G.p, tet_mass(prm), sd, d_aL, c->d_halo.p, c->stream);
   HIPCHK(c, hipGetLastError());
   return adj_out(c, precision, G);
 }
@@ -1481,7 +1484,7 @@ int hexopt_hvp(hexopt_ctx* c, const hexopt_params* prm, int precision, const voi
   int rc = adj_common(c, precision, X, v);
   if (rc) return rc;
   SoftDev sd = soft_with(c, prm);
-  launch_hvp(precision, c->tm, refop(c), c->d_X.p, c->d_v.p, c->d_G.p, prm->weight_opt, sd, c->d_halo.p, c->stream);
+  launch_hvp(precision, c->tm, refop(c), c->d_X.p, c->d_v.p, c->d_G.p, tet_mass(prm), sd, c->d_halo.p, c->stream);
   HIPCHK(c, hipGetLastError());
   return adj_out(c, precision, Hv);
 }

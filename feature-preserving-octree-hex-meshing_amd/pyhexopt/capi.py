@@ -29,7 +29,7 @@ _LIB = None
 
 class Params(C.Structure):
     _fields_ = [("energy", C.c_int), ("weight_opt", C.c_double), ("exp_factor", C.c_double),
-                ("projection", C.c_int), ("lamda_projection", C.c_double)]
+                ("projection", C.c_int), ("lamda_projection", C.c_double), ("tet_mass", C.c_double)]
 
 
 class Constraints(C.Structure):
@@ -177,11 +177,12 @@ class HexOpt:
         self._chk(lib().hexopt_set_constraints(C.c_void_p(self.h), C.byref(cs)), "set_constraints")
 
     @staticmethod
-    def params(energy=SYMMETRIC_DIRICHLET, weight_opt=1.0, exp_factor=1.0, projection=False, lam_proj=1e5):
-        return Params(energy, weight_opt, exp_factor, int(projection), lam_proj)
+    def params(energy=SYMMETRIC_DIRICHLET, weight_opt=1.0, exp_factor=1.0, projection=False, lam_proj=1e5,
+               tet_mass=0.0):
+        return Params(energy, weight_opt, exp_factor, int(projection), lam_proj, tet_mass)
 
-    def slim_solve(self, iters, energy=SYMMETRIC_DIRICHLET, weight_opt=1.0):
-        prm = self.params(energy, weight_opt)
+    def slim_solve(self, iters, energy=SYMMETRIC_DIRICHLET, weight_opt=1.0, tet_mass=0.0):
+        prm = self.params(energy, weight_opt, tet_mass=tet_mass)
         logs = (IterLog * max(iters, 1))()
         E, Eq, Es = C.c_double(), C.c_double(), C.c_double()
         aL = np.zeros(max(getattr(self, "nL", 0), 1))

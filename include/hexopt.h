@@ -42,6 +42,10 @@ typedef struct {
   double exp_factor;   /* exponential energies; slim_precompute forces 1.0 (slim_m.cpp:1125) */
   int projection;      /* Tetralize_Set::projection: adds lamda_projection * |x_b - bc|^2 */
   double lamda_projection;
+  double tet_mass;     /* per-tet M of the energy and the operator; 0: weight_opt. The igl::slim
+                          variant that smooths the padded mesh (igl/slim.cpp:1241-1247, called by
+                          optimization::slim_opt_igl) uses weight_opt / 2 with the same proximal
+                          term 1e-5 * weight_opt */
 } hexopt_params;
 
 /* Soft constraints of Tetralize_Set / Feature_Constraints (global_types.h:583-650).

@@ -13,12 +13,17 @@
 //      lambda_T update.
 //   2  closest point + pseudonormal only (igl::AABB + normals as build_aabb_tree,
 //      grid_hex_meshing.cpp:4231-4248).
+//   5  the padding smoothing: grid_hex_meshing_bijective::smoothing (grid_hex_meshing.cpp:
+//      2979-3056) without feature types: per outer iteration compute_referenceMesh(cube) and
+//      optimization::slim_opt_igl (optimization.cpp:967-1003), i.e. the igl::slim variant
+//      (igl/slim.cpp:1190-1287), run one logged igl::slim_solve(data, 1) at a time.
 //   4  the padding untangle: optimization::glueInternalNodes' set-up of the exploded mesh
 //      (optimization.cpp:552-628) and slim_solve2's gluing branch (slim_m.cpp:1191-1268),
 //      its loop written out here around the reference's own functions so that every
 //      iteration is logged, then checked against one slim_solve2(data, -1, 1) call.
 #include "slim_m.h"
 #include "igl/flip_avoiding_line_search.h"
+#include "igl/slim.h"
 #include "igl/AABB.h"
 #include "igl/per_face_normals.h"
 #include "igl/per_vertex_normals.h"
@@ -274,6 +279,75 @@ int glue_mode(In& in, Out& out) {
   return 0;
 }
 
+// Input: nV, nH, outer, inner, weight_opt, V (nV x 3), H (nH x 8)
+int smooth_mode(In& in, Out& out) {
+  const int nV = in.get<int32_t>(), nH = in.get<int32_t>(), outer = in.get<int32_t>(), inner = in.get<int32_t>();
+  const double weight_opt = in.get<double>();
+  auto V = in.arr<double>((size_t)nV * 3);
+  auto H = in.arr<int32_t>((size_t)nH * 8);
+  Mesh m;
+  m.type = Mesh_type::Hex;
+  m.V.resize(3, nV);
+  for (int i = 0; i < nV; ++i) for (int d = 0; d < 3; ++d) m.V(d, i) = V[(size_t)i * 3 + d];
+  for (int h = 0; h < nH; ++h) {
+    Hybrid hy; hy.id = h;
+    for (int i = 0; i < 8; ++i) hy.vs.push_back(H[(size_t)h * 8 + i]);
+    m.Hs.push_back(hy);
+  }
+  std::vector<bool> H_inout_tag(nH, true);
+  std::vector<uint32_t> Hids;
+  for (int h = 0; h < nH; ++h) Hids.push_back(h);
+  Tetralize_Set ts;
+  ts.V = m.V.transpose();
+  ts.T.resize(nH * 8, 4);
+  for (int h = 0; h < nH; ++h)
+    for (int i = 0; i < 8; ++i)
+      for (int j = 0; j < 4; ++j) ts.T(h * 8 + i, j) = H[(size_t)h * 8 + hex_tetra_table[i][j]];
+  ts.energy_type = SYMMETRIC_DIRICHLET;
+  ts.UV = ts.V;
+  ts.projection = false;
+  ts.global = true;
+  ts.glue = false;
+  ts.lamda_glue = 0;
+  ts.lamda_region = 0;
+  ts.record_Sequence = false;
+  ts.b.resize(0); ts.bc.resize(0, 3);
+  ts.lamda_b = 0;
+  std::vector<double> log, e0;
+  for (int o = 0; o < outer; ++o) {
+    compute_referenceMesh(ts.V, m.Hs, H_inout_tag, Hids, ts.RT, true);
+    // optimization::slim_opt_igl with empty feature, region and group sets
+    igl::SLIMData sData;
+    sData.weight_opt = weight_opt;
+    sData.soft_const_p = ts.lamda_b;
+    sData.exp_factor = 5.0;
+    sData.lamda_C = sData.lamda_T = sData.lamda_L = 0;
+    sData.lamda_region = ts.lamda_region;
+    sData.lamda_glue = ts.lamda_glue;
+    Eigen::VectorXi ids_C, ids_L, ids_T, regionb;
+    Eigen::MatrixXd C(0, 3), Axa(0, 3), Org(0, 3), nT(0, 3), regionbc(0, 3);
+    Eigen::VectorXd dT;
+    igl::slim_precompute(ts.V, ts.T, ts.V, sData, igl::SLIMData::SYMMETRIC_DIRICHLET, ts.b, ts.bc, ids_C, C, ids_L,
+                         Axa, Org, ids_T, nT, dT, regionb, regionbc, false, ts.global, ts.RT);
+    e0.push_back(sData.energy);
+    for (int k = 0; k < inner; ++k) {
+      igl::slim_solve(sData, 1);
+      log.push_back(sData.energy);
+    }
+    ts.V = sData.V_o;
+    m.V = ts.V.transpose();
+  }
+  out.putd("E0", e0);
+  out.putd("log", log);
+  out.putd("UV", flat3(ts.V));
+  Mesh_Quality mq;
+  scaled_jacobian(m, mq);
+  std::vector<double> q = {mq.min_Jacobian, mq.ave_Jacobian};
+  out.putd("SJ", q);
+  fclose(out.f);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc != 3) { fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]); return 2; }
   In in{fopen(argv[1], "rb")};
@@ -306,6 +380,7 @@ int main(int argc, char** argv) {
   }
 
   if (mode == 4) return glue_mode(in, out);
+  if (mode == 5) return smooth_mode(in, out);
 
   int nV = in.get<int32_t>(), nH = in.get<int32_t>();
   int energy = in.get<int32_t>(), n_outer = in.get<int32_t>(), n_inner = in.get<int32_t>();

@@ -225,6 +225,22 @@ def glue_cases():
              **{"ref_" + k: v for k, v in r.items()})
 
 
+def smooth_cases():
+    """Padding smoothing through the igl::slim variant (optimization::slim_opt_igl): 3 outer
+    iterations of compute_referenceMesh(cube) + 5 SLIM iterations with M = weight_opt / 2."""
+    n, seed = 6, 11
+    V, H = mg.grid_hex(n)
+    bm = mg.grid_boundary_mask(n)
+    V = mg.jitter(V, ~bm, 0.2 * 2.0 / n, seed=seed)
+    V, _ = mg.invert_hexes(V, H, 0.01, seed=seed)
+    r = refio.run_smooth(V, H, 3, 5, 1.0)
+    # the reference's CG order sensitivity on the first system of the same iteration shape
+    # (slim_m's assembly at weight_opt / 2; the igl variant's proximal term is 1e-5 weight_opt)
+    sens = order_sensitivity(V, H, weight_opt=0.5)
+    save("smooth_grid6", V=V, H=H, outer=3, inner=5, weight_opt=1.0, ref_cg_order_sensitivity=sens,
+         **{"ref_" + k: v for k, v in r.items()})
+
+
 def glue_system(r, start):
     """The reference's first known_value_post system (slim_m.cpp:403-446) from its assembled
     L and rhs: L over the free rows and columns, b = rhs - L[free, pinned] x_pinned, and the
@@ -258,7 +274,10 @@ if __name__ == "__main__":
         metro_cases()
     elif sys.argv[1:] == ["glue"]:
         glue_cases()
+    elif sys.argv[1:] == ["smooth"]:
+        smooth_cases()
     else:
         main()
         metro_cases()
         glue_cases()
+        smooth_cases()

@@ -84,6 +84,19 @@ def run_glue(V, H, fixed, cubes, energy=1, weight_opt=1.0):
         return read_out(pout)
 
 
+def run_smooth(V, H, outer=3, inner=5, weight_opt=1.0):
+    """Mode 5: the padding smoothing through the igl::slim variant (slim_opt_igl)."""
+    with tempfile.TemporaryDirectory() as td:
+        pin, pout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(pin, "wb") as f:
+            _w(f, "i", 5)
+            _w(f, "iiii", V.shape[0], H.shape[0], outer, inner)
+            _w(f, "d", weight_opt)
+            _arr(f, V, "<f8"); _arr(f, H, "<i4")
+        subprocess.run([REF_BIN, pin, pout], check=True, stdout=subprocess.DEVNULL)
+        return read_out(pout)
+
+
 def run_aabb(V, F, P):
     with tempfile.TemporaryDirectory() as td:
         pin, pout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
