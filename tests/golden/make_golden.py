@@ -268,9 +268,62 @@ def glue_order_sensitivity(r, start):
     return float(np.abs(xa - xb).max() / np.abs(xb).max())
 
 
+def io_cases():
+    """io_files.npz: for every writer case its inputs and the bytes the reference's writer
+    wrote; for every reader case the file's bytes and what the reference's reader made of it."""
+    import tempfile
+    import iocases as ic
+    if not ic.available():
+        raise SystemExit("oracle/_ref/io_ref is not built (make -C oracle/ref)")
+    out = {}
+    files = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, c in ic.write_cases().items():
+            path = os.path.join(td, name)
+            ic.ref_write(c, path)
+            raw = open(path, "rb").read()
+            out["w__%s__fmt" % name] = np.array(c["fmt"])
+            out["w__%s__type" % name] = np.array(c["type"])
+            out["w__%s__V" % name] = c["V"]
+            if isinstance(c["cells"], tuple):
+                out["w__%s__off" % name], out["w__%s__idx" % name] = c["cells"]
+            else:
+                out["w__%s__cells" % name] = c["cells"]
+            if c.get("boundary") is not None:
+                out["w__%s__boundary" % name] = c["boundary"]
+            out["w__%s__ref" % name] = np.frombuffer(raw, np.uint8)
+            if c["fmt"] in ("vtk", "mesh", "obj"):
+                files[name] = (c["fmt"], raw)
+        for name, c in ic.fgraph_cases().items():
+            path = os.path.join(td, name)
+            ic.ref_write(c, path)
+            raw = open(path, "rb").read()
+            for key in ("angle", "oc", "ocs", "corners", "pairs", "other"):
+                out["g__%s__%s" % (name, key)] = np.asarray(c[key])
+            out["g__%s__ref" % name] = np.frombuffer(raw, np.uint8)
+            files[name] = ("fgraph", raw)
+        files.update(ic.handwritten_files())
+        # the reference's VTK reader takes cells of 4, 5, 6 or 8 vertices only, its MESH reader
+        # hexahedra or tetrahedra only: other files are not read cases
+        skip = {"tri_vtk", "hyb_vtk", "tri_mesh", "hsur_mesh", "qua_mesh", "tet_mesh", "hyb_obj"}
+        for name, (fmt, raw) in files.items():
+            if name in skip:
+                continue
+            path = os.path.join(td, "r_" + name)
+            open(path, "wb").write(raw)
+            r = ic.ref_read(fmt, path)
+            out["r__%s__fmt" % name] = np.array(fmt)
+            out["r__%s__file" % name] = np.frombuffer(raw, np.uint8)
+            for key, val in r.items():
+                out["r__%s__%s" % (name, key)] = np.asarray(val)
+    save("io_files", **out)
+
+
 if __name__ == "__main__":
     import sys
-    if sys.argv[1:] == ["metro"]:
+    if sys.argv[1:] == ["io"]:
+        io_cases()
+    elif sys.argv[1:] == ["metro"]:
         metro_cases()
     elif sys.argv[1:] == ["glue"]:
         glue_cases()
@@ -281,3 +334,4 @@ if __name__ == "__main__":
         metro_cases()
         glue_cases()
         smooth_cases()
+        io_cases()

@@ -9,10 +9,10 @@ from pyhexopt import capi
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared(header):
+def declared(header, prefix="hexopt"):
     src = open(os.path.join(REPO, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(hexopt_[a-z_0-9]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(%s_[a-z_0-9]+)\s*\(" % prefix, src)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -22,6 +22,20 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert sorted(names) == sorted(capi.EXPORTS)
+
+
+def test_library_exports_every_file_format_symbol():
+    from pyhexopt import meshio
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    names = declared("hexio.h", "hexio")
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(names) == sorted(meshio.EXPORTS)
+
+
+def test_every_header_is_checked():
+    assert sorted(os.listdir(os.path.join(REPO, "include"))) == ["hexio.h", "hexopt.h"]
 
 
 def test_create_fails_cleanly_on_bad_mesh():
