@@ -22,6 +22,7 @@
 #include "bvh.h"
 #include "occupancy.h"
 #include "lines.h"
+#include "octree.h"
 
 namespace hexopt {
 namespace {
@@ -1228,6 +1229,71 @@ int hexopt_hex_inside(hexopt_ctx* c, hexopt_surface* s, int which, double* signe
     if (signed_dis) signed_dis[h] = S[i];
     if (H_flag) H_flag[h] = S[i] < 0 ? 1 : 0;
   }
+  return HEXOPT_OK;
+}
+
+struct hexopt_octree {
+  OctreeOut r;
+};
+
+namespace {
+thread_local std::string g_octree_error;
+}
+
+const char* hexopt_octree_error(void) { return g_octree_error.c_str(); }
+
+hexopt_octree* hexopt_octree_build(hexopt_surface* s, const int32_t* grid3, const double* origin, double voxel_size,
+                                   const double* mesh_transform, int stop_extent, int graded, int paired) {
+  g_octree_error.clear();
+  if (!s || !grid3 || !origin) {
+    g_octree_error = "null argument";
+    return nullptr;
+  }
+  if (hipSetDevice(s->device) != hipSuccess) {
+    g_octree_error = "hipSetDevice failed";
+    return nullptr;
+  }
+  double base[3];
+  for (int d = 0; d < 3; ++d) base[d] = (mesh_transform ? mesh_transform[d] : 0.0) + origin[d];
+  const int grid[3] = {grid3[0], grid3[1], grid3[2]};
+  auto o = new (std::nothrow) hexopt_octree;
+  if (!o) {
+    g_octree_error = "out of host memory";
+    return nullptr;
+  }
+  std::string err;
+  bool ok = false;
+  try {
+    ok = build_octree(s->view, grid, base, voxel_size, stop_extent, graded != 0, paired != 0, s->stream, o->r, err);
+  } catch (const std::exception& e) {
+    err = std::string("octree: ") + e.what();
+  }
+  if (!ok) {
+    g_octree_error = err;
+    delete o;
+    return nullptr;
+  }
+  return o;
+}
+
+void hexopt_octree_destroy(hexopt_octree* o) { delete o; }
+
+int hexopt_octree_sizes(const hexopt_octree* o, int64_t* out4, double* ms) {
+  if (!o || !out4) return HEXOPT_EINVAL;
+  out4[0] = (int64_t)(o->r.leaves.size() / 4);
+  out4[1] = (int64_t)(o->r.nodes.size() / 3);
+  out4[2] = o->r.levels;
+  out4[3] = o->r.evaluated;
+  if (ms) *ms = o->r.ms;
+  return HEXOPT_OK;
+}
+
+int hexopt_octree_copy(const hexopt_octree* o, int32_t* leaves, int32_t* nodes, double* V, int32_t* H) {
+  if (!o) return HEXOPT_EINVAL;
+  if (leaves) std::copy(o->r.leaves.begin(), o->r.leaves.end(), leaves);
+  if (nodes) std::copy(o->r.nodes.begin(), o->r.nodes.end(), nodes);
+  if (V) std::copy(o->r.V.begin(), o->r.V.end(), V);
+  if (H) std::copy(o->r.hexes.begin(), o->r.hexes.end(), H);
   return HEXOPT_OK;
 }
 

@@ -22,7 +22,9 @@ EXPORTS = ["hexopt_create", "hexopt_create_error", "hexopt_destroy", "hexopt_las
            "hexopt_compute_surface_normals", "hexopt_surface_closest", "hexopt_project_planes",
            "hexopt_occ_classify_cells", "hexopt_occ_voxel_parity", "hexopt_occ_cell_parity", "hexopt_hex_inside",
            "hexopt_curves_create", "hexopt_curves_error", "hexopt_curves_destroy", "hexopt_project_lines",
-           "hexopt_refresh_lines", "hexopt_metro_hausdorff", "hexopt_glue_setup", "hexopt_glue_solve"]
+           "hexopt_refresh_lines", "hexopt_metro_hausdorff", "hexopt_glue_setup", "hexopt_glue_solve",
+           "hexopt_octree_build", "hexopt_octree_error", "hexopt_octree_destroy", "hexopt_octree_sizes",
+           "hexopt_octree_copy"]
 
 _LIB = None
 
@@ -85,6 +87,16 @@ def lib():
                      "hexopt_occ_cell_parity", "hexopt_hex_inside", "hexopt_project_lines",
                      "hexopt_refresh_lines", "hexopt_metro_hausdorff", "hexopt_glue_setup", "hexopt_glue_solve"]:
             getattr(L, name).restype = C.c_int
+        L.hexopt_octree_build.restype = C.c_void_p
+        L.hexopt_octree_build.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_int,
+                                          C.c_int, C.c_int]
+        L.hexopt_octree_error.restype = C.c_char_p
+        L.hexopt_octree_destroy.argtypes = [C.c_void_p]
+        L.hexopt_octree_destroy.restype = None
+        L.hexopt_octree_sizes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hexopt_octree_sizes.restype = C.c_int
+        L.hexopt_octree_copy.argtypes = [C.c_void_p] * 5
+        L.hexopt_octree_copy.restype = C.c_int
         L.hexopt_curves_create.restype = C.c_void_p
         L.hexopt_curves_create.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.hexopt_curves_error.restype = C.c_char_p
@@ -384,6 +396,31 @@ class Surface:
                                                     _ptr(_d(origin)), C.c_double(voxel_size), _ptr(mt),
                                                     int(stop_extent), _ptr(out, C.c_uint8)), "occ_classify_cells")
         return out
+
+    def octree(self, grid, origin, voxel_size, mesh_transform=None, stop_extent=1, graded=True, paired=True):
+        """OctreeGrid(grid) + subdivide(should_subdivide, graded, paired) + octree_mesh's export:
+        dict(leaves (n, 4), nodes (m, 3) integer positions, V (m, 3), H (n, 8), levels,
+        evaluated, ms)."""
+        g = np.ascontiguousarray(grid, dtype=np.int32).reshape(3)
+        mt = None if mesh_transform is None else _d(mesh_transform)
+        o = lib().hexopt_octree_build(C.c_void_p(self.h), g.ctypes.data, _d(origin).ctypes.data, float(voxel_size),
+                                      None if mt is None else mt.ctypes.data, int(stop_extent), int(bool(graded)),
+                                      int(bool(paired)))
+        if not o:
+            raise RuntimeError("hexopt_octree_build: " + lib().hexopt_octree_error().decode())
+        try:
+            sz = (C.c_int64 * 4)()
+            ms = C.c_double()
+            lib().hexopt_octree_sizes(o, sz, C.byref(ms))
+            n, m = int(sz[0]), int(sz[1])
+            leaves = np.empty((n, 4), np.int32)
+            nodes = np.empty((m, 3), np.int32)
+            V = np.empty((m, 3), np.float64)
+            H = np.empty((n, 8), np.int32)
+            lib().hexopt_octree_copy(o, leaves.ctypes.data, nodes.ctypes.data, V.ctypes.data, H.ctypes.data)
+        finally:
+            lib().hexopt_octree_destroy(o)
+        return dict(leaves=leaves, nodes=nodes, V=V, H=H, levels=int(sz[2]), evaluated=int(sz[3]), ms=ms.value)
 
     def voxel_parity(self, nx, ny, nz, origin, spacing):
         """Inside flags of voxel centres, shaped (nz, ny, nx)."""

@@ -248,6 +248,31 @@ int hexopt_occ_cell_parity(hexopt_surface* s, int n, const int32_t* cells, const
  * (nH bytes, nullable), in the caller's hex order. */
 int hexopt_hex_inside(hexopt_ctx* ctx, hexopt_surface* s, int which, double* signed_dis, uint8_t* H_flag);
 
+/* ---- octree subdivision ----------------------------------------------------------------
+ * Replaces, in octree_mesh (grid_hex_meshing.cpp:490-562), OctreeGrid(grid) (octree.cpp:17-117)
+ * followed by OctreeGrid::subdivide(should_subdivide, graded, paired) (octree.cpp:648-690,
+ * splitCell :501-593, makeCellGraded :598-627, pairing :569-589) and the leaf-to-hex export.
+ * grid: cells per axis in voxel units, powers of two (next_pow2 of the surface extent);
+ * should_subdivide is hexopt_occ_classify_cells' predicate with the same origin, voxel_size,
+ * mesh_transform (nullable) and stop_extent. Runs level-synchronously on the surface's
+ * device. The leaf set, the node positions and each leaf's corner positions equal the
+ * reference's; leaves and nodes are numbered differently (leaves level by level, x fastest
+ * within a level; nodes in lattice order). NULL on a bad grid, a level above 2^31 cells or a
+ * device error: hexopt_octree_error() says why. */
+typedef struct hexopt_octree hexopt_octree;
+hexopt_octree* hexopt_octree_build(hexopt_surface* s, const int32_t* grid3, const double* origin, double voxel_size,
+                                   const double* mesh_transform, int stop_extent, int graded, int paired);
+const char* hexopt_octree_error(void);
+void hexopt_octree_destroy(hexopt_octree* o);
+/* out4 = { leaves, nodes, levels holding leaves, predicate evaluations }; *ms (nullable) the
+ * device time of the build */
+int hexopt_octree_sizes(const hexopt_octree* o, int64_t* out4, double* ms);
+/* leaves[n][4] (x, y, z, extent), nodes[m][3] (integer positions), V[m][3]
+ * (mesh_transform + origin + voxel_size * position, octree_mesh's mo.V) and H[n][8] (node
+ * ids of each leaf's corners in the reference's corner order, octree_mesh's mo.Hs); any may
+ * be NULL. */
+int hexopt_octree_copy(const hexopt_octree* o, int32_t* leaves, int32_t* nodes, double* V, int32_t* H);
+
 /* ---- feature-curve projection (LINE branch of dirty_graph_projection,
  * grid_hex_meshing.cpp:3961-4026, with point_line_projection, global_functions.cpp:3454-3465) ----
  * A curve set is the input surface's feature curves (mf.curve_vs / mf.circles): curve c is

@@ -319,9 +319,61 @@ def io_cases():
     save("io_files", **out)
 
 
+def octree_surfaces():
+    """name -> (V, F, num_voxels, stop_extent, modes) for the octree fixtures."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from occ_cases import sphere
+    sV, sF = sphere(3, 0.9)
+    tV, tF = mg.torus_surface(R=1.0, r=0.35, nu=40, nv=20)
+    cV, cF = mg.cube_surface(n=16, lo=0.0, hi=1.0)
+    both = [(1, 1), (0, 1), (1, 0), (0, 0)]
+    return {
+        "sphere64": (sV, sF, 64, 2, both),
+        "ellipsoid64": (sV * np.array([1.0, 0.45, 0.3]), sF, 64, 1, [(1, 1)]),
+        "torus128": (tV, tF.astype(np.int32), 128, 2, [(1, 1)]),
+        # the unit-cube input of the mesher's defaults: num_voxels 2^20, STOP_EXTENT 15
+        "cube_default": (cV, cF.astype(np.int32), 1 << 20, 1 << 15, [(1, 1)]),
+    }
+
+
+def octree_cases():
+    """octree_cases.npz: the reference's leaf set (sorted (x, y, z, extent) rows), its node
+    count and its own is2to1Graded / isPaired verdicts, per surface and mode. The reference's
+    nodes are checked here to be exactly the leaves' corner positions, so the leaf set fixes
+    them."""
+    import octcases as oc
+    if not oc.available():
+        raise SystemExit("oracle/_ref/octree_ref is not built (make -C oracle/ref)")
+    out = {}
+    for name, (V, F, nvox, stop, modes) in octree_surfaces().items():
+        origin, vs, grid, mt = oc.mesher_frame(V, nvox)
+        out["%s__V" % name] = V
+        out["%s__F" % name] = F
+        out["%s__frame" % name] = np.concatenate([origin, [vs], mt])
+        out["%s__grid" % name] = np.array(grid, np.int32)
+        out["%s__stop" % name] = np.array(stop)
+        for g, p in modes:
+            r = oc.ref_octree(V, F, grid, origin, vs, mt, stop, bool(g), bool(p))
+            key = "%s__g%dp%d" % (name, g, p)
+            lv = r["leaves"]
+            out[key + "__leaves"] = lv[np.lexsort(lv.T[::-1])]
+            nd = r["nodes"]
+            corners = np.concatenate([lv[:, :3] + lv[:, 3:4] * np.array(c) for c in mg.HEX_CORNERS])
+            assert len(np.unique(nd, axis=0)) == len(nd)
+            assert np.array_equal(np.unique(corners, axis=0), np.unique(nd, axis=0))
+            # corner k of each leaf is the node at the leaf's k-th corner position
+            assert np.array_equal(nd[r["corners"]], lv[:, None, :3] + lv[:, None, 3:4] * np.array(mg.HEX_CORNERS))
+            out[key + "__nnodes"] = np.array(len(nd))
+            out[key + "__flags"] = np.array([r["graded"], r["paired"]], np.int32)
+            print(name, grid, g, p, len(lv), "leaves", len(nd), "nodes")
+    save("octree_cases", **out)
+
+
 if __name__ == "__main__":
     import sys
-    if sys.argv[1:] == ["io"]:
+    if sys.argv[1:] == ["octree"]:
+        octree_cases()
+    elif sys.argv[1:] == ["io"]:
         io_cases()
     elif sys.argv[1:] == ["metro"]:
         metro_cases()
@@ -335,3 +387,4 @@ if __name__ == "__main__":
         glue_cases()
         smooth_cases()
         io_cases()
+        octree_cases()
