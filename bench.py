@@ -150,6 +150,27 @@ def glue_measure(target, steps):
             "fp64_operator_kernel_us": round(1e3 * ms_apply, 2)}
 
 
+def octree_measure(reps=5):
+    """The octree subdivision of octree_mesh (hexopt_octree_build: predicate, grading and
+    pairing cascades, leaf-to-hex export) around the 40,000-triangle clipped torus at
+    num_voxels 4096 and STOP extent 8 (3,717,184 leaves). Device time of the build, best of
+    `reps`, and the leaf and node counts."""
+    from pyhexopt import capi, meshgen as mg
+    V, F, _ = mg.clipped_torus_surface(R_MAJ, R_MIN, Z_CUT, nu=200, nband=30, nring=20)
+    lo, hi = V.min(0), V.max(0)
+    vs = float((hi - lo).max()) / 4096
+    grid = [1 << int(np.ceil(np.log2(np.ceil(e / vs)))) for e in (hi - lo)]
+    mt = (lo + hi) / 2 - (lo + (lo + vs * np.array(grid, np.float64))) * 0.5
+    s = capi.Surface(V, F)
+    best = None
+    for _ in range(reps):
+        out = s.octree(grid, lo, vs, mt, 8)
+        best = out["ms"] if best is None else min(best, out["ms"])
+    return {"surface_triangles": int(F.shape[0]), "grid": [int(g) for g in grid], "leaves": int(len(out["leaves"])),
+            "nodes": int(len(out["nodes"])), "levels": out["levels"], "predicate_evaluations": out["evaluated"],
+            "ms_build": round(best, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +179,7 @@ def main():
     ap.add_argument("--hexes", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-glue", action="store_true")
+    ap.add_argument("--no-octree", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see the design notes on "
@@ -237,6 +259,8 @@ def main():
     }
     if not args.no_glue:
         out["padding_untangle"] = glue_measure(max(args.hexes // 8, 20000), 5)
+    if not args.no_octree:
+        out["octree_build"] = octree_measure()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(max(args.hexes // 8, 20000))
     print(json.dumps(out))

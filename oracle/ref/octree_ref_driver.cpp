@@ -15,9 +15,13 @@
 // the geogram library; its queue loop is therefore restated below around the reference's own
 // splitCell (octree.cpp:501-593). The predicate restates octree_mesh's should_subdivide
 // (grid_meshing/grid_hex_meshing.cpp:502-517) with the facet-box test of
-// GEO::MeshFacetsAABB::compute_bbox_facet_bbox_intersections done over every facet
-// (closed intervals, geogram basic/geometry.h bboxes_overlap).
+// GEO::MeshFacetsAABB::compute_bbox_facet_bbox_intersections (closed intervals, geogram
+// basic/geometry.h bboxes_overlap) over the facets binned in a uniform grid: a facet box and a
+// cell box that touch share the bin of the touching coordinate, since both map it through the
+// same expression, so the answer is the every-facet answer.
+// The subdivision's wall time (queue loop and splits, not the binning) goes to stdout.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -80,6 +84,33 @@ int main(int argc, char** argv) {
         fb[6 * t + d] = mn;
         fb[6 * t + 3 + d] = mx;
       }
+    // uniform bins over the facet boxes' bounding box
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int t = 0; t < nF; ++t)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], fb[6 * t + d]);
+        hi[d] = std::max(hi[d], fb[6 * t + 3 + d]);
+      }
+    const int nb = std::max(1, std::min(128, (int)std::cbrt((double)nF)));
+    double w[3];
+    for (int d = 0; d < 3; ++d) w[d] = hi[d] > lo[d] ? (hi[d] - lo[d]) / nb : 1.0;
+    auto bin = [&](double x, int d) {
+      const double t = std::floor((x - lo[d]) / w[d]);
+      return t < 0 ? 0 : t >= nb ? nb - 1 : (int)t;
+    };
+    std::vector<std::vector<int>> bins((size_t)nb * nb * nb);
+    for (int t = 0; t < nF; ++t) {
+      int b0[3], b1[3];
+      for (int d = 0; d < 3; ++d) {
+        b0[d] = bin(fb[6 * t + d], d);
+        b1[d] = bin(fb[6 * t + 3 + d], d);
+      }
+      for (int k = b0[2]; k <= b1[2]; ++k)
+        for (int j = b0[1]; j <= b1[1]; ++j)
+          for (int i = b0[0]; i <= b1[0]; ++i) bins[((size_t)k * nb + j) * nb + i].push_back(t);
+    }
+    std::vector<int> stamp(nF, -1);
+    int query = 0;
     auto should_subdivide = [&](int x, int y, int z, int extent) {
       if (extent <= stop_extent) return false;
       const int p[3] = {x, y, z};
@@ -88,16 +119,30 @@ int main(int argc, char** argv) {
         mn[d] = mesh_transform[d] + origin[d] + voxel_size * p[d];
         mx[d] = mn[d] + voxel_size * extent;
       }
-      for (int t = 0; t < nF; ++t) {
-        bool ov = true;
-        for (int d = 0; d < 3 && ov; ++d) ov = !(mx[d] < fb[6 * t + d] || fb[6 * t + 3 + d] < mn[d]);
-        if (ov) return true;
+      for (int d = 0; d < 3; ++d)
+        if (mx[d] < lo[d] || hi[d] < mn[d]) return false;
+      int b0[3], b1[3];
+      for (int d = 0; d < 3; ++d) {
+        b0[d] = bin(mn[d], d);
+        b1[d] = bin(mx[d], d);
       }
+      ++query;
+      for (int k = b0[2]; k <= b1[2]; ++k)
+        for (int j = b0[1]; j <= b1[1]; ++j)
+          for (int i = b0[0]; i <= b1[0]; ++i)
+            for (int t : bins[((size_t)k * nb + j) * nb + i]) {
+              if (stamp[t] == query) continue;
+              stamp[t] = query;
+              bool ov = true;
+              for (int d = 0; d < 3 && ov; ++d) ov = !(mx[d] < fb[6 * t + d] || fb[6 * t + 3 + d] < mn[d]);
+              if (ov) return true;
+            }
       return false;
     };
 
     OctreeGrid octree(Eigen::Vector3i(hdr[0], hdr[1], hdr[2]));
     // the queue of OctreeGrid::subdivide (octree.cpp:648-684), maxCells unlimited
+    const auto t0 = std::chrono::steady_clock::now();
     std::queue<int> pending;
     for (int i = 0; i < octree.numCells(); ++i)
       if (octree.cellIsLeaf(i)) pending.push(i);
@@ -112,6 +157,8 @@ int main(int argc, char** argv) {
         for (int k = 0; k < 8; ++k) pending.push(octree.m_Cells[id].firstChild + k);
       }
     }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    printf("subdivide %.6f s, %d cells, %d nodes\n", secs, octree.numCells(), octree.numNodes());
 
     std::vector<int32_t> leaves, corners;
     for (int c = 0; c < octree.numCells(); ++c) {

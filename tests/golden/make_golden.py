@@ -366,6 +366,19 @@ def octree_cases():
             out[key + "__nnodes"] = np.array(len(nd))
             out[key + "__flags"] = np.array([r["graded"], r["paired"]], np.int32)
             print(name, grid, g, p, len(lv), "leaves", len(nd), "nodes")
+    # full size: the clipped torus of the bench at num_voxels 4096, STOP extent 8 (about 3.7M
+    # leaves); only the counts and a SHA-256 of the sorted leaf rows are kept
+    import hashlib
+    V, F, _ = mg.clipped_torus_surface(1.0, 0.35, 0.2, nu=200, nband=30, nring=20)
+    F = F.astype(np.int32)
+    origin, vs, grid, mt = oc.mesher_frame(V, 4096)
+    r = oc.ref_octree(V, F, grid, origin, vs, mt, 8)
+    lv = np.ascontiguousarray(r["leaves"][np.lexsort(r["leaves"].T[::-1])], np.int32)
+    out["torus_full__frame"] = np.concatenate([origin, [vs], mt])
+    out["torus_full__grid"] = np.array(grid, np.int32)
+    out["torus_full__counts"] = np.array([len(lv), len(r["nodes"]), r["graded"], r["paired"]])
+    out["torus_full__sha256"] = np.array(hashlib.sha256(lv.tobytes()).hexdigest())
+    print("torus_full", grid, len(lv), "leaves", len(r["nodes"]), "nodes")
     save("octree_cases", **out)
 
 

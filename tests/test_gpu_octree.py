@@ -1,8 +1,8 @@
 """GPU octree subdivision (hexopt_octree_build) against the compiled reference's OctreeGrid
 (fixtures in tests/golden/octree_cases.npz): the same leaf set for every surface and every
 graded/paired mode, the same node count, octree_mesh's hex export consistent with the leaves
-and node positions rounded as the reference rounds them; at full size (about a million
-leaves) the octree properties of tests/octree_checks.py."""
+and node positions rounded as the reference rounds them; at full size (3.7M leaves) the same
+leaf set by SHA-256 and the octree properties of tests/octree_checks.py."""
 import os
 
 import numpy as np
@@ -54,21 +54,22 @@ def test_octree_mesh_feeds_the_optimiser_context():
     assert mn > 0.999 and avg > 0.999   # axis-aligned cubes
 
 
-def test_full_size_octree_properties():
-    """About a million leaves around the 40,000-triangle clipped torus: tiling, pairing and
-    grading hold, and the export is consistent."""
+def test_full_size_octree_matches_reference():
+    """3,717,184 leaves around the 40,000-triangle clipped torus (num_voxels 4096, STOP extent
+    8): the SHA-256 of the sorted leaf rows and the node count equal the reference's, and
+    tiling, pairing, grading and the export hold."""
+    import hashlib
     from pyhexopt import capi
     V, F, _ = mg.clipped_torus_surface(1.0, 0.35, 0.2, nu=200, nband=30, nring=20)
-    lo, hi = V.min(0), V.max(0)
-    num_voxels = 1 << 12
-    vs = float((hi - lo).max()) / num_voxels
-    grid = [1 << int(np.ceil(np.log2(np.ceil(e / vs)))) for e in (hi - lo)]
-    origin = lo
-    mt = (lo + hi) / 2 - (origin + (origin + vs * np.array(grid, np.float64))) * 0.5
+    fr, grid = FIX["torus_full__frame"], FIX["torus_full__grid"]
+    origin, vs, mt = fr[:3], float(fr[3]), fr[4:]
     s = capi.Surface(V, F)
     out = s.octree(grid, origin, vs, mt, 8)
     print("full-size octree:", len(out["leaves"]), "leaves", len(out["nodes"]), "nodes", out["levels"], "levels",
           "%.2f ms" % out["ms"], flush=True)
-    assert len(out["leaves"]) > 200000
+    counts = FIX["torus_full__counts"]
+    assert len(out["leaves"]) == counts[0] and len(out["nodes"]) == counts[1]
+    lv = np.ascontiguousarray(_sorted(out["leaves"]), np.int32)
+    assert hashlib.sha256(lv.tobytes()).hexdigest() == str(FIX["torus_full__sha256"])
     assert check_octree(out["leaves"], grid)
     assert check_mesh(out, mt + origin, vs)
