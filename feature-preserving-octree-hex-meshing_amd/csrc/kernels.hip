@@ -1410,46 +1410,70 @@ __global__ __launch_bounds__(256) void cg32_lines_kernel(SoftDev sd, int nV, con
   if (i < sd.nL) line_row(sd, nV, z, pprev, pnew, out, (float)st->beta, i);
 }
 
-// one thread per vertex, which also updates its line unknown (each line unknown belongs to
-// exactly one vertex)
+// a grid-stride pass over the vertices (each thread also updates its vertices' line unknowns:
+// each line unknown belongs to exactly one vertex). With ntiles > 0 every block first forms
+// alpha itself from the operator pass's ntiles partials, as cg32_alpha_kernel would (the
+// same sum in every block, so every block holds the same alpha).
 __global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev sd, double prox,
                                                           const float4* __restrict__ halo,
                                                           const float* __restrict__ p, const float* __restrict__ out,
                                                           const float* __restrict__ invd, float* __restrict__ x,
                                                           float* __restrict__ r, float* __restrict__ z,
-                                                          const CGState* st, double* part_rr, double* part_rz,
+                                                          const CGState* st, const double* part_tile, int ntiles,
+                                                          double* part_rr, double* part_rz,
                                                           double* part_qz, double* part_bzq, double* part_qq) {
   __shared__ double red[5 * 16];
   if (st->done) return;
   const int nV = tm.nV;
   const size_t nA = 3 * (size_t)nV;
-  const float alpha = (float)st->alpha;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  float alpha;
+  if (ntiles > 0) {
+    const double e = sum_parts(part_tile, ntiles, red);
+    const double b = (double)(float)st->beta;
+    alpha = (float)(st->absNew / (e + st->qz + 2.0 * b * st->bzq + b * b * st->qq));
+  } else {
+    alpha = (float)st->alpha;
+  }
   double srr = 0, srz = 0, fz = 0, fzq = 0, fq = 0;
-  if (g < nV) {
-    // E p: the sum of the vertex's run (one place per tile holding it, in tile order)
-    double a[3] = {0.0, 0.0, 0.0};
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nV; g += gridDim.x * blockDim.x) {
+    // every load that does not wait on another goes out first: the vertex's run bounds, its
+    // p, r, x, 1/diag and constraint row; then the R rows (which wait only on the constraint
+    // row) and the run of E p places (one per tile holding the vertex, in tile order)
     const int k0 = tm.v_dst[g], k1 = tm.v_dst[g + 1];
-    for (int k = k0; k < k1; ++k) {
-      const float4 hv = halo[k];
-      a[0] += (double)hv.x; a[1] += (double)hv.y; a[2] += (double)hv.z;
-    }
-    double pv[3];
+    float pf[3], rf[3], xf[3], df[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) pv[d] = (double)p[(size_t)d * nV + g];
+    for (int d = 0; d < 3; ++d) {
+      const size_t j = (size_t)d * nV + g;
+      pf[d] = p[j];
+      rf[d] = r[j];
+      xf[d] = x[j];
+      df[d] = invd[j];
+    }
     const int2 cl = sd.cl ? reinterpret_cast<const int2*>(sd.cl)[g] : make_int2(-1, -1);
     const int ci = cl.x, ln = cl.y;
+    double pv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pv[d] = (double)pf[d];
     const double pa = ln >= 0 ? (double)p[nA + ln] : 0.0;
     double vr[3];
     vertex_rows(sd, prox, ci, ln, pv, pa, vr);
+    const float4 h0 = k1 > k0 ? halo[k0] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 h1 = k1 > k0 + 1 ? halo[k0 + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+    double a[3] = {0.0, 0.0, 0.0};
+    if (k1 > k0) { a[0] += (double)h0.x; a[1] += (double)h0.y; a[2] += (double)h0.z; }
+    if (k1 > k0 + 1) { a[0] += (double)h1.x; a[1] += (double)h1.y; a[2] += (double)h1.z; }
+    for (int k = k0 + 2; k < k1; ++k) {
+      const float4 hv = halo[k];
+      a[0] += (double)hv.x; a[1] += (double)hv.y; a[2] += (double)hv.z;
+    }
     double zn[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const size_t j = (size_t)d * nV + g;
-      x[j] += alpha * p[j];
-      const float rv = r[j] - alpha * (float)(a[d] + vr[d]);
+      x[j] = xf[d] + alpha * pf[d];
+      const float rv = rf[d] - alpha * (float)(a[d] + vr[d]);
       r[j] = rv;
-      const float zv = invd[j] * rv;
+      const float zv = df[d] * rv;
       z[j] = zv;
       zn[d] = (double)zv;
       srr += (double)rv * (double)rv;
@@ -1468,9 +1492,9 @@ __global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev s
       srz += (double)rv * (double)zv;
     }
     // the next direction is z + beta p: its R sums
-    fz = vertex_form(sd, prox, ci, ln, zn, za, zn, za);
-    fzq = vertex_form(sd, prox, ci, ln, zn, za, pv, pa);
-    fq = vertex_form(sd, prox, ci, ln, pv, pa, pv, pa);
+    fz += vertex_form(sd, prox, ci, ln, zn, za, zn, za);
+    fzq += vertex_form(sd, prox, ci, ln, zn, za, pv, pa);
+    fq += vertex_form(sd, prox, ci, ln, pv, pa, pv, pa);
   }
   double v[5] = {srr, srz, fz, fzq, fq};
   block_sum_n<5>(v, red);
@@ -2208,6 +2232,25 @@ int launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const
   return -tm.nTiles;
 }
 
+// blocks of the grid-stride update pass: as many as are resident at once on the device (so
+// no block waits for another to finish all its vertices), which keeps the following one-block
+// reduction of their partials short
+static int update_grid(int nV) {
+  static int fit[64] = {0};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const int want = std::max(1, (nV + 255) / 256);
+  if (dev < 0 || dev >= 64) return std::min(want, 1024);
+  if (fit[dev] == 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg32_update_kernel, 256, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      per_cu = cus = 0;
+    fit[dev] = std::max(64, per_cu * cus);
+  }
+  return std::min(want, fit[dev]);
+}
+
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
                            const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
@@ -2217,11 +2260,15 @@ void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, 
   // a positive count: the operator pass did the lines and the vertices without hexes
   const int np = launch_cg32_apply(tm, ro, K, sd, iso, niso, z, pprev, pnew, out, halo, cs, part_tile, st);
   const bool done_iso = np > 0;
-  hipLaunchKernelGGL(cg32_alpha_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_tile, done_iso ? np : -np,
-                     tm.nV, iso, done_iso ? 0 : niso, z, pprev, pnew);
-  const int nbu = (tm.nV + 255) / 256;
-  hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, (const float4*)halo, (const float*)pnew, (const float*)out, invd, x, r, zout,
-                     (const CGState*)cs, part_rr, part_rz, part_qz, part_bzq, part_qq);
+  // alpha folds into the update pass unless the vertices without hexes still need p
+  const bool fold = done_iso || niso == 0;
+  if (!fold)
+    hipLaunchKernelGGL(cg32_alpha_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_tile, -np, tm.nV, iso,
+                       niso, z, pprev, pnew);
+  const int nbu = update_grid(tm.nV);
+  hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, (const float4*)halo,
+                     (const float*)pnew, (const float*)out, invd, x, r, zout, (const CGState*)cs,
+                     (const double*)part_tile, fold ? std::abs(np) : 0, part_rr, part_rz, part_qz, part_bzq, part_qq);
   hipLaunchKernelGGL(cg32_beta_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_rr, (const double*)part_rz,
                      (const double*)part_qz, (const double*)part_bzq, (const double*)part_qq, nbu);
 }
