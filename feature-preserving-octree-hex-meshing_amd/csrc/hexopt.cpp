@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1233,7 +1234,15 @@ int hexopt_hex_inside(hexopt_ctx* c, hexopt_surface* s, int which, double* signe
 }
 
 struct hexopt_octree {
+  int device = 0;
+  OctreeDev* dev = nullptr;
   OctreeOut r;
+  ~hexopt_octree() {
+    if (dev) {
+      (void)hipSetDevice(device);
+      octree_destroy(dev);
+    }
+  }
 };
 
 namespace {
@@ -1256,24 +1265,36 @@ hexopt_octree* hexopt_octree_build(hexopt_surface* s, const int32_t* grid3, cons
   double base[3];
   for (int d = 0; d < 3; ++d) base[d] = (mesh_transform ? mesh_transform[d] : 0.0) + origin[d];
   const int grid[3] = {grid3[0], grid3[1], grid3[2]};
-  auto o = new (std::nothrow) hexopt_octree;
+  std::unique_ptr<hexopt_octree> o(new (std::nothrow) hexopt_octree);
   if (!o) {
     g_octree_error = "out of host memory";
     return nullptr;
   }
+  o->device = s->device;
   std::string err;
-  bool ok = false;
   try {
-    ok = build_octree(s->view, grid, base, voxel_size, stop_extent, graded != 0, paired != 0, s->stream, o->r, err);
+    o->dev = octree_create(grid, base, voxel_size, graded != 0, paired != 0, err);
+    if (o->dev && octree_timed(*o->dev, &s->view, stop_extent, 0, nullptr, s->stream, o->r, err)) return o.release();
   } catch (const std::exception& e) {
     err = std::string("octree: ") + e.what();
   }
-  if (!ok) {
-    g_octree_error = err;
-    delete o;
-    return nullptr;
+  g_octree_error = err;
+  return nullptr;
+}
+
+int hexopt_octree_refine(hexopt_octree* o, hexopt_surface* s, int stop_extent, int n, const int32_t* cells) {
+  g_octree_error.clear();
+  if (!o || !s || n < 0 || (n > 0 && !cells)) return (g_octree_error = "bad argument", HEXOPT_EINVAL);
+  if (s->device != o->device) return (g_octree_error = "surface and octree are on different devices", HEXOPT_EINVAL);
+  if (hipSetDevice(o->device) != hipSuccess) return (g_octree_error = "hipSetDevice failed", HEXOPT_EHIP);
+  std::string err;
+  try {
+    if (octree_timed(*o->dev, &s->view, stop_extent, n, cells, s->stream, o->r, err)) return HEXOPT_OK;
+  } catch (const std::exception& e) {
+    err = std::string("octree: ") + e.what();
   }
-  return o;
+  g_octree_error = err;
+  return err.find("is not a cell") != std::string::npos ? HEXOPT_EINVAL : HEXOPT_EHIP;
 }
 
 void hexopt_octree_destroy(hexopt_octree* o) { delete o; }

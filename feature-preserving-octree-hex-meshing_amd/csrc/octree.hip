@@ -24,6 +24,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <memory>
 
 #include "occupancy.h"
 
@@ -227,6 +228,20 @@ __global__ __launch_bounds__(kOctBlock) void hex_kernel(const int4* __restrict__
   }
 }
 
+// listed cells of one level for a targeted subdivision: a cell is split when it is a leaf now
+// (its parent is split, or it is a root, and it is not split itself) and the predicate held
+__global__ __launch_bounds__(kOctBlock) void refine_mark_kernel(const int4* __restrict__ cells,
+                                                                 const uint8_t* __restrict__ pred, int n,
+                                                                 const uint8_t* __restrict__ Sp, Dims dp,
+                                                                 uint8_t* __restrict__ S, Dims d) {
+  const int i = blockIdx.x * kOctBlock + threadIdx.x;
+  if (i >= n) return;
+  const int4 c = cells[i];
+  const int x = c.x / c.w, y = c.y / c.w, z = c.z / c.w;
+  const bool exists = Sp == nullptr || Sp[lin3(x >> 1, y >> 1, z >> 1, dp)] != 0;
+  if (exists && pred[i] != 0 && c.w > 1) S[lin3(x, y, z, d)] = 1;
+}
+
 struct ToInt {
   __host__ __device__ int operator()(uint8_t v) const { return v; }
 };
@@ -246,152 +261,225 @@ bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
     }                                                                         \
   } while (0)
 
-bool build_octree(const SurfaceDev& s, const int grid[3], const double base[3], double voxel_size, int stop_extent,
-                  bool graded, bool paired, hipStream_t st, OctreeOut& out, std::string& err) {
-  out = OctreeOut();
+struct OctreeDev {
+  int grid[3];
+  double base[3];
+  double vs;
+  bool graded, paired;
+  int R = 0, D = 0;  // root extent, level of extent-1 cells
+  Dims root{0, 0, 0};
+  std::vector<std::unique_ptr<OctBuf>> S;  // split flags per level, allocated and zeroed on demand
+  int Lmax = -1;                            // deepest level whose grid may hold a split
+  OctBuf cand, flag, sel, tmp, num, idx;
+  int* h_num = nullptr;
+  ~OctreeDev() {
+    if (h_num) (void)hipHostFree(h_num);
+  }
+  Dims dims(int L) const { return Dims{root.x << L, root.y << L, root.z << L}; }
+  int extent(int L) const { return R >> L; }
+};
+
+namespace {
+
+constexpr long long kMaxGrid = (1LL << 31) - 1;
+
+bool level_grid(OctreeDev& o, int L, hipStream_t st, std::string& err) {
+  while ((int)o.S.size() <= L) o.S.emplace_back(new OctBuf());
+  if (o.S[L]->p) return true;
+  const Dims d = o.dims(L);
+  if (d.count() > kMaxGrid) {
+    err = "octree: level " + std::to_string(L) + " exceeds the dense grid limit of 2^31 cells";
+    return false;
+  }
+  OCT_CK(o.S[L]->ensure((size_t)d.count()));
+  OCT_CK(hipMemsetAsync(o.S[L]->p, 0, (size_t)d.count(), st));
+  return true;
+}
+
+bool read_count(OctreeDev& o, hipStream_t st, int& n, std::string& err) {
+  OCT_CK(hipMemcpyAsync(o.h_num, o.num.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  OCT_CK(hipStreamSynchronize(st));
+  n = *o.h_num;
+  return true;
+}
+
+// pairing and grading closure, bottom-up over every allocated level
+bool close_splits(OctreeDev& o, hipStream_t st, std::string& err) {
+  for (int L = o.Lmax; L >= 1; --L) {
+    const Dims d = o.dims(L), dp = o.dims(L - 1);
+    if (o.paired)
+      hipLaunchKernelGGL(pair_kernel, dim3(blocks(dp.count())), dim3(kOctBlock), 0, st, o.S[L]->as<uint8_t>(), d, dp,
+                         dp.count());
+    hipLaunchKernelGGL(up_kernel, dim3(blocks(d.count())), dim3(kOctBlock), 0, st, o.S[L]->as<uint8_t>(), d,
+                       d.count(), o.S[L - 1]->as<uint8_t>(), dp, o.graded ? 1 : 0);
+    OCT_CK(hipGetLastError());
+  }
+  if (o.Lmax >= 0 && o.paired) {  // one split root splits every root
+    std::vector<uint8_t> r((size_t)o.root.count());
+    OCT_CK(hipMemcpyAsync(r.data(), o.S[0]->p, r.size(), hipMemcpyDeviceToHost, st));
+    OCT_CK(hipStreamSynchronize(st));
+    if (std::find(r.begin(), r.end(), 1) != r.end()) OCT_CK(hipMemsetAsync(o.S[0]->p, 1, r.size(), st));
+  }
+  return true;
+}
+
+}  // namespace
+
+OctreeDev* octree_create(const int grid[3], const double base[3], double voxel_size, bool graded, bool paired,
+                         std::string& err) {
   for (int a = 0; a < 3; ++a)
     if (!is_pow2(grid[a]) || grid[a] > (1 << 30)) {
       err = "octree: grid sizes must be powers of two up to 2^30";
-      return false;
+      return nullptr;
     }
-  const int R = std::min(grid[0], std::min(grid[1], grid[2]));  // root cell extent
-  int D = 0;                                                    // level of extent-1 cells
-  while ((1 << D) < R) ++D;
-  const Dims root{grid[0] / R, grid[1] / R, grid[2] / R};
-  auto dims = [&](int L) { return Dims{root.x << L, root.y << L, root.z << L}; };
-  auto extent = [&](int L) { return R >> L; };
-  constexpr long long kMaxGrid = (1LL << 31) - 1;
+  auto o = std::make_unique<OctreeDev>();
+  for (int a = 0; a < 3; ++a) {
+    o->grid[a] = grid[a];
+    o->base[a] = base[a];
+  }
+  o->vs = voxel_size;
+  o->graded = graded;
+  o->paired = paired;
+  o->R = std::min(grid[0], std::min(grid[1], grid[2]));
+  while ((1 << o->D) < o->R) ++o->D;
+  o->root = Dims{grid[0] / o->R, grid[1] / o->R, grid[2] / o->R};
+  if (hipHostMalloc((void**)&o->h_num, sizeof(int)) != hipSuccess || o->num.ensure(sizeof(int)) != hipSuccess) {
+    err = "octree: device allocation failed";
+    return nullptr;
+  }
+  return o.release();
+}
 
-  hipEvent_t ev0, ev1;
-  OCT_CK(hipEventCreate(&ev0));
-  OCT_CK(hipEventCreate(&ev1));
-  struct EvGuard {
-    hipEvent_t a, b;
-    ~EvGuard() {
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
-    }
-  } evg{ev0, ev1};
-  OCT_CK(hipEventRecord(ev0, st));
+void octree_destroy(OctreeDev* o) { delete o; }
 
-  std::vector<OctBuf> S(D + 1);  // split flags per level
-  OctBuf cand, flag, sel, tmp, num, cube;
-  int* h_num = nullptr;
-  OCT_CK(hipHostMalloc((void**)&h_num, sizeof(int)));
-  struct PinGuard {
-    int* p;
-    ~PinGuard() { (void)hipHostFree(p); }
-  } pg{h_num};
-  OCT_CK(num.ensure(sizeof(int)));
-
-  auto select_count = [&](int& n) -> bool {
-    OCT_CK(hipMemcpyAsync(h_num, num.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    OCT_CK(hipStreamSynchronize(st));
-    n = *h_num;
-    return true;
-  };
-
-  // ---- A: predicate splits, top-down --------------------------------------------------------
+bool octree_subdivide(OctreeDev& o, const SurfaceDev& s, int stop_extent, hipStream_t st, long& evaluated,
+                      std::string& err) {
+  // ---- predicate splits, top-down from the root cells ----
   std::vector<int> roots;
-  for (int z = 0; z < root.z; ++z)
-    for (int y = 0; y < root.y; ++y)
-      for (int x = 0; x < root.x; ++x) roots.insert(roots.end(), {x * R, y * R, z * R, R});
+  const int R = o.R;
+  for (int z = 0; z < o.root.z; ++z)
+    for (int y = 0; y < o.root.y; ++y)
+      for (int x = 0; x < o.root.x; ++x) roots.insert(roots.end(), {x * R, y * R, z * R, R});
   long long ncand = (long long)roots.size() / 4;
-  OCT_CK(cand.ensure(roots.size() * sizeof(int)));
-  OCT_CK(hipMemcpyAsync(cand.p, roots.data(), roots.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  int Lmax = -1;  // deepest level holding a split
-  for (int L = 0; L <= D && ncand > 0; ++L) {
-    const Dims d = dims(L);
-    if (d.count() > kMaxGrid || ncand > kMaxGrid / 8) {
-      err = "octree: level " + std::to_string(L) + " exceeds the dense grid limit of 2^31 cells";
+  OCT_CK(o.cand.ensure(roots.size() * sizeof(int)));
+  OCT_CK(hipMemcpyAsync(o.cand.p, roots.data(), roots.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  for (int L = 0; L <= o.D && ncand > 0; ++L) {
+    if (ncand > kMaxGrid / 8) {
+      err = "octree: more than 2^28 candidate cells at level " + std::to_string(L);
       return false;
     }
-    out.evaluated += ncand;
-    OCT_CK(S[L].ensure((size_t)d.count()));
-    OCT_CK(hipMemsetAsync(S[L].p, 0, (size_t)d.count(), st));
-    OCT_CK(flag.ensure((size_t)ncand));
-    launch_occ_classify(s, (int)ncand, cand.as<int>(), base, voxel_size, stop_extent, flag.as<uint8_t>(), st);
+    if (!level_grid(o, L, st, err)) return false;
+    const Dims d = o.dims(L);
+    evaluated += ncand;
+    OCT_CK(o.flag.ensure((size_t)ncand));
+    launch_occ_classify(s, (int)ncand, o.cand.as<int>(), o.base, o.vs, stop_extent, o.flag.as<uint8_t>(), st);
     OCT_CK(hipGetLastError());
-    hipLaunchKernelGGL(mark_split_kernel, dim3(blocks(ncand)), dim3(kOctBlock), 0, st, cand.as<int4>(),
-                       flag.as<uint8_t>(), (int)ncand, d, S[L].as<uint8_t>());
+    hipLaunchKernelGGL(mark_split_kernel, dim3(blocks(ncand)), dim3(kOctBlock), 0, st, o.cand.as<int4>(),
+                       o.flag.as<uint8_t>(), (int)ncand, d, o.S[L]->as<uint8_t>());
     OCT_CK(hipGetLastError());
-    OCT_CK(sel.ensure((size_t)ncand * sizeof(int4)));
+    OCT_CK(o.sel.ensure((size_t)ncand * sizeof(int4)));
     size_t tb = 0;
-    OCT_CK(hipcub::DeviceSelect::Flagged(nullptr, tb, cand.as<int4>(), flag.as<uint8_t>(), sel.as<int4>(),
-                                         num.as<int>(), (int)ncand, st));
-    OCT_CK(tmp.ensure(tb));
-    OCT_CK(hipcub::DeviceSelect::Flagged(tmp.p, tb, cand.as<int4>(), flag.as<uint8_t>(), sel.as<int4>(),
-                                         num.as<int>(), (int)ncand, st));
+    OCT_CK(hipcub::DeviceSelect::Flagged(nullptr, tb, o.cand.as<int4>(), o.flag.as<uint8_t>(), o.sel.as<int4>(),
+                                         o.num.as<int>(), (int)ncand, st));
+    OCT_CK(o.tmp.ensure(tb));
+    OCT_CK(hipcub::DeviceSelect::Flagged(o.tmp.p, tb, o.cand.as<int4>(), o.flag.as<uint8_t>(), o.sel.as<int4>(),
+                                         o.num.as<int>(), (int)ncand, st));
     int nsel = 0;
-    if (!select_count(nsel)) return false;
+    if (!read_count(o, st, nsel, err)) return false;
     if (nsel == 0) break;
-    Lmax = L;
+    o.Lmax = std::max(o.Lmax, L);
     ncand = 8LL * nsel;
-    OCT_CK(cand.ensure((size_t)ncand * sizeof(int4)));
-    hipLaunchKernelGGL(children_kernel, dim3(blocks(ncand)), dim3(kOctBlock), 0, st, sel.as<int4>(), (int)ncand,
-                       cand.as<int4>());
+    OCT_CK(o.cand.ensure((size_t)ncand * sizeof(int4)));
+    hipLaunchKernelGGL(children_kernel, dim3(blocks(ncand)), dim3(kOctBlock), 0, st, o.sel.as<int4>(), (int)ncand,
+                       o.cand.as<int4>());
     OCT_CK(hipGetLastError());
   }
+  // ---- cascades, bottom-up ----
+  return close_splits(o, st, err);
+}
 
-  // ---- B: cascades, bottom-up ----------------------------------------------------------------
-  for (int L = Lmax; L >= 1; --L) {
-    const Dims d = dims(L), dp = dims(L - 1);
-    if (paired)
-      hipLaunchKernelGGL(pair_kernel, dim3(blocks(dp.count())), dim3(kOctBlock), 0, st, S[L].as<uint8_t>(), d, dp,
-                         dp.count());
-    hipLaunchKernelGGL(up_kernel, dim3(blocks(d.count())), dim3(kOctBlock), 0, st, S[L].as<uint8_t>(), d, d.count(),
-                       S[L - 1].as<uint8_t>(), dp, graded ? 1 : 0);
-    OCT_CK(hipGetLastError());
+bool octree_refine(OctreeDev& o, const SurfaceDev& s, int stop_extent, int n, const int* cells, hipStream_t st,
+                   long& evaluated, std::string& err) {
+  // the listed cells, level by level (a cell's level follows from its extent)
+  std::vector<std::vector<int>> by_level(o.D + 1);
+  for (int i = 0; i < n; ++i) {
+    const int* c = cells + 4 * (size_t)i;
+    const int e = c[3];
+    int L = 0;
+    while (L <= o.D && o.extent(L) != e) ++L;
+    bool ok = L <= o.D;
+    for (int a = 0; a < 3 && ok; ++a) ok = c[a] >= 0 && c[a] % e == 0 && c[a] + e <= o.grid[a];
+    if (!ok) {
+      err = "octree: refine cell " + std::to_string(i) + " is not a cell of this octree's grid";
+      return false;
+    }
+    by_level[L].insert(by_level[L].end(), c, c + 4);
   }
-  if (Lmax >= 0 && paired) {  // one split root splits every root
-    std::vector<uint8_t> r((size_t)root.count());
-    OCT_CK(hipMemcpyAsync(r.data(), S[0].p, r.size(), hipMemcpyDeviceToHost, st));
+  for (int L = 0; L <= o.D; ++L) {
+    const long long m = (long long)by_level[L].size() / 4;
+    if (m == 0) continue;
+    if (!level_grid(o, L, st, err)) return false;
+    if (L > 0 && !level_grid(o, L - 1, st, err)) return false;
+    evaluated += m;
+    OCT_CK(o.cand.ensure((size_t)m * sizeof(int4)));
+    OCT_CK(hipMemcpyAsync(o.cand.p, by_level[L].data(), (size_t)m * sizeof(int4), hipMemcpyHostToDevice, st));
+    OCT_CK(o.flag.ensure((size_t)m));
+    launch_occ_classify(s, (int)m, o.cand.as<int>(), o.base, o.vs, stop_extent, o.flag.as<uint8_t>(), st);
+    OCT_CK(hipGetLastError());
+    hipLaunchKernelGGL(refine_mark_kernel, dim3(blocks(m)), dim3(kOctBlock), 0, st, o.cand.as<int4>(),
+                       o.flag.as<uint8_t>(), (int)m, L > 0 ? o.S[L - 1]->as<uint8_t>() : nullptr,
+                       L > 0 ? o.dims(L - 1) : o.dims(0), o.S[L]->as<uint8_t>(), o.dims(L));
+    OCT_CK(hipGetLastError());
+    // the copy above reuses `cand`: wait before the next level overwrites it
     OCT_CK(hipStreamSynchronize(st));
-    if (std::find(r.begin(), r.end(), 1) != r.end()) OCT_CK(hipMemsetAsync(S[0].p, 1, r.size(), st));
+    o.Lmax = std::max(o.Lmax, L);
   }
+  return close_splits(o, st, err);
+}
 
-  // ---- C: leaves, nodes, hexes ---------------------------------------------------------------
-  const int Lleaf = std::min(Lmax + 1, D);  // deepest level that can hold leaves
-  OctBuf leaves, idx;
+bool octree_export(OctreeDev& o, hipStream_t st, OctreeOut& out, std::string& err) {
+  const long evaluated = out.evaluated;
+  out = OctreeOut();
+  out.evaluated = evaluated;
+  const int Lleaf = std::min(o.Lmax + 1, o.D);  // deepest level that can hold leaves
+  for (int L = 0; L <= Lleaf; ++L)
+    if (!level_grid(o, L, st, err)) return false;
+  OctBuf leaves;
   long long nleaves = 0;
   int Lf = 0;  // deepest level that holds leaves
   for (int L = 0; L <= Lleaf; ++L) {
-    const Dims d = dims(L);
-    if (d.count() > kMaxGrid) {
-      err = "octree: level " + std::to_string(L) + " exceeds the dense grid limit of 2^31 cells";
-      return false;
-    }
-    const uint8_t* Sp = L > 0 ? S[L - 1].as<uint8_t>() : nullptr;
-    const uint8_t* Sl = L <= Lmax ? S[L].as<uint8_t>() : nullptr;
-    OCT_CK(flag.ensure((size_t)d.count()));
-    OCT_CK(idx.ensure((size_t)d.count() * sizeof(int)));
-    hipLaunchKernelGGL(leaf_flag_kernel, dim3(blocks(d.count())), dim3(kOctBlock), 0, st, Sp, L > 0 ? dims(L - 1) : d,
-                       Sl, d, d.count(), flag.as<uint8_t>());
+    const Dims d = o.dims(L);
+    const uint8_t* Sp = L > 0 ? o.S[L - 1]->as<uint8_t>() : nullptr;
+    const uint8_t* Sl = L <= o.Lmax ? o.S[L]->as<uint8_t>() : nullptr;
+    OCT_CK(o.flag.ensure((size_t)d.count()));
+    OCT_CK(o.idx.ensure((size_t)d.count() * sizeof(int)));
+    hipLaunchKernelGGL(leaf_flag_kernel, dim3(blocks(d.count())), dim3(kOctBlock), 0, st, Sp,
+                       L > 0 ? o.dims(L - 1) : d, Sl, d, d.count(), o.flag.as<uint8_t>());
     OCT_CK(hipGetLastError());
     hipcub::CountingInputIterator<int> it(0);
     size_t tb = 0;
-    OCT_CK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, flag.as<uint8_t>(), idx.as<int>(), num.as<int>(),
+    OCT_CK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, o.flag.as<uint8_t>(), o.idx.as<int>(), o.num.as<int>(),
                                          (int)d.count(), st));
-    OCT_CK(tmp.ensure(tb));
-    OCT_CK(hipcub::DeviceSelect::Flagged(tmp.p, tb, it, flag.as<uint8_t>(), idx.as<int>(), num.as<int>(),
+    OCT_CK(o.tmp.ensure(tb));
+    OCT_CK(hipcub::DeviceSelect::Flagged(o.tmp.p, tb, it, o.flag.as<uint8_t>(), o.idx.as<int>(), o.num.as<int>(),
                                          (int)d.count(), st));
     int nl = 0;
-    if (!select_count(nl)) return false;
+    if (!read_count(o, st, nl, err)) return false;
     if (nl == 0) continue;
     if (nleaves + nl > kMaxGrid / 8) {
       err = "octree: more than 2^28 leaves";
       return false;
     }
     OCT_CK(leaves.grow((size_t)(nleaves + nl) * sizeof(int4) * 2, (size_t)nleaves * sizeof(int4), st));
-    hipLaunchKernelGGL(leaf_emit_kernel, dim3(blocks(nl)), dim3(kOctBlock), 0, st, idx.as<int>(), nl, d, extent(L),
-                       leaves.as<int4>() + nleaves);
+    hipLaunchKernelGGL(leaf_emit_kernel, dim3(blocks(nl)), dim3(kOctBlock), 0, st, o.idx.as<int>(), nl, d,
+                       o.extent(L), leaves.as<int4>() + nleaves);
     OCT_CK(hipGetLastError());
     nleaves += nl;
     Lf = L;
   }
   out.levels = Lf + 1;
-  const int ef = extent(Lf);
-  const Dims df = dims(Lf);
+  const int ef = o.extent(Lf);
+  const Dims df = o.dims(Lf);
   const Dims m{df.x + 1, df.y + 1, df.z + 1};
   if (m.count() > kMaxGrid) {
     err = "octree: the node lattice at level " + std::to_string(Lf) + " exceeds 2^31 points";
@@ -401,36 +489,31 @@ bool build_octree(const SurfaceDev& s, const int grid[3], const double base[3], 
   OCT_CK(nflag.ensure((size_t)m.count()));
   OCT_CK(nid.ensure((size_t)m.count() * sizeof(int)));
   OCT_CK(hipMemsetAsync(nflag.p, 0, (size_t)m.count(), st));
-  hipLaunchKernelGGL(node_mark_kernel, dim3(blocks(nleaves)), dim3(kOctBlock), 0, st, leaves.as<int4>(), (int)nleaves,
-                     ef, m, nflag.as<uint8_t>());
+  hipLaunchKernelGGL(node_mark_kernel, dim3(blocks(nleaves)), dim3(kOctBlock), 0, st, leaves.as<int4>(),
+                     (int)nleaves, ef, m, nflag.as<uint8_t>());
   OCT_CK(hipGetLastError());
   {
     hipcub::TransformInputIterator<int, ToInt, const uint8_t*> in(nflag.as<uint8_t>(), ToInt());
     size_t tb = 0;
     OCT_CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, nid.as<int>(), (int)m.count(), st));
-    OCT_CK(tmp.ensure(tb));
-    OCT_CK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, in, nid.as<int>(), (int)m.count(), st));
+    OCT_CK(o.tmp.ensure(tb));
+    OCT_CK(hipcub::DeviceScan::ExclusiveSum(o.tmp.p, tb, in, nid.as<int>(), (int)m.count(), st));
   }
-  int last[2] = {0, 0};
-  {
-    uint8_t lf = 0;
-    OCT_CK(hipMemcpyAsync(&last[0], nid.as<int>() + (m.count() - 1), sizeof(int), hipMemcpyDeviceToHost, st));
-    OCT_CK(hipMemcpyAsync(&lf, nflag.as<uint8_t>() + (m.count() - 1), 1, hipMemcpyDeviceToHost, st));
-    OCT_CK(hipStreamSynchronize(st));
-    last[1] = lf;
-  }
-  const long long nnodes = (long long)last[0] + last[1];
+  int last = 0;
+  uint8_t lastf = 0;
+  OCT_CK(hipMemcpyAsync(&last, nid.as<int>() + (m.count() - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+  OCT_CK(hipMemcpyAsync(&lastf, nflag.as<uint8_t>() + (m.count() - 1), 1, hipMemcpyDeviceToHost, st));
+  OCT_CK(hipStreamSynchronize(st));
+  const long long nnodes = (long long)last + lastf;
   OCT_CK(dnodes.ensure((size_t)nnodes * 3 * sizeof(int)));
   OCT_CK(dV.ensure((size_t)nnodes * 3 * sizeof(double)));
   OCT_CK(dH.ensure((size_t)nleaves * 8 * sizeof(int)));
   hipLaunchKernelGGL(node_emit_kernel, dim3(blocks(m.count())), dim3(kOctBlock), 0, st, nflag.as<uint8_t>(),
-                     nid.as<int>(), m, m.count(), ef, base[0], base[1], base[2], voxel_size, dnodes.as<int>(),
+                     nid.as<int>(), m, m.count(), ef, o.base[0], o.base[1], o.base[2], o.vs, dnodes.as<int>(),
                      dV.as<double>());
   hipLaunchKernelGGL(hex_kernel, dim3(blocks(nleaves)), dim3(kOctBlock), 0, st, leaves.as<int4>(), (int)nleaves,
                      nid.as<int>(), m, ef, dH.as<int>());
   OCT_CK(hipGetLastError());
-  OCT_CK(hipEventRecord(ev1, st));
-
   out.leaves.resize((size_t)nleaves * 4);
   out.nodes.resize((size_t)nnodes * 3);
   out.V.resize((size_t)nnodes * 3);
@@ -440,6 +523,35 @@ bool build_octree(const SurfaceDev& s, const int grid[3], const double base[3], 
   OCT_CK(hipMemcpyAsync(out.V.data(), dV.p, out.V.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   OCT_CK(hipMemcpyAsync(out.hexes.data(), dH.p, out.hexes.size() * sizeof(int), hipMemcpyDeviceToHost, st));
   OCT_CK(hipStreamSynchronize(st));
+  return true;
+}
+
+// one timed subdivide + export (device time between events on the stream, host copies of the
+// results excluded)
+bool octree_timed(OctreeDev& o, const SurfaceDev* s, int stop_extent, int n, const int* cells, hipStream_t st,
+                  OctreeOut& out, std::string& err) {
+  hipEvent_t ev0, ev1;
+  OCT_CK(hipEventCreate(&ev0));
+  if (hipEventCreate(&ev1) != hipSuccess) {
+    (void)hipEventDestroy(ev0);
+    err = "octree: hipEventCreate failed";
+    return false;
+  }
+  struct EvGuard {
+    hipEvent_t a, b;
+    ~EvGuard() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } evg{ev0, ev1};
+  long evaluated = 0;
+  OCT_CK(hipEventRecord(ev0, st));
+  const bool ok = cells ? octree_refine(o, *s, stop_extent, n, cells, st, evaluated, err)
+                        : octree_subdivide(o, *s, stop_extent, st, evaluated, err);
+  if (!ok) return false;
+  OCT_CK(hipEventRecord(ev1, st));
+  out.evaluated = evaluated;
+  if (!octree_export(o, st, out, err)) return false;
   float ms = 0;
   OCT_CK(hipEventElapsedTime(&ms, ev0, ev1));
   out.ms = ms;

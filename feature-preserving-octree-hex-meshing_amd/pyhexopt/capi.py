@@ -24,7 +24,7 @@ EXPORTS = ["hexopt_create", "hexopt_create_error", "hexopt_destroy", "hexopt_las
            "hexopt_curves_create", "hexopt_curves_error", "hexopt_curves_destroy", "hexopt_project_lines",
            "hexopt_refresh_lines", "hexopt_metro_hausdorff", "hexopt_glue_setup", "hexopt_glue_solve",
            "hexopt_octree_build", "hexopt_octree_error", "hexopt_octree_destroy", "hexopt_octree_sizes",
-           "hexopt_octree_copy"]
+           "hexopt_octree_copy", "hexopt_octree_refine"]
 
 _LIB = None
 
@@ -97,6 +97,8 @@ def lib():
         L.hexopt_octree_sizes.restype = C.c_int
         L.hexopt_octree_copy.argtypes = [C.c_void_p] * 5
         L.hexopt_octree_copy.restype = C.c_int
+        L.hexopt_octree_refine.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.hexopt_octree_refine.restype = C.c_int
         L.hexopt_curves_create.restype = C.c_void_p
         L.hexopt_curves_create.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.hexopt_curves_error.restype = C.c_char_p
@@ -401,26 +403,11 @@ class Surface:
         """OctreeGrid(grid) + subdivide(should_subdivide, graded, paired) + octree_mesh's export:
         dict(leaves (n, 4), nodes (m, 3) integer positions, V (m, 3), H (n, 8), levels,
         evaluated, ms)."""
-        g = np.ascontiguousarray(grid, dtype=np.int32).reshape(3)
-        mt = None if mesh_transform is None else _d(mesh_transform)
-        o = lib().hexopt_octree_build(C.c_void_p(self.h), g.ctypes.data, _d(origin).ctypes.data, float(voxel_size),
-                                      None if mt is None else mt.ctypes.data, int(stop_extent), int(bool(graded)),
-                                      int(bool(paired)))
-        if not o:
-            raise RuntimeError("hexopt_octree_build: " + lib().hexopt_octree_error().decode())
+        o = Octree(self, grid, origin, voxel_size, mesh_transform, stop_extent, graded, paired)
         try:
-            sz = (C.c_int64 * 4)()
-            ms = C.c_double()
-            lib().hexopt_octree_sizes(o, sz, C.byref(ms))
-            n, m = int(sz[0]), int(sz[1])
-            leaves = np.empty((n, 4), np.int32)
-            nodes = np.empty((m, 3), np.int32)
-            V = np.empty((m, 3), np.float64)
-            H = np.empty((n, 8), np.int32)
-            lib().hexopt_octree_copy(o, leaves.ctypes.data, nodes.ctypes.data, V.ctypes.data, H.ctypes.data)
+            return o.result()
         finally:
-            lib().hexopt_octree_destroy(o)
-        return dict(leaves=leaves, nodes=nodes, V=V, H=H, levels=int(sz[2]), evaluated=int(sz[3]), ms=ms.value)
+            o.close()
 
     def voxel_parity(self, nx, ny, nz, origin, spacing):
         """Inside flags of voxel centres, shaped (nz, ny, nx)."""
@@ -455,6 +442,52 @@ class Curves:
     def close(self):
         if self.h:
             lib().hexopt_curves_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Octree:
+    """An octree built on a Surface's device (hexopt_octree_build); refine() runs the targeted
+    subdivision of listed cells (hexopt_octree_refine)."""
+
+    def __init__(self, surface, grid, origin, voxel_size, mesh_transform=None, stop_extent=1, graded=True,
+                 paired=True):
+        g = np.ascontiguousarray(grid, dtype=np.int32).reshape(3)
+        mt = None if mesh_transform is None else _d(mesh_transform)
+        self.h = lib().hexopt_octree_build(C.c_void_p(surface.h), g.ctypes.data, _d(origin).ctypes.data,
+                                           float(voxel_size), None if mt is None else mt.ctypes.data,
+                                           int(stop_extent), int(bool(graded)), int(bool(paired)))
+        if not self.h:
+            raise RuntimeError("hexopt_octree_build: " + lib().hexopt_octree_error().decode())
+
+    def refine(self, surface, stop_extent, cells):
+        c = _i(cells).reshape(-1, 4)
+        rc = lib().hexopt_octree_refine(C.c_void_p(self.h), C.c_void_p(surface.h), int(stop_extent), c.shape[0],
+                                        c.ctypes.data)
+        if rc:
+            raise RuntimeError("hexopt_octree_refine (%d): %s" % (rc, lib().hexopt_octree_error().decode()))
+
+    def result(self):
+        sz = (C.c_int64 * 4)()
+        ms = C.c_double()
+        lib().hexopt_octree_sizes(C.c_void_p(self.h), sz, C.byref(ms))
+        n, m = int(sz[0]), int(sz[1])
+        leaves = np.empty((n, 4), np.int32)
+        nodes = np.empty((m, 3), np.int32)
+        V = np.empty((m, 3), np.float64)
+        H = np.empty((n, 8), np.int32)
+        lib().hexopt_octree_copy(C.c_void_p(self.h), leaves.ctypes.data, nodes.ctypes.data, V.ctypes.data,
+                                 H.ctypes.data)
+        return dict(leaves=leaves, nodes=nodes, V=V, H=H, levels=int(sz[2]), evaluated=int(sz[3]), ms=ms.value)
+
+    def close(self):
+        if self.h:
+            lib().hexopt_octree_destroy(C.c_void_p(self.h))
             self.h = None
 
     def __del__(self):

@@ -265,8 +265,16 @@ hexopt_octree* hexopt_octree_build(hexopt_surface* s, const int32_t* grid3, cons
 const char* hexopt_octree_error(void);
 void hexopt_octree_destroy(hexopt_octree* o);
 /* out4 = { leaves, nodes, levels holding leaves, predicate evaluations }; *ms (nullable) the
- * device time of the build */
+ * device time of the last subdivision (build or refine) */
 int hexopt_octree_sizes(const hexopt_octree* o, int64_t* out4, double* ms);
+/* The targeted OctreeGrid::subdivide(should_subdivide, tb_subdivided_cells, graded, paired)
+ * (octree.cpp:691-729) that the mesher's refinement loop runs on its existing octree
+ * (grid_hex_meshing.cpp:518-521): each of the n listed cells (x, y, z, extent rows) that is a
+ * leaf and passes the predicate at this stop_extent is split, with its cascades; its children
+ * are not tested. Cells that are not leaves are skipped, as there; a row that is not a cell of
+ * the grid is an error (HEXOPT_EINVAL). The octree's leaves, nodes and hexes are then those of
+ * the refined tree (hexopt_octree_sizes / _copy). s must be on the octree's device. */
+int hexopt_octree_refine(hexopt_octree* o, hexopt_surface* s, int stop_extent, int n, const int32_t* cells);
 /* leaves[n][4] (x, y, z, extent), nodes[m][3] (integer positions), V[m][3]
  * (mesh_transform + origin + voxel_size * position, octree_mesh's mo.V) and H[n][8] (node
  * ids of each leaf's corners in the reference's corner order, octree_mesh's mo.Hs); any may

@@ -6,7 +6,9 @@
 //   octree_ref <in.bin> <out.bin>
 //
 // in.bin (little-endian): int32 gx, gy, gz, stop_extent, graded, paired, nV, nF; double
-// origin[3], voxel_size, mesh_transform[3]; double V[nV][3]; int32 F[nF][3].
+// origin[3], voxel_size, mesh_transform[3]; double V[nV][3]; int32 F[nF][3]; optionally int32
+// stop2, nList and int32 list[nList][4]: after the subdivision, the targeted subdivide of
+// octree.cpp:691-729 over the leaves at those (x, y, z, extent), with the predicate at stop2.
 // out.bin: int32 nLeaves, nNodes, is_graded, is_paired; int32 leaves[nLeaves][4] (x, y, z,
 // extent of each leaf, in cell order); int32 nodes[nNodes][3] (node positions, in node order);
 // int32 corners[nLeaves][8] (node ids of each leaf's corners, the hexes of octree_mesh).
@@ -66,6 +68,9 @@ int main(int argc, char** argv) {
     const int nV = hdr[6], nF = hdr[7];
     auto V = read_arr<double>(f, (size_t)nV * 3);
     auto F = read_arr<int32_t>(f, (size_t)nF * 3);
+    int32_t tail[2] = {0, 0};
+    const bool refine = fread(tail, sizeof(int32_t), 2, f) == 2;
+    auto list = refine ? read_arr<int32_t>(f, (size_t)tail[1] * 4) : std::vector<int32_t>();
     fclose(f);
     const int stop_extent = hdr[3];
     const bool graded = hdr[4] != 0, paired = hdr[5] != 0;
@@ -111,8 +116,9 @@ int main(int argc, char** argv) {
     }
     std::vector<int> stamp(nF, -1);
     int query = 0;
+    int stop = stop_extent;
     auto should_subdivide = [&](int x, int y, int z, int extent) {
-      if (extent <= stop_extent) return false;
+      if (extent <= stop) return false;
       const int p[3] = {x, y, z};
       double mn[3], mx[3];
       for (int d = 0; d < 3; ++d) {
@@ -155,6 +161,28 @@ int main(int argc, char** argv) {
         if (extent == 1) continue;
         if (octree.cellIsLeaf(id)) (octree.*split_cell_ptr())(id, graded, paired);
         for (int k = 0; k < 8; ++k) pending.push(octree.m_Cells[id].firstChild + k);
+      }
+    }
+    if (refine) {
+      // the targeted variant (octree.cpp:691-722): the listed leaves, no children queued
+      stop = tail[0];
+      std::vector<int> ids;
+      for (int i = 0; i < tail[1]; ++i)
+        for (int c = 0; c < octree.numCells(); ++c) {
+          if (!octree.cellIsLeaf(c) || octree.cellExtent(c) != list[4 * i + 3]) continue;
+          const Eigen::Vector3i p = octree.cellCornerPos(c, 0);
+          if (p[0] == list[4 * i] && p[1] == list[4 * i + 1] && p[2] == list[4 * i + 2]) ids.push_back(c);
+        }
+      std::queue<int> listed;
+      for (int c : ids)
+        if (octree.cellIsLeaf(c)) listed.push(c);
+      while (!listed.empty()) {
+        const int id = listed.front();
+        listed.pop();
+        const int extent = octree.cellExtent(id);
+        const Eigen::Vector3i pos = octree.cellCornerPos(id, 0);
+        if (should_subdivide(pos[0], pos[1], pos[2], extent) && extent != 1 && octree.cellIsLeaf(id))
+          (octree.*split_cell_ptr())(id, graded, paired);
       }
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -366,9 +366,26 @@ def octree_cases():
             out[key + "__nnodes"] = np.array(len(nd))
             out[key + "__flags"] = np.array([r["graded"], r["paired"]], np.int32)
             print(name, grid, g, p, len(lv), "leaves", len(nd), "nodes")
+    import hashlib
+    # the refinement loop's targeted subdivide: a seeded sample of the leaves (the hexes the
+    # Hausdorff check flags), refined at half the STOP extent
+    rng = np.random.default_rng(11)
+    for name in ("sphere64", "torus128"):
+        V, F, nvox, stop, _ = octree_surfaces()[name]
+        origin, vs, grid, mt = oc.mesher_frame(V, nvox)
+        lv = out["%s__g1p1__leaves" % name]
+        pick = lv[np.sort(rng.choice(len(lv), len(lv) // 10, replace=False))]
+        cells = np.concatenate([pick, pick[:5]])          # repeats are skipped once split
+        r = oc.ref_octree(V, F, grid, origin, vs, mt, stop, refine=(stop // 2, cells))
+        rl = np.ascontiguousarray(r["leaves"][np.lexsort(r["leaves"].T[::-1])], np.int32)
+        out["%s__refine__cells" % name] = cells
+        out["%s__refine__stop" % name] = np.array(stop // 2)
+        out["%s__refine__counts" % name] = np.array([len(rl), len(r["nodes"])])
+        out["%s__refine__sha256" % name] = np.array(hashlib.sha256(rl.tobytes()).hexdigest())
+        out["%s__refine__flags" % name] = np.array([r["graded"], r["paired"]], np.int32)
+        print(name, "refined", len(cells), "listed:", len(lv), "->", len(rl), "leaves")
     # full size: the clipped torus of the bench at num_voxels 4096, STOP extent 8 (about 3.7M
     # leaves); only the counts and a SHA-256 of the sorted leaf rows are kept
-    import hashlib
     V, F, _ = mg.clipped_torus_surface(1.0, 0.35, 0.2, nu=200, nband=30, nring=20)
     F = F.astype(np.int32)
     origin, vs, grid, mt = oc.mesher_frame(V, 4096)

@@ -17,9 +17,11 @@ def available():
     return os.path.exists(OCT_BIN)
 
 
-def ref_octree(V, F, grid, origin, voxel_size, mesh_transform, stop_extent, graded=True, paired=True):
+def ref_octree(V, F, grid, origin, voxel_size, mesh_transform, stop_extent, graded=True, paired=True,
+               refine=None):
     """dict(leaves (n, 4) x y z extent in cell order, nodes (m, 3) positions in node order,
-    corners (n, 8) node ids, graded, paired: the reference's own is2to1Graded / isPaired)."""
+    corners (n, 8) node ids, graded, paired: the reference's own is2to1Graded / isPaired).
+    refine = (stop2, cells (k, 4)): then the targeted subdivide of those leaves at stop2."""
     V = np.ascontiguousarray(V, np.float64)
     F = np.ascontiguousarray(F, np.int32)
     with tempfile.TemporaryDirectory() as td:
@@ -29,6 +31,10 @@ def ref_octree(V, F, grid, origin, voxel_size, mesh_transform, stop_extent, grad
             f.write(struct.pack("<7d", *origin, voxel_size, *mesh_transform))
             f.write(V.tobytes())
             f.write(F.tobytes())
+            if refine is not None:
+                cells = np.ascontiguousarray(refine[1], np.int32).reshape(-1, 4)
+                f.write(struct.pack("<2i", int(refine[0]), len(cells)))
+                f.write(cells.tobytes())
         subprocess.run([OCT_BIN, pin, pout], check=True, timeout=600)
         raw = open(pout, "rb").read()
     nl, nn, g, p = struct.unpack_from("<4i", raw, 0)

@@ -42,6 +42,51 @@ def test_leaf_set_matches_reference(name, mode):
     assert check_mesh(out, mt + origin, vs)
 
 
+@pytest.mark.parametrize("name", ["sphere64", "torus128"])
+def test_targeted_refinement_matches_reference(name):
+    """The refinement loop's targeted subdivide (octree.cpp:691-729) of a tenth of the leaves
+    at half the STOP extent, on the built octree: the reference's leaf set (SHA-256 of the
+    sorted rows) and node count; the refined tree is still graded and paired."""
+    import hashlib
+    from pyhexopt import capi
+    print("octree refine", name, flush=True)
+    V, F = FIX[name + "__V"], FIX[name + "__F"]
+    fr = FIX[name + "__frame"]
+    origin, vs, mt = fr[:3], float(fr[3]), fr[4:]
+    grid = FIX[name + "__grid"]
+    s = capi.Surface(V, F)
+    o = capi.Octree(s, grid, origin, vs, mt, int(FIX[name + "__stop"]))
+    try:
+        o.refine(s, int(FIX[name + "__refine__stop"]), FIX[name + "__refine__cells"])
+        out = o.result()
+    finally:
+        o.close()
+    counts = FIX[name + "__refine__counts"]
+    assert len(out["leaves"]) == counts[0] and len(out["nodes"]) == counts[1]
+    lv = np.ascontiguousarray(_sorted(out["leaves"]), np.int32)
+    assert hashlib.sha256(lv.tobytes()).hexdigest() == str(FIX[name + "__refine__sha256"])
+    assert check_octree(out["leaves"], grid)
+    assert check_mesh(out, mt + origin, vs)
+
+
+def test_refine_rejects_cells_off_the_grid():
+    from pyhexopt import capi
+    V, F = FIX["sphere64__V"], FIX["sphere64__F"]
+    fr = FIX["sphere64__frame"]
+    s = capi.Surface(V, F)
+    o = capi.Octree(s, FIX["sphere64__grid"], fr[:3], float(fr[3]), fr[4:], 2)
+    try:
+        before = o.result()["leaves"]
+        with pytest.raises(RuntimeError, match="not a cell"):
+            o.refine(s, 1, np.array([[3, 0, 0, 2]]))      # position not a multiple of the extent
+        with pytest.raises(RuntimeError, match="not a cell"):
+            o.refine(s, 1, np.array([[0, 0, 0, 3]]))      # extent not a power of two
+        o.refine(s, 1, np.zeros((0, 4), np.int32))        # nothing listed: the same tree
+        assert np.array_equal(_sorted(o.result()["leaves"]), _sorted(before))
+    finally:
+        o.close()
+
+
 def test_octree_mesh_feeds_the_optimiser_context():
     """octree_mesh's export is a valid hex mesh for hexopt_create (a uniform sphere octree)."""
     from pyhexopt import capi

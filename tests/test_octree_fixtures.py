@@ -20,6 +20,7 @@ sys.path.insert(0, os.path.join(HERE, "golden"))
 import octcases as oc  # noqa: E402
 
 FIX = np.load(os.path.join(HERE, "golden", "octree_cases.npz"))
+live = pytest.mark.skipif(not oc.available(), reason="oracle/_ref/octree_ref not built (reference tree absent)")
 
 
 def cases():
@@ -42,6 +43,24 @@ def test_reference_leaf_sets_have_the_octree_properties(name, mode):
     if p:
         assert flags[1] == 1
     assert check_octree(lv, FIX[name + "__grid"], graded=bool(g), paired=bool(p))
+
+
+@pytest.mark.parametrize("name", ["sphere64", "torus128"])
+def test_reference_refinement_stays_graded_and_paired(name):
+    assert list(FIX[name + "__refine__flags"]) == [1, 1]
+    assert FIX[name + "__refine__counts"][0] > len(FIX[name + "__g1p1__leaves"])
+
+
+@live
+def test_driver_reproduces_refinement_fixture():
+    import hashlib
+    name = "sphere64"
+    V, F = FIX[name + "__V"], FIX[name + "__F"]
+    fr = FIX[name + "__frame"]
+    r = oc.ref_octree(V, F, FIX[name + "__grid"].tolist(), fr[:3], fr[3], fr[4:], int(FIX[name + "__stop"]),
+                      refine=(int(FIX[name + "__refine__stop"]), FIX[name + "__refine__cells"]))
+    lv = np.ascontiguousarray(r["leaves"][np.lexsort(r["leaves"].T[::-1])], np.int32)
+    assert hashlib.sha256(lv.tobytes()).hexdigest() == str(FIX[name + "__refine__sha256"])
 
 
 def test_planted_split_breaks_pairing():
@@ -73,9 +92,6 @@ def test_planted_overlap_and_gap_break_the_tiling():
     dup = np.concatenate([lv, lv[:1]])
     with pytest.raises(AssertionError):
         check_octree(dup, grid, graded=False, paired=False)
-
-
-live = pytest.mark.skipif(not oc.available(), reason="oracle/_ref/octree_ref not built (reference tree absent)")
 
 
 @live
