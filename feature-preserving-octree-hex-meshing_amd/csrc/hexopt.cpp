@@ -782,7 +782,7 @@ static int cg32(hexopt_ctx* c, const SoftDev& sd, double prox, const float* b, f
                           P[k & 1], P[(k + 1) & 1], c->f_tmp.as<float>(), c->f_halo.as<float>(), c->f_invd.as<float>(),
                           x, r, z, c->d_cg.p, c->d_part[2].as<double>(), c->d_part[4].as<double>(),
                           c->d_part[5].as<double>(), c->d_part[3].as<double>(), c->d_part[6].as<double>(),
-                          c->d_part[7].as<double>(), st);
+                          c->d_part[7].as<double>(), k, k == iters - 1, st);
   HIPCHK(c, hipGetLastError());
   if (it_out || err_out) {
     std::vector<unsigned char> cgs(cg_state_bytes());

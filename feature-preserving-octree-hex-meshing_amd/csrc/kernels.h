@@ -121,7 +121,9 @@ void launch_cg_zero_if32(int n, void* state, float* x, int nb, hipStream_t st);
 // Fused FP32 CG (Eigen's algorithm, FP64 dot products) on the CG operator of launch_apply32:
 // launch_cg32_init sets x = 0, r = b, z = b / diag, p_prev = 0 and the state; each
 // launch_cg32_iteration is one CG iteration reading z and p_prev, writing p_new and the
-// updated x, r, z (zout may alias z). Every tile writes E p of each of its vertices as a float4
+// updated x, r, z (zout may alias z); k is the CG iteration and last marks the final one (the
+// operator pass of cube references forms the previous iteration's beta step, so the one-block
+// beta pass runs only after the last update). Every tile writes E p of each of its vertices as a float4
 // at lv_dst (halo holds one float4 per (tile, vertex) pair); the update sums vertex v's run
 // v_dst[v] .. v_dst[v+1]. iso lists the vertices without hexes.
 void launch_cg32_init(int nV, int nL, const SoftDev& sd, double prox, const float* b, const float* invd, float* x,
@@ -131,14 +133,15 @@ void launch_cg32_init(int nV, int nL, const SoftDev& sd, double prox, const floa
 // a_L rows of the line unknowns). halo holds a float4 per (tile, vertex) pair. Returns the number of
 // partials, positive when the pass also set p of the vertices without hexes (iso) and
 // negative when that is left to the caller.
+struct UpdParts;
 int launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const SoftDev& sd, const int* iso, int niso,
-                      const float* z, const float* pprev, float* pnew, float* out, float* halo, const void* state,
-                      double* part, hipStream_t st);
+                      const float* z, const float* pprev, float* pnew, float* out, float* halo, void* state,
+                      double* part, hipStream_t st, const UpdParts* fold = nullptr);
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
                            const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
                            double* part_tile, double* part_rr, double* part_rz, double* part_qz, double* part_bzq,
-                           double* part_qq, hipStream_t st);
+                           double* part_qq, int k, int last, hipStream_t st);
 void launch_residual64(int n, const float* b, const double* Ld, float* r32, double* part, int nb, double* out,
                        hipStream_t st);
 void launch_f2d(int n, const float* a, double* b, hipStream_t st);

@@ -873,6 +873,14 @@ struct CGState {
   double rhsNorm2, threshold, absNew, alpha, beta, rn2, tol_error;
   int done, iters, zero_rhs, pad;
   double qz, bzq, qq;  // FP32 CG: z.R z, z.R q, q.R q of the next direction z + beta q
+  double hist[2];      // FP32 CG: r.z after update k-1 in hist[k & 1] (the beta step's absOld)
+};
+
+// The update pass's five partial arrays (|r|^2, r.z, z.R z, z.R q, q.R q) of CG iteration k,
+// n per array; n == 0 when the operator pass does not form the beta step itself.
+struct UpdParts {
+  const double* p[5];
+  int n, k;
 };
 
 // r = rhs - tmp, partial |rhs|^2, |r|^2
@@ -1135,6 +1143,8 @@ __global__ void cg32_init_fin_kernel(CGState* st, const double* part_b, const do
     st->rn2 = b2;
     if (b2 < st->threshold) { st->done = 1; st->tol_error = 1; return; }
     st->absNew = rz;
+    st->hist[0] = rz;
+    st->hist[1] = 0;
   }
 }
 
@@ -1286,14 +1296,13 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_pipe_kernel(TileMesh tm, 
                                                                    float* __restrict__ pnew, float* __restrict__ out,
                                                                    float4* __restrict__ halo, SoftDev sd,
                                                                    const int* __restrict__ iso, int niso,
-                                                                   const CGState* st, double* partial) {
+                                                                   CGState* st, double* partial, UpdParts up) {
   __shared__ float4 sp[kMaxLV];
   __shared__ float4 acc[kMaxLV];
-  __shared__ double red[16];
+  __shared__ double red[5 * 16];
   if (st->done) return;
   const int tid = threadIdx.x;
   const int nV = tm.nV, nT = tm.nTiles;
-  const float beta = (float)st->beta;
   const int q8 = nT / 8, r8 = nT % 8;
   const int x = blockIdx.x % 8, rank = blockIdx.x / 8;
   const int nb = ((int)gridDim.x + 7 - x) / 8;  // blocks on XCD x
@@ -1301,6 +1310,37 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_pipe_kernel(TileMesh tm, 
   int tile = x * q8 + (x < r8 ? x : r8) + rank;
   PipeTile P;
   if (tile < tend) pipe_fetch(tm, K, tile, P);
+  // (the first tile's loads are in flight while the beta step is formed)
+  float beta;
+  if (up.n > 0) {
+    // the beta step of the previous CG iteration (cg32_beta_kernel's), formed by every block
+    // from the update pass's partials; block 0 records it in the state
+    const double* const parts[5] = {up.p[0], up.p[1], up.p[2], up.p[3], up.p[4]};
+    double v[5];
+    sum_parts_n<5>(parts, up.n, red, v);
+    const double rr = v[0], rz = v[1];
+    const bool conv = rr < st->threshold;
+    const double absOld = st->hist[up.k & 1];
+    if (blockIdx.x == 0 && tid == 0) {
+      st->rn2 = rr;
+      if (conv) {
+        st->done = 1;
+      } else {
+        st->absNew = rz;
+        st->hist[(up.k + 1) & 1] = rz;
+        st->beta = rz / absOld;
+        st->iters += 1;
+        st->qz = v[2];
+        st->bzq = v[3];
+        st->qq = v[4];
+      }
+      st->tol_error = sqrt(rr / st->rhsNorm2);
+    }
+    if (conv) return;
+    beta = (float)(rz / absOld);
+  } else {
+    beta = (float)st->beta;
+  }
   // the work outside the tiles, spread over all blocks: line unknowns, and p of the
   // vertices without hexes
   for (int i = blockIdx.x * kTileHex + tid; i < sd.nL; i += gridDim.x * kTileHex)
@@ -2215,13 +2255,15 @@ int pipe_grid(int ntiles) {
 }  // namespace
 
 int launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const SoftDev& sd, const int* iso, int niso,
-                      const float* z, const float* pprev, float* pnew, float* out, float* halo, const void* state,
-                      double* part, hipStream_t st) {
+                      const float* z, const float* pprev, float* pnew, float* out, float* halo, void* state,
+                      double* part, hipStream_t st, const UpdParts* fold) {
   float4* h4 = reinterpret_cast<float4*>(halo);
   if (ro.mode == REF_CUBE) {
     const int grid = pipe_grid(tm.nTiles);
+    UpdParts up{};
+    if (fold) up = *fold;
     hipLaunchKernelGGL(cg32_apply_pipe_kernel, dim3(grid), dim3(kTileHex), 0, st, tm, K, z, pprev, pnew, out, h4, sd,
-                       iso, niso, (const CGState*)state, part);
+                       iso, niso, (CGState*)state, part, up);
     return grid;
   }
   hipLaunchKernelGGL(cg32_apply_kernel<false>, dim3(tm.nTiles), dim3(kTileHex), 0, st, tm, ro, K, z, pprev, pnew, out,
@@ -2255,10 +2297,15 @@ void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, 
                            const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,
                            double* part_tile, double* part_rr, double* part_rz, double* part_qz, double* part_bzq,
-                           double* part_qq, hipStream_t st) {
+                           double* part_qq, int k, int last, hipStream_t st) {
   CGState* cs = (CGState*)state;
+  // the operator pass of cube references forms the previous iteration's beta step itself; the
+  // one-block beta pass then runs only after the last update
+  const bool fold_beta = ro.mode == REF_CUBE;
+  UpdParts up{{part_rr, part_rz, part_qz, part_bzq, part_qq}, update_grid(tm.nV), k - 1};
   // a positive count: the operator pass did the lines and the vertices without hexes
-  const int np = launch_cg32_apply(tm, ro, K, sd, iso, niso, z, pprev, pnew, out, halo, cs, part_tile, st);
+  const int np = launch_cg32_apply(tm, ro, K, sd, iso, niso, z, pprev, pnew, out, halo, cs, part_tile, st,
+                                   fold_beta && k > 0 ? &up : nullptr);
   const bool done_iso = np > 0;
   // alpha folds into the update pass unless the vertices without hexes still need p
   const bool fold = done_iso || niso == 0;
@@ -2269,8 +2316,10 @@ void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, 
   hipLaunchKernelGGL(cg32_update_kernel, dim3(nbu), dim3(256), 0, st, tm, sd, prox, (const float4*)halo,
                      (const float*)pnew, (const float*)out, invd, x, r, zout, (const CGState*)cs,
                      (const double*)part_tile, fold ? std::abs(np) : 0, part_rr, part_rz, part_qz, part_bzq, part_qq);
-  hipLaunchKernelGGL(cg32_beta_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_rr, (const double*)part_rz,
-                     (const double*)part_qz, (const double*)part_bzq, (const double*)part_qq, nbu);
+  if (!fold_beta || last)
+    hipLaunchKernelGGL(cg32_beta_kernel, dim3(1), dim3(1024), 0, st, cs, (const double*)part_rr,
+                       (const double*)part_rz, (const double*)part_qz, (const double*)part_bzq,
+                       (const double*)part_qq, nbu);
 }
 
 void launch_max_step(const double* cur, const double* dst, int nV, const int* conn, int nH, double* part,
