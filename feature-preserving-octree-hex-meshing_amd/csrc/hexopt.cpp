@@ -930,9 +930,14 @@ int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch
                      c->d_halo.as<double>(), c->d_part[2].as<double>(), nullptr, nullptr, st);
         break;
       case 1:
-        (void)launch_cg32_apply(c->tm, ro, c->f_K.as<float>(), sd, c->d_iso.as<int>(), c->n_iso, c->f_z.as<float>(),
-                                c->f_p.as<float>(), c->f_p2.as<float>(), c->f_tmp.as<float>(), c->f_halo.as<float>(),
-                                c->d_cg.p, c->d_part[2].as<double>(), st);
+        // as inside the CG loop after its first iteration: the previous beta step formed from
+        // the update partials of the last solve (a zero threshold keeps every pass running)
+        (void)launch_cg32_apply_folded(c->tm, ro, c->f_K.as<float>(), sd, c->d_iso.as<int>(), c->n_iso,
+                                       c->f_z.as<float>(), c->f_p.as<float>(), c->f_p2.as<float>(),
+                                       c->f_tmp.as<float>(), c->f_halo.as<float>(), c->d_cg.p,
+                                       c->d_part[2].as<double>(), c->d_part[4].as<double>(),
+                                       c->d_part[5].as<double>(), c->d_part[3].as<double>(),
+                                       c->d_part[6].as<double>(), c->d_part[7].as<double>(), 0, st);
         break;
       case 2:
         launch_local_step(tm, ro, c->d_cur.as<double>(), c->d_K.as<double>(), HEXOPT_SYMMETRIC_DIRICHLET, 1.0, 1.0,
@@ -950,7 +955,7 @@ int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch
     HIPCHK(c, hipMemsetAsync(c->d_cg.p, 0, cg_state_bytes(), st));
     launch_cg32_init(c->L.nV, c->nL, sd, 1e-5, c->f_rhs.as<float>(), c->f_invd.as<float>(), c->f_d.as<float>(),
                      c->f_r.as<float>(), c->f_z.as<float>(), c->f_p.as<float>(), c->d_cg.p, c->d_part[0].as<double>(),
-                     c->d_part[1].as<double>(), c->d_part[3].as<double>(), 1e-8, st);
+                     c->d_part[1].as<double>(), c->d_part[3].as<double>(), 0.0, st);
   }
   one();  // warm
   HIPCHK(c, hipEventRecord(e0, st));

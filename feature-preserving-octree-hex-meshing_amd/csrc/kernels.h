@@ -137,6 +137,14 @@ struct UpdParts;
 int launch_cg32_apply(const TileMesh& tm, const RefOp& ro, const float* K, const SoftDev& sd, const int* iso, int niso,
                       const float* z, const float* pprev, float* pnew, float* out, float* halo, void* state,
                       double* part, hipStream_t st, const UpdParts* fold = nullptr);
+// The operator pass as launch_cg32_iteration runs it from the second CG iteration on: every
+// block first forms CG iteration k's beta step from the update pass's partials (the five
+// arrays, one entry per resident update block).
+int launch_cg32_apply_folded(const TileMesh& tm, const RefOp& ro, const float* K, const SoftDev& sd, const int* iso,
+                             int niso, const float* z, const float* pprev, float* pnew, float* out, float* halo,
+                             void* state, double* part, const double* part_rr, const double* part_rz,
+                             const double* part_qz, const double* part_bzq, const double* part_qq, int k,
+                             hipStream_t st);
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
                            const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,

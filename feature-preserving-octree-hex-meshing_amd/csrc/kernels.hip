@@ -2293,6 +2293,15 @@ static int update_grid(int nV) {
   return std::min(want, fit[dev]);
 }
 
+int launch_cg32_apply_folded(const TileMesh& tm, const RefOp& ro, const float* K, const SoftDev& sd, const int* iso,
+                             int niso, const float* z, const float* pprev, float* pnew, float* out, float* halo,
+                             void* state, double* part, const double* part_rr, const double* part_rz,
+                             const double* part_qz, const double* part_bzq, const double* part_qq, int k,
+                             hipStream_t st) {
+  UpdParts up{{part_rr, part_rz, part_qz, part_bzq, part_qq}, update_grid(tm.nV), k};
+  return launch_cg32_apply(tm, ro, K, sd, iso, niso, z, pprev, pnew, out, halo, state, part, st, &up);
+}
+
 void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, double prox, const SoftDev& sd,
                            const int* iso, int niso, const float* z, const float* pprev, float* pnew,
                            float* out, float* halo, const float* invd, float* x, float* r, float* zout, void* state,

@@ -158,7 +158,8 @@ int hexopt_opt32_solve(hexopt_ctx* ctx, const hexopt_params* prm, const hexopt_o
                        hexopt_opt32_log* log, double* E);
 
 /* Average duration (ms, HIP events on the context's stream) of one launch of a kernel:
- * 0 FP64 CG operator tile pass, 1 FP32 fused CG operator pass (cg32_apply), 2 FP64 local step
+ * 0 FP64 CG operator tile pass, 1 FP32 fused CG operator pass (cg32_apply, forming the previous
+ * iteration's beta step as it does inside the CG loop), 2 FP64 local step
  * tile pass, 3 FP32 adjoint/local tile pass. Needs a prior solve of the same precision
  * (operands resident). */
 int hexopt_time_kernel(hexopt_ctx* ctx, int which, int reps, double* ms_per_launch);
