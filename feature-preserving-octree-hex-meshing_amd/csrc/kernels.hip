@@ -919,6 +919,34 @@ __device__ void sum_parts_n(const double* const (&p)[N], int n, double* red, dou
   block_sum_n<N>(out, red);
 }
 
+// sum_parts_n with the first U * blockDim partials of every array loaded before any is added
+// (one round trip instead of U); the additions keep sum_parts_n's order, so the sums are the
+// same
+template <int N, int U>
+__device__ void sum_parts_n_wide(const double* const (&p)[N], int n, double* red, double out[N]) {
+  double v[U][N];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    const int i = threadIdx.x + j * (int)blockDim.x;
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[j][k] = i < n ? p[k][i] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) out[k] = 0;
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    const int i = threadIdx.x + j * (int)blockDim.x;
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+      if (i < n) out[k] += v[j][k];
+  }
+  for (int i = threadIdx.x + U * (int)blockDim.x; i < n; i += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] += p[k][i];
+  }
+  block_sum_n<N>(out, red);
+}
+
 // Eigen ConjugateGradient.h:44-62 (zero rhs, early exit) and the first direction.
 __global__ void cg_init_fin_kernel(CGState* st, const double* part_b, int nb, const double* part_r, double tol) {
   __shared__ double red[16];
@@ -1317,7 +1345,7 @@ __global__ __launch_bounds__(kTileHex) void cg32_apply_pipe_kernel(TileMesh tm, 
     // from the update pass's partials; block 0 records it in the state
     const double* const parts[5] = {up.p[0], up.p[1], up.p[2], up.p[3], up.p[4]};
     double v[5];
-    sum_parts_n<5>(parts, up.n, red, v);
+    sum_parts_n_wide<5, 6>(parts, up.n, red, v);
     const double rr = v[0], rz = v[1];
     const bool conv = rr < st->threshold;
     const double absOld = st->hist[up.k & 1];
