@@ -1496,7 +1496,10 @@ __global__ __launch_bounds__(256) void cg32_update_kernel(TileMesh tm, SoftDev s
   const size_t nA = 3 * (size_t)nV;
   float alpha;
   if (ntiles > 0) {
-    const double e = sum_parts(part_tile, ntiles, red);
+    const double* const pt[1] = {part_tile};
+    double ev[1];
+    sum_parts_n_wide<1, 4>(pt, ntiles, red, ev);
+    const double e = ev[0];
     const double b = (double)(float)st->beta;
     alpha = (float)(st->absNew / (e + st->qz + 2.0 * b * st->bzq + b * b * st->qq));
   } else {
