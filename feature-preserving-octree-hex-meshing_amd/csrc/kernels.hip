@@ -279,9 +279,16 @@ __device__ __forceinline__ void scatter_tile4(const TileMesh& tm, int tile, int 
 //                                                       the hex's share of an optional sum
 //   __device__ double finish(int v, T acc[NC])        writes vertex v; returns its share of
 //                                                       the kernel's optional vertex sum
+// minimum waves per SIMD the register allocation of a tile pass must leave room for (1: no
+// constraint)
+template <class Op>
+struct TileWaves {
+  static constexpr int value = 1;
+};
+
 template <class T, int NC, class Op>
-__global__ __launch_bounds__(kTileHex) void tile_scatter_kernel(TileMesh tm, Op op, T* halo, double* partial,
-                                                                 double* hex_partial) {
+__global__ __launch_bounds__(kTileHex) __attribute__((amdgpu_waves_per_eu(TileWaves<Op>::value)))
+void tile_scatter_kernel(TileMesh tm, Op op, T* halo, double* partial, double* hex_partial) {
   __shared__ T acc[NC * kMaxLV];
   __shared__ double red[16];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -637,6 +644,13 @@ struct LocalStepOp {
     }
     return 0.0;
   }
+};
+
+// the FP64 local step needs all 256 registers on its own (one wave per SIMD); two waves per
+// SIMD hide its FP64 latency better than the registers it then spills cost
+template <>
+struct TileWaves<LocalStepOp> {
+  static constexpr int value = 2;
 };
 
 // CG operator L p = A^T M A p + prox p + soft terms; finish() also returns p.(Lp).
