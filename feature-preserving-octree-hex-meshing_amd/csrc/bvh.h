@@ -42,6 +42,13 @@ struct SurfaceDev {
 
 constexpr int kBvhBlock = 128;
 constexpr int kBvhMaxDepth = 48;
+// traversal stack entries per lane for a tree of `depth` levels (root = level 1): a
+// depth-first walk holds at most one frame per level, plus one spare; never above
+// kBvhMaxDepth + 2 (surfaces with deeper trees are refused at creation)
+__host__ __device__ inline int stack_frames(int depth) {
+  const int d = depth < 1 ? 1 : depth;
+  return d + 2 < kBvhMaxDepth + 2 ? d + 2 : kBvhMaxDepth + 2;
+}
 
 // Queries: P (SoA x[d*stride + i] when soa, else nq*3 rows), optional index list `ids`
 // (query i uses point ids[i]). Outputs (nullable): S, I, C (nq*3), N (nq*3), sqrD. status:

@@ -91,7 +91,10 @@ __device__ __forceinline__ unsigned fr_make(int n, unsigned st, unsigned seen) {
 __global__ __launch_bounds__(kBvhBlock) void closest_kernel(SurfaceDev s, int nq, const double* P, int soa, int stride,
                                                             const int* ids, const int* order, double* S, int* I,
                                                             double* Cout, double* Nout, double* sqrDout, int* status) {
-  __shared__ unsigned stack[kBvhMaxDepth * kBvhBlock];
+  // one frame per tree level: (depth + 1) x kBvhBlock entries of dynamic LDS (see
+  // launch_closest), so shallow trees leave room for more resident blocks
+  extern __shared__ unsigned stack[];
+  const int cap = stack_frames(s.depth);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nq) return;
   const int q = order ? order[k] : k;
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(kBvhBlock) void closest_kernel(SurfaceDev s, int nq
     }
     if (child < 0) { --top; continue; }
     stk[(top - 1) * kBvhBlock] = fr_make(n, state, seen);
-    if (top >= kBvhMaxDepth) { overflow = true; break; }
+    if (top >= cap) { overflow = true; break; }
     stk[top * kBvhBlock] = fr_make(child, 0, 0);
     ++top;
   }
@@ -320,8 +323,9 @@ void launch_metro_reduce(int n, const double* sqrD, double upper, double* part, 
 void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int stride, const int* ids, double* S, int* I,
                     double* C, double* N, double* sqrD, int* status, hipStream_t st, const int* order) {
   if (nq <= 0) return;
-  hipLaunchKernelGGL(closest_kernel, dim3((nq + kBvhBlock - 1) / kBvhBlock), dim3(kBvhBlock), 0, st, s, nq, P, soa,
-                     stride, ids, order, S, I, C, N, sqrD, status);
+  const size_t lds = (size_t)stack_frames(s.depth) * kBvhBlock * sizeof(unsigned);
+  hipLaunchKernelGGL(closest_kernel, dim3((nq + kBvhBlock - 1) / kBvhBlock), dim3(kBvhBlock), lds, st, s, nq, P,
+                     soa, stride, ids, order, S, I, C, N, sqrD, status);
 }
 
 }  // namespace hexopt

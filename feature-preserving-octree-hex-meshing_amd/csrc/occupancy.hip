@@ -47,7 +47,10 @@ __device__ __forceinline__ bool overlap_closed(const double* nb, const double mn
 __global__ __launch_bounds__(kOccBlock) void occ_classify_kernel(SurfaceDev s, int n, const int* __restrict__ cells,
                                                                   double b0, double b1, double b2, double vs,
                                                                   int stop_extent, uint8_t* __restrict__ split) {
-  __shared__ int stack[kBvhMaxDepth + 2][kOccBlock];
+  // depth-first, at most one pending sibling per level plus the node being expanded:
+  // stack_frames(depth) x kOccBlock entries of dynamic LDS (see launch_occ_classify)
+  extern __shared__ int stack_mem[];
+  int (*stack)[kOccBlock] = reinterpret_cast<int (*)[kOccBlock]>(stack_mem);
   const int i = blockIdx.x * kOccBlock + threadIdx.x;
   if (i >= n) return;
   const int x = cells[4 * i], y = cells[4 * i + 1], z = cells[4 * i + 2], ext = cells[4 * i + 3];
@@ -247,7 +250,8 @@ __global__ void hex_centre_kernel(const double* __restrict__ X, int nV, const in
 void launch_occ_classify(const SurfaceDev& s, int n, const int* cells, const double base[3], double voxel_size,
                          int stop_extent, uint8_t* split, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(occ_classify_kernel, dim3((n + kOccBlock - 1) / kOccBlock), dim3(kOccBlock), 0, st, s, n, cells,
+  const size_t lds = (size_t)stack_frames(s.depth) * kOccBlock * sizeof(int);
+  hipLaunchKernelGGL(occ_classify_kernel, dim3((n + kOccBlock - 1) / kOccBlock), dim3(kOccBlock), lds, st, s, n, cells,
                      base[0], base[1], base[2], voxel_size, stop_extent, split);
 }
 
