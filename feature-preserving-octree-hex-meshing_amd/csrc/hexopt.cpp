@@ -1126,11 +1126,9 @@ int hexopt_project_planes(hexopt_ctx* c, hexopt_surface* s, double lamda_T) {
   launch_refold_planes(c->sd, nT, s->dC.as<double>(), const_cast<double*>(c->sd.t_d), c->d_S0.as<double>(),
                        c->d_b0.as<double>(), const_cast<double*>(c->sd.S), const_cast<double*>(c->sd.b), lamda_T, st);
   c->sd.lamT = lamda_T;
-  int stat = 0;
-  HIPCHK(c, hipMemcpyAsync(&stat, s->dstat.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  // no host round trip: the walk's stack holds the tree's depth plus two frames, and trees
+  // deeper than the stack limit are refused at creation, so its overflow flag stays clear
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipStreamSynchronize(st));
-  if (stat) return fail(c, HEXOPT_EUNSUPPORTED, "traversal stack overflow");
   return HEXOPT_OK;
 }
 
@@ -1329,6 +1327,7 @@ struct hexopt_curves {
   int nv = 0, ncurves = 0;
   std::vector<int> off;
   DevBuf dV, doff, dvs, dcirc, drows, dcid, dorig, daxis;
+  std::vector<int32_t> cid_host;  // what dcid holds (empty: unknown)
   std::string err;
 };
 
@@ -1402,6 +1401,7 @@ int hexopt_project_lines(hexopt_ctx* c, hexopt_curves* cv, int which, int nL, co
   HIPCHK(c, cv->daxis.ensure((size_t)nL * 24));
   HIPCHK(c, hipMemcpyAsync(cv->drows.p, rows.data(), (size_t)nL * 4, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(cv->dcid.p, curve_L, (size_t)nL * 4, hipMemcpyHostToDevice, st));
+  cv->cid_host.clear();
   CurvesDev view{cv->dV.as<double>(), cv->doff.as<int>(), cv->dvs.as<int>(), cv->dcirc.as<uint8_t>()};
   launch_project_lines(view, (which == HEXOPT_REST ? c->d_rest : c->d_cur).as<double>(), c->L.nV, nL,
                        cv->drows.as<int>(), cv->dcid.as<int>(), cv->dorig.as<double>(), cv->daxis.as<double>(), st);
@@ -1422,10 +1422,18 @@ int hexopt_refresh_lines(hexopt_ctx* c, hexopt_curves* cv, const int32_t* curve_
   for (int i = 0; i < nL; ++i)
     if (curve_L[i] < 0 || curve_L[i] >= cv->ncurves) return fail(c, HEXOPT_EINVAL, "curve_L out of range");
   hipStream_t st = c->stream;
+  void* const cid_before = cv->dcid.p;
   HIPCHK(c, cv->dcid.ensure((size_t)nL * 4));
+  if (cv->dcid.p != cid_before) cv->cid_host.clear();  // a new buffer holds nothing yet
   HIPCHK(c, cv->dorig.ensure((size_t)nL * 24));
   HIPCHK(c, cv->daxis.ensure((size_t)nL * 24));
-  HIPCHK(c, hipMemcpyAsync(cv->dcid.p, curve_L, (size_t)nL * 4, hipMemcpyHostToDevice, st));
+  // the curve of each line vertex, uploaded only when it changed (the feature-alignment loop
+  // passes the same list every iteration), so the refresh needs no host round trip
+  if (cv->cid_host.size() != (size_t)nL || !std::equal(curve_L, curve_L + nL, cv->cid_host.begin())) {
+    HIPCHK(c, hipStreamSynchronize(st));
+    HIPCHK(c, hipMemcpy(cv->dcid.p, curve_L, (size_t)nL * 4, hipMemcpyHostToDevice));
+    cv->cid_host.assign(curve_L, curve_L + nL);
+  }
   CurvesDev view{cv->dV.as<double>(), cv->doff.as<int>(), cv->dvs.as<int>(), cv->dcirc.as<uint8_t>()};
   launch_project_lines(view, c->d_cur.as<double>(), c->L.nV, nL, c->sd.lv, cv->dcid.as<int>(), cv->dorig.as<double>(),
                        cv->daxis.as<double>(), st);
@@ -1433,7 +1441,6 @@ int hexopt_refresh_lines(hexopt_ctx* c, hexopt_curves* cv, const int32_t* curve_
   launch_refold_lines(c->sd, cv->dorig.as<double>(), cv->daxis.as<double>(), c->d_S0.as<double>(),
                       c->d_b0.as<double>(), c->d_rhs_a.as<double>(), c->d_invd_a.as<double>(), st);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipStreamSynchronize(st));
   return HEXOPT_OK;
 }
 
