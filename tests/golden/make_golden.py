@@ -384,6 +384,15 @@ def octree_cases():
         out["%s__refine__sha256" % name] = np.array(hashlib.sha256(rl.tobytes()).hexdigest())
         out["%s__refine__flags" % name] = np.array([r["graded"], r["paired"]], np.int32)
         print(name, "refined", len(cells), "listed:", len(lv), "->", len(rl), "leaves")
+    # compute_octree (voxelization.cpp:353-391): no mesh_transform, cells of extent 1 never
+    # split, graded and paired; the sphere at spacing = extent / 64
+    V, F, nvox, _, _ = octree_surfaces()["sphere64"]
+    origin, vs, grid, _ = oc.mesher_frame(V, nvox)
+    r = oc.ref_octree(V, F, grid, origin, vs, np.zeros(3), 1)
+    rl = np.ascontiguousarray(r["leaves"][np.lexsort(r["leaves"].T[::-1])], np.int32)
+    out["compute_octree__counts"] = np.array([len(rl), len(r["nodes"]), r["graded"], r["paired"]])
+    out["compute_octree__sha256"] = np.array(hashlib.sha256(rl.tobytes()).hexdigest())
+    print("compute_octree", grid, len(rl), "leaves")
     # full size: the clipped torus of the bench at num_voxels 4096, STOP extent 8 (about 3.7M
     # leaves); only the counts and a SHA-256 of the sorted leaf rows are kept
     V, F, _ = mg.clipped_torus_surface(1.0, 0.35, 0.2, nu=200, nband=30, nring=20)

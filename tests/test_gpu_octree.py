@@ -87,6 +87,28 @@ def test_refine_rejects_cells_off_the_grid():
         o.close()
 
 
+def test_compute_octree_composition():
+    """voxelization.cpp's compute_octree (:353-391) is hexopt_octree_build without
+    mesh_transform at stop extent 1, then compute_sign on the leaves (hexopt_occ_cell_parity):
+    the reference's leaf set (SHA-256) and the oracle's inside flags of those leaves."""
+    import hashlib
+    import oracle_py as O
+    from pyhexopt import capi
+    V, F = FIX["sphere64__V"], FIX["sphere64__F"]
+    fr = FIX["sphere64__frame"]
+    origin, vs = fr[:3], float(fr[3])
+    s = capi.Surface(V, F)
+    out = s.octree(FIX["sphere64__grid"], origin, vs, None, 1)
+    counts = FIX["compute_octree__counts"]
+    assert len(out["leaves"]) == counts[0] and len(out["nodes"]) == counts[1]
+    lv = np.ascontiguousarray(_sorted(out["leaves"]), np.int32)
+    assert hashlib.sha256(lv.tobytes()).hexdigest() == str(FIX["compute_octree__sha256"])
+    pick = out["leaves"][:: max(1, len(out["leaves"]) // 4000)]
+    got = s.cell_parity(pick, origin, vs)
+    assert np.array_equal(got, O.cell_parity(V, F, pick, origin, vs))
+    assert 0 < got.sum() < len(got)
+
+
 def test_octree_mesh_feeds_the_optimiser_context():
     """octree_mesh's export is a valid hex mesh for hexopt_create (a uniform sphere octree)."""
     from pyhexopt import capi
