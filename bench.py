@@ -171,6 +171,23 @@ def octree_measure(reps=5):
             "ms_build": round(best, 3)}
 
 
+def metro_measure(target):
+    """Metro Hausdorff (hexopt_metro_hausdorff: vertex sampling both ways, closest points on
+    the device) between the 40,000-triangle clipped torus and the boundary of its hex mesh
+    (each boundary quad as two triangles), as the feature-alignment loop's Hausdorff check
+    measures it. Wall time of the whole call: both surfaces' trees are built on the host."""
+    from pyhexopt import capi, meshgen as mg
+    V, H, _ = mg.torus_hex_mesh(target, R_MAJ, R_MIN, Z_CUT)
+    T = mg.hex_boundary_triangles(H).astype(np.int32)
+    SV, SF, _ = mg.clipped_torus_surface(R_MAJ, R_MIN, Z_CUT, nu=200, nband=30, nring=20)
+    capi.metro_hausdorff(SV, SF, V, T)    # warm-up (device code loaded)
+    t0 = time.perf_counter()
+    diag, mx, mean = capi.metro_hausdorff(SV, SF, V, T)
+    secs = time.perf_counter() - t0
+    return {"surface_triangles": int(SF.shape[0]), "hex_boundary_triangles": int(T.shape[0]),
+            "bbox_diagonal": diag, "max": mx, "mean": mean, "ms": round(1e3 * secs, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +197,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-glue", action="store_true")
     ap.add_argument("--no-octree", action="store_true")
+    ap.add_argument("--no-metro", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see the design notes on "
@@ -261,6 +279,8 @@ def main():
         out["padding_untangle"] = glue_measure(max(args.hexes // 8, 20000), 5)
     if not args.no_octree:
         out["octree_build"] = octree_measure()
+    if not args.no_metro:
+        out["metro_hausdorff"] = metro_measure(args.hexes)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(max(args.hexes // 8, 20000))
     print(json.dumps(out))
