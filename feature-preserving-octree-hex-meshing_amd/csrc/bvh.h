@@ -58,9 +58,20 @@ __host__ __device__ inline int stack_frames(int depth) {
 void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int stride, const int* ids, double* S, int* I,
                     double* C, double* N, double* sqrD, int* status, hipStream_t st, const int* order = nullptr);
 
-// (max, sum, count) of sqrt(sqrD[i]) over the queries below `upper`, into out3; part needs
-// 3 * 1024 doubles.
-void launch_metro_reduce(int n, const double* sqrD, double upper, double* part, double* out3, hipStream_t st);
+// Per-face edge-plane data of vcg's tri::UpdateEdges<>::Set (vcg/complex/algorithms/update/
+// component_ep.h:48-73), kEPRec doubles per face: the three edges scaled by 1 / the normal's
+// dominant component (9), the twice-normalised normal (3), the plane offset (1), the dominant
+// axis 0/1/2 (1), padding (2). Metro's point-to-face distance (PointDistanceEP) reads them.
+constexpr int kEPRec = 16;
+void build_ep_faces(int nv, const double* V, int nf, const int* F, std::vector<double>& ep);
+// Metro's sampling distance of each query (vcg Sampling::AddSample, extern/vcg/sampling.h:
+// 224-258): min(upper, min over faces of PointDistanceEP), through the surface tree. dist[i]
+// == upper when no face is closer.
+void launch_metro_ep(const SurfaceDev& s, const double* ep, int nq, const double* P, double upper, double* dist,
+                     int* status, hipStream_t st);
+// (max, sum, count) of the distances below `upper`, summed one by one in query order as
+// AddSample accumulates them, into out3 (one thread: the order fixes the rounding)
+void launch_metro_seq(int n, const double* dist, double upper, double* out3, hipStream_t st);
 
 struct SoftDev;
 void launch_refold_planes(const SoftDev& sd, int nT, const double* Cp, double* t_d, const double* S0, const double* b0,

@@ -290,34 +290,131 @@ void launch_refold_planes(const SoftDev& sd, int nT, const double* Cp, double* t
 // Vertex-sampled metro statistics of per-query squared distances: a query counts when its
 // distance is below `upper`; per block (max, sum, count), then one block folds the blocks
 // in order.
-__global__ void metro_reduce_kernel(int n, const double* __restrict__ sqrD, double upper, double* part) {
-  __shared__ double smax[256], ssum[256], scnt[256];
-  const int t = threadIdx.x;
-  double mx = -HUGE_VAL, sm = 0, cn = 0;
-  for (int i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x) {
-    const double d = sqrt(sqrD[i]);
-    if (d < upper) { mx = fmax(mx, d); sm += d; cn += 1; }
-  }
-  smax[t] = mx; ssum[t] = sm; scnt[t] = cn;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (t < o) { smax[t] = fmax(smax[t], smax[t + o]); ssum[t] += ssum[t + o]; scnt[t] += scnt[t + o]; }
-    __syncthreads();
-  }
-  if (t == 0) { part[3 * blockIdx.x] = smax[0]; part[3 * blockIdx.x + 1] = ssum[0]; part[3 * blockIdx.x + 2] = scnt[0]; }
+// vcg PSDist (vcg/space/deprecated_point3.h:553-565): distance from p to the segment v1 v2
+__device__ __forceinline__ double ps_dist(const double p[3], const double v1[3], const double v2[3]) {
+  const double e[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+  double t = ((p[0] - v1[0]) * e[0] + (p[1] - v1[1]) * e[1] + (p[2] - v1[2]) * e[2]) /
+             (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  if (t < 0) t = 0;
+  else if (t > 1) t = 1;
+  const double q[3] = {v1[0] + e[0] * t, v1[1] + e[1] * t, v1[2] + e[2] * t};
+  const double d[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+  return sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
 }
 
-__global__ void metro_fin_kernel(int nb, const double* part, double* out) {
-  if (threadIdx.x != 0) return;
-  double mx = -HUGE_VAL, sm = 0, cn = 0;
-  for (int b = 0; b < nb; ++b) { mx = fmax(mx, part[3 * b]); sm += part[3 * b + 1]; cn += part[3 * b + 2]; }
-  out[0] = mx; out[1] = sm; out[2] = cn;
+// vcg face::PointDistanceEP (vcg/simplex/face/distance.h:48-180) for the face with corners
+// a, b, c and edge-plane record r (build_ep_faces): lowers dist to the face's distance when
+// that is smaller. The corner differences and the inside tests keep the reference's operand
+// order; the three axis cases differ only in which two coordinates the edge tests use.
+__device__ __forceinline__ void point_distance_ep(const double* __restrict__ r, const double a[3], const double b[3],
+                                                  const double c[3], const double q[3], double& dist) {
+  const double* E0 = r;
+  const double* E1 = r + 3;
+  const double* E2 = r + 6;
+  const double* n = r + 9;
+  const double d = (n[0] * q[0] + n[1] * q[1] + n[2] * q[2]) - r[12];
+  if (d > dist || d < -dist) return;
+  const double p[3] = {q[0] - n[0] * d, q[1] - n[1] * d, q[2] - n[2] * d};
+  const int axis = (int)r[13];
+  // (u, v): the two coordinates of the projection, as in the NORMX / NORMY / NORMZ cases
+  const int u = axis == 0 ? 1 : axis == 1 ? 2 : 0;
+  const int v = axis == 0 ? 2 : axis == 1 ? 0 : 1;
+  const double b0 = E1[u] * (p[v] - b[v]) - E1[v] * (p[u] - b[u]);
+  if (b0 <= 0) {
+    const double bt = ps_dist(q, b, c);
+    if (dist > bt) dist = bt;
+    return;
+  }
+  const double b1 = E2[u] * (p[v] - c[v]) - E2[v] * (p[u] - c[u]);
+  if (b1 <= 0) {
+    const double bt = ps_dist(q, c, a);
+    if (dist > bt) dist = bt;
+    return;
+  }
+  const double b2 = E0[u] * (p[v] - a[v]) - E0[v] * (p[u] - a[u]);
+  if (b2 <= 0) {
+    const double bt = ps_dist(q, a, b);
+    if (dist > bt) dist = bt;
+    return;
+  }
+  const double bm = b0 < b1 ? (b0 < b2 ? b0 : b2) : (b1 < b2 ? b1 : b2);
+  const double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  const double x[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+  const double area2 = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  if (bm < 0.000001 * area2) {
+    const double bt = bm == b0 ? ps_dist(q, b, c) : bm == b1 ? ps_dist(q, c, a) : ps_dist(q, a, b);
+    if (dist > bt) dist = bt;
+    return;
+  }
+  dist = fabs(d);
 }
 
-void launch_metro_reduce(int n, const double* sqrD, double upper, double* part, double* out3, hipStream_t st) {
-  const int nb = std::max(1, std::min((n + 255) / 256, 1024));
-  hipLaunchKernelGGL(metro_reduce_kernel, dim3(nb), dim3(256), 0, st, n, sqrD, upper, part);
-  hipLaunchKernelGGL(metro_fin_kernel, dim3(1), dim3(64), 0, st, nb, (const double*)part, out3);
+// min(upper, min over faces of point_distance_ep) through the tree: nearer child first, a
+// child skipped when its box lies further than the current distance plus `margin` (a bound
+// far above the rounding of the face distances, so no face that could lower it is skipped)
+__global__ __launch_bounds__(kBvhBlock) void metro_ep_kernel(SurfaceDev s, const double* __restrict__ ep, int nq,
+                                                             const double* __restrict__ P, double upper,
+                                                             double margin, double* __restrict__ dist, int* status) {
+  extern __shared__ int mstack[];
+  const int cap = stack_frames(s.depth);
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nq) return;
+  double q[3];
+  load3(P, k, q);
+  double best = upper;
+  int* stk = mstack + threadIdx.x;
+  int top = 0;
+  stk[0] = 0;
+  top = 1;
+  bool overflow = false;
+  while (top > 0) {
+    const int n = stk[(--top) * kBvhBlock];
+    const double2* r2 = reinterpret_cast<const double2*>(s.rec + (size_t)n * kNodeRec);
+    double x[12];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) { const double2 w = r2[j]; x[2 * j] = w.x; x[2 * j + 1] = w.y; }
+    const int4 w = reinterpret_cast<const int4*>(r2)[6];
+    if (w.z >= 0) {
+      point_distance_ep(ep + (size_t)w.z * kEPRec, x, x + 3, x + 6, q, best);
+      continue;
+    }
+    const double lim = (best + margin) * (best + margin);
+    const double dl = box_ext_dist(x, q), dr = box_ext_dist(x + 6, q);
+    const int nearc = dl <= dr ? w.x : w.y, farc = dl <= dr ? w.y : w.x;
+    const double dn = dl <= dr ? dl : dr, df = dl <= dr ? dr : dl;
+    if (top + 2 > cap) { overflow = true; break; }
+    if (farc >= 0 && df <= lim) stk[(top++) * kBvhBlock] = farc;
+    if (nearc >= 0 && dn <= lim) stk[(top++) * kBvhBlock] = nearc;
+  }
+  if (overflow) atomicOr(status, 1);
+  dist[k] = best;
+}
+
+__global__ void metro_seq_kernel(int n, const double* __restrict__ dist, double upper, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double mx = -HUGE_VAL, sm = 0, cn = 0;
+  for (int i = 0; i < n; ++i) {
+    const double d = dist[i];
+    if (d == upper) continue;
+    if (d > mx) mx = d;
+    sm += d;
+    cn += 1;
+  }
+  out[0] = mx;
+  out[1] = sm;
+  out[2] = cn;
+}
+
+void launch_metro_ep(const SurfaceDev& s, const double* ep, int nq, const double* P, double upper, double* dist,
+                     int* status, hipStream_t st) {
+  if (nq <= 0) return;
+  const size_t lds = (size_t)stack_frames(s.depth) * kBvhBlock * sizeof(int);
+  hipLaunchKernelGGL(metro_ep_kernel, dim3((nq + kBvhBlock - 1) / kBvhBlock), dim3(kBvhBlock), lds, st, s, ep, nq, P,
+                     upper, 1e-9 * upper, dist, status);
+}
+
+void launch_metro_seq(int n, const double* dist, double upper, double* out3, hipStream_t st) {
+  hipLaunchKernelGGL(metro_seq_kernel, dim3(1), dim3(64), 0, st, n, dist, upper, out3);
 }
 
 void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int stride, const int* ids, double* S, int* I,

@@ -261,4 +261,52 @@ void build_column_bins(int nv, const double* V, int nf, const int* F, ColumnBins
   }
 }
 
+// vcg tri::UpdateEdges<>::ComputeEdgePlane (vcg/complex/algorithms/update/component_ep.h:48-73)
+// with the face's Plane3<double, true>: SetDirection normalises the cross product of the first
+// two edges, the offset is taken with that direction, then Normalize() runs once more; the
+// edges are scaled by the inverse of the normal's dominant component.
+void build_ep_faces(int nv, const double* V, int nf, const int* F, std::vector<double>& ep) {
+  (void)nv;
+  ep.assign((size_t)nf * kEPRec, 0.0);
+  auto normalize = [](double n[3]) {
+    const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (len > 0.0) {
+      n[0] /= len;
+      n[1] /= len;
+      n[2] /= len;
+    }
+  };
+  for (int f = 0; f < nf; ++f) {
+    const double* P[3] = {V + (size_t)F[3 * (size_t)f] * 3, V + (size_t)F[3 * (size_t)f + 1] * 3,
+                          V + (size_t)F[3 * (size_t)f + 2] * 3};
+    double E[3][3];
+    for (int j = 0; j < 3; ++j)
+      for (int d = 0; d < 3; ++d) E[j][d] = P[(j + 1) % 3][d] - P[j][d];
+    double n[3] = {E[0][1] * E[1][2] - E[0][2] * E[1][1], E[0][2] * E[1][0] - E[0][0] * E[1][2],
+                   E[0][0] * E[1][1] - E[0][1] * E[1][0]};
+    normalize(n);
+    const double off = n[0] * P[0][0] + n[1] * P[0][1] + n[2] * P[0][2];
+    normalize(n);
+    const double nx = std::fabs(n[0]), ny = std::fabs(n[1]), nz = std::fabs(n[2]);
+    int axis;
+    double dinv;
+    if (nx > ny && nx > nz) {
+      axis = 0;
+      dinv = 1 / n[0];
+    } else if (ny > nz) {
+      axis = 1;
+      dinv = 1 / n[1];
+    } else {
+      axis = 2;
+      dinv = 1 / n[2];
+    }
+    double* r = &ep[(size_t)f * kEPRec];
+    for (int j = 0; j < 3; ++j)
+      for (int d = 0; d < 3; ++d) r[3 * j + d] = E[j][d] * dinv;
+    for (int d = 0; d < 3; ++d) r[9 + d] = n[d];
+    r[12] = off;
+    r[13] = (double)axis;
+  }
+}
+
 }  // namespace hexopt

@@ -985,6 +985,7 @@ struct hexopt_surface {
   bool bins_ready = false;
   ColumnBins bins;
   DevBuf dboff, dbfid, dbfbox, dcells, dflags;
+  DevBuf dep;  // metro: the faces' edge-plane records
   ColumnBinsDev bview{};
   std::string err;
 };
@@ -1444,9 +1445,11 @@ int hexopt_refresh_lines(hexopt_ctx* c, hexopt_curves* cv, const int32_t* curve_
   return HEXOPT_OK;
 }
 
-// One metro direction: every face-referenced vertex of (V, F) against surface `to`.
-static int metro_direction(hexopt_surface* to, int nv, const double* V, int nf, const int32_t* F, double upper,
-                           double out3[3]) {
+// One metro direction (vcg Sampling::Hausdorff with vertex sampling): every face-referenced
+// vertex of (V, F), in index order, against surface `to` (whose mesh is Vt, Ft), each at
+// vcg's edge-plane point-to-face distance; the distances are added one by one in that order.
+static int metro_direction(hexopt_surface* to, int nvt, const double* Vt, int nft, const int32_t* Ft, int nv,
+                           const double* V, int nf, const int32_t* F, double upper, double out3[3]) {
   std::vector<char> ref(nv, 0);
   for (long i = 0; i < 3L * nf; ++i) ref[F[i]] = 1;
   std::vector<double> P;
@@ -1454,19 +1457,22 @@ static int metro_direction(hexopt_surface* to, int nv, const double* V, int nf, 
   for (int v = 0; v < nv; ++v)
     if (ref[v]) P.insert(P.end(), V + (size_t)v * 3, V + (size_t)v * 3 + 3);
   const int nq = (int)(P.size() / 3);
+  std::vector<double> ep;
+  build_ep_faces(nvt, Vt, nft, Ft, ep);
   hipStream_t st = to->stream;
+  HIPCHK(to, to->dep.ensure(ep.size() * 8 + 8));
   HIPCHK(to, to->dP.ensure(P.size() * 8 + 8));
   HIPCHK(to, to->dS.ensure((size_t)nq * 8 + 8));
-  HIPCHK(to, to->dC.ensure((size_t)3 * 1024 * 8 + 3 * 8));
+  HIPCHK(to, to->dC.ensure(3 * 8));
+  HIPCHK(to, hipMemcpyAsync(to->dep.p, ep.data(), ep.size() * 8, hipMemcpyHostToDevice, st));
   HIPCHK(to, hipMemcpyAsync(to->dP.p, P.data(), P.size() * 8, hipMemcpyHostToDevice, st));
   HIPCHK(to, hipMemsetAsync(to->dstat.p, 0, sizeof(int), st));
-  launch_closest(to->view, nq, to->dP.as<double>(), 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
-                 to->dS.as<double>(), to->dstat.as<int>(), st);
-  double* red = to->dC.as<double>();
-  launch_metro_reduce(nq, to->dS.as<double>(), upper, red, red + 3 * 1024, st);
+  launch_metro_ep(to->view, to->dep.as<double>(), nq, to->dP.as<double>(), upper, to->dS.as<double>(),
+                  to->dstat.as<int>(), st);
+  launch_metro_seq(nq, to->dS.as<double>(), upper, to->dC.as<double>(), st);
   HIPCHK(to, hipGetLastError());
   int stat = 0;
-  HIPCHK(to, hipMemcpyAsync(out3, red + 3 * 1024, 3 * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(to, hipMemcpyAsync(out3, to->dC.p, 3 * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(to, hipMemcpyAsync(&stat, to->dstat.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(to, hipStreamSynchronize(st));
   if (stat) return (to->err = "traversal stack overflow", HEXOPT_EUNSUPPORTED);
@@ -1500,8 +1506,8 @@ int hexopt_metro_hausdorff(int nv0, const double* V0, int nf0, const int32_t* F0
   hexopt_surface* s1 = hexopt_surface_create(nv1, V1, nf1, F1, nullptr, nullptr, nullptr, device);
   if (!s1) { hexopt_surface_destroy(s0); return HEXOPT_EINVAL; }
   double fwd[3] = {0, 0, 0}, bwd[3] = {0, 0, 0};
-  int rc = metro_direction(s1, nv0, V0, nf0, F0, D, fwd);
-  if (rc == HEXOPT_OK) rc = metro_direction(s0, nv1, V1, nf1, F1, D, bwd);
+  int rc = metro_direction(s1, nv1, V1, nf1, F1, nv0, V0, nf0, F0, D, fwd);
+  if (rc == HEXOPT_OK) rc = metro_direction(s0, nv0, V0, nf0, F0, nv1, V1, nf1, F1, D, bwd);
   if (rc != HEXOPT_OK) g_surface_error = s0->err.empty() ? s1->err : s0->err;
   hexopt_surface_destroy(s0);
   hexopt_surface_destroy(s1);

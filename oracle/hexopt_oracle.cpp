@@ -1489,17 +1489,94 @@ extern "C" void orc_line_projection(int nv, const double* V, int ncurves, const 
 // metro_hausdorff.cpp:358-505; vcg Sampling::Hausdorff / VertexSampling / AddSample,
 // extern/vcg/sampling.h:224-300,548-601): the box of all vertices of both meshes is grown by
 // 2% of its diagonal on every side; each face-referenced vertex of one mesh is sampled
-// against the other mesh's triangles (exact point-triangle distance, brute force here); a
-// sample counts when its distance is below the grown box's diagonal. out[0] = that diagonal,
+// against the other mesh's triangles at vcg's edge-plane point-to-face distance
+// (face::PointDistanceEP, vcg/simplex/face/distance.h:48-180, on the planes and scaled edges
+// of tri::UpdateEdges::ComputeEdgePlane, vcg/complex/algorithms/update/component_ep.h:48-73),
+// brute force over the faces here; a sample counts when its distance is below the grown box's
+// diagonal, and the counted distances are added in vertex order. out[0] = that diagonal,
 // out[1] = the larger of the two directions' maxima, out[2] = the larger of their means.
 namespace {
+struct EPFace {
+  double e[3][3];  // edges P1-P0, P2-P1, P0-P2 scaled by 1 / n[axis]
+  double n[3], off;
+  int axis;
+  const double* P[3];
+};
+
+void unit(double n[3]) {
+  const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (len > 0.0) { n[0] /= len; n[1] /= len; n[2] /= len; }
+}
+
+EPFace ep_face(const double* V, const int* F, int f) {
+  EPFace r;
+  for (int j = 0; j < 3; ++j) r.P[j] = V + (size_t)F[3 * (size_t)f + j] * 3;
+  for (int j = 0; j < 3; ++j)
+    for (int d = 0; d < 3; ++d) r.e[j][d] = r.P[(j + 1) % 3][d] - r.P[j][d];
+  r.n[0] = r.e[0][1] * r.e[1][2] - r.e[0][2] * r.e[1][1];
+  r.n[1] = r.e[0][2] * r.e[1][0] - r.e[0][0] * r.e[1][2];
+  r.n[2] = r.e[0][0] * r.e[1][1] - r.e[0][1] * r.e[1][0];
+  unit(r.n);                                                    // SetDirection (normalising plane)
+  r.off = r.n[0] * r.P[0][0] + r.n[1] * r.P[0][1] + r.n[2] * r.P[0][2];
+  unit(r.n);                                                    // Plane().Normalize()
+  const double ax = std::fabs(r.n[0]), ay = std::fabs(r.n[1]), az = std::fabs(r.n[2]);
+  double s;
+  if (ax > ay && ax > az) { r.axis = 0; s = 1 / r.n[0]; }
+  else if (ay > az) { r.axis = 1; s = 1 / r.n[1]; }
+  else { r.axis = 2; s = 1 / r.n[2]; }
+  for (int j = 0; j < 3; ++j)
+    for (int d = 0; d < 3; ++d) r.e[j][d] *= s;
+  return r;
+}
+
+double seg_dist(const double* p, const double* a, const double* b) {  // PSDist
+  const double e[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  double t = ((p[0] - a[0]) * e[0] + (p[1] - a[1]) * e[1] + (p[2] - a[2]) * e[2]) / (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  t = t < 0 ? 0 : (t > 1 ? 1 : t);
+  const double c[3] = {a[0] + e[0] * t, a[1] + e[1] * t, a[2] + e[2] * t};
+  const double w[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+  return std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+}
+
+// PointDistanceEP: lowers dist when the face is closer
+void ep_distance(const EPFace& f, const double* q, double& dist) {
+  const double d = (f.n[0] * q[0] + f.n[1] * q[1] + f.n[2] * q[2]) - f.off;
+  if (d > dist || d < -dist) return;
+  const double p[3] = {q[0] - f.n[0] * d, q[1] - f.n[1] * d, q[2] - f.n[2] * d};
+  static const int U[3] = {1, 2, 0}, W[3] = {2, 0, 1};
+  const int u = U[f.axis], w = W[f.axis];
+  double b[3];
+  const int order[3][3] = {{1, 1, 2}, {2, 2, 0}, {0, 0, 1}};  // edge, its start corner, its end corner
+  for (int k = 0; k < 3; ++k) {
+    const double* e = f.e[order[k][0]];
+    const double* a = f.P[order[k][1]];
+    b[k] = e[u] * (p[w] - a[w]) - e[w] * (p[u] - a[u]);
+    if (b[k] <= 0) {
+      const double t = seg_dist(q, f.P[order[k][1]], f.P[(order[k][1] + 1) % 3]);
+      if (dist > t) dist = t;
+      return;
+    }
+  }
+  const double bm = std::min(b[0], std::min(b[1], b[2]));
+  const double* P0 = f.P[0];
+  const double ab[3] = {f.P[1][0] - P0[0], f.P[1][1] - P0[1], f.P[1][2] - P0[2]};
+  const double ac[3] = {f.P[2][0] - P0[0], f.P[2][1] - P0[1], f.P[2][2] - P0[2]};
+  const double x[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+  if (bm < 0.000001 * std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2])) {
+    const int k = bm == b[0] ? 0 : (bm == b[1] ? 1 : 2);
+    const double t = seg_dist(q, f.P[order[k][1]], f.P[(order[k][1] + 1) % 3]);
+    if (dist > t) dist = t;
+    return;
+  }
+  dist = std::fabs(d);
+}
+
 void metro_direction(int nvA, const double* VA, int nfA, const int* FA, int nfB, const double* VB, const int* FB,
                      double upper, double& mx, double& mean) {
   std::vector<char> ref(nvA, 0);
   for (long i = 0; i < 3L * nfA; ++i) ref[FA[i]] = 1;
-  Tree tb;  // only its Ericson point-triangle routine is used
-  tb.V = VB;
-  tb.F = FB;
+  std::vector<EPFace> faces(nfB);
+  for (int f = 0; f < nfB; ++f) faces[f] = ep_face(VB, FB, f);
   mx = -HUGE_VAL;
   double sum = 0;
   long n = 0;
@@ -1507,14 +1584,9 @@ void metro_direction(int nvA, const double* VA, int nfA, const int* FA, int nfB,
     if (!ref[v]) continue;
     const double* p = VA + (size_t)v * 3;
     double d = upper;
-    for (int f = 0; f < nfB; ++f) {
-      double c[3], bary[3];
-      tb.closest_tri(p, f, c, bary);
-      const double d0 = p[0] - c[0], d1 = p[1] - c[1], d2 = p[2] - c[2];
-      d = std::min(d, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
-    }
-    if (!(d < upper)) continue;
-    mx = std::max(mx, d);
+    for (int f = 0; f < nfB; ++f) ep_distance(faces[f], p, d);
+    if (d == upper) continue;
+    if (d > mx) mx = d;
     sum += d;
     ++n;
   }

@@ -187,6 +187,18 @@ def main():
 
 
 
+def metro_full():
+    """metro_full.npz: the reference's metro between the bench's clipped torus surface and the
+    boundary of its 1,007,235-hex mesh (vertex sampling both ways)."""
+    V, H, _ = mg.torus_hex_mesh(1_000_000, 1.0, 0.35, 0.2)
+    T = mg.hex_boundary_triangles(H).astype(np.int32)
+    SV, SF, _ = mg.clipped_torus_surface(1.0, 0.35, 0.2, nu=200, nband=30, nring=20)
+    r = refio.run_metro(SV, SF.astype(np.int32), V, T)
+    save("metro_full", target=np.array(1_000_000), hexes=np.array(H.shape[0]),
+         hex_boundary_triangles=np.array(T.shape[0]),
+         ref_bbox_diagonal=r["bbox_diagonal"][0], ref_max=r["max"][0], ref_mean=r["mean"][0])
+
+
 def metro_cases():
     """Vertex-sampled metro Hausdorff distance (metro_hausdorff.cpp compute, const overload)."""
     if not os.path.exists(refio.METRO_BIN):
@@ -412,6 +424,8 @@ if __name__ == "__main__":
     import sys
     if sys.argv[1:] == ["octree"]:
         octree_cases()
+    elif sys.argv[1:] == ["metro_full"]:
+        metro_full()
     elif sys.argv[1:] == ["io"]:
         io_cases()
     elif sys.argv[1:] == ["metro"]:
@@ -427,3 +441,4 @@ if __name__ == "__main__":
         smooth_cases()
         io_cases()
         octree_cases()
+        metro_full()

@@ -16,8 +16,8 @@ def test_metro_matches_reference(name):
     c = load(name)
     D, mx, mean = O.metro_hausdorff(c["V0"], c["F0"], c["V1"], c["F1"])
     assert D == c["ref_bbox_diagonal"][0]
-    assert abs(mx - c["ref_max"][0]) <= 1e-15 * c["ref_max"][0]
-    assert abs(mean - c["ref_mean"][0]) <= 1e-14 * c["ref_mean"][0]
+    assert mx == c["ref_max"][0]
+    assert mean == c["ref_mean"][0]
 
 
 def test_metro_closed_forms():
