@@ -35,7 +35,7 @@ def ref_octree(V, F, grid, origin, voxel_size, mesh_transform, stop_extent, grad
                 cells = np.ascontiguousarray(refine[1], np.int32).reshape(-1, 4)
                 f.write(struct.pack("<2i", int(refine[0]), len(cells)))
                 f.write(cells.tobytes())
-        subprocess.run([OCT_BIN, pin, pout], check=True, timeout=600)
+        subprocess.run([OCT_BIN, pin, pout], check=True, timeout=600, stdout=subprocess.DEVNULL)
         raw = open(pout, "rb").read()
     nl, nn, g, p = struct.unpack_from("<4i", raw, 0)
     a = np.frombuffer(raw, "<i4", offset=16)

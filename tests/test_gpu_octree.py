@@ -109,6 +109,29 @@ def test_compute_octree_composition():
     assert 0 < got.sum() < len(got)
 
 
+def test_octree_edge_cases():
+    """No split: a surface outside the grid, or a stop extent at the root extent, leaves the
+    root cells (and all of them for an anisotropic grid); a single split root with pairing
+    splits every root. Leaf and node counts as the reference's OctreeGrid gives them (8 and
+    30, 8 and 30, 64 and 135)."""
+    from pyhexopt import capi
+    from occ_cases import sphere
+    V, F = sphere(2, 0.5)
+    s = capi.Surface(V, F)
+    grid = [64, 32, 16]                 # 4 x 2 x 1 root cells of extent 16
+    far = s.octree(grid, np.array([10.0, 10.0, 10.0]), 0.01, None, 1)
+    assert len(far["leaves"]) == 8 and set(far["leaves"][:, 3]) == {16}
+    assert len(far["nodes"]) == 5 * 3 * 2 and far["levels"] == 1
+    stop = s.octree(grid, np.array([-0.6, -0.6, -0.6]), 1.2 / 16, None, 16)
+    assert len(stop["leaves"]) == 8
+    # the sphere touches only the first root column: pairing splits every root once
+    one = s.octree(grid, np.array([-0.6, -0.6, -0.6]), 1.2 / 16, None, 8)
+    assert check_octree(one["leaves"], grid)
+    assert len(one["leaves"]) >= 8 * 8 and set(one["leaves"][:, 3]) <= {8, 16}
+    assert (one["leaves"][:, 3] == 8).sum() == 64 and len(one["leaves"]) == 64
+    assert len(one["nodes"]) == 135
+
+
 def test_octree_mesh_feeds_the_optimiser_context():
     """octree_mesh's export is a valid hex mesh for hexopt_create (a uniform sphere octree)."""
     from pyhexopt import capi
