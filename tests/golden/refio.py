@@ -1,7 +1,8 @@
 """Input/output of the compiled-reference driver (oracle/_ref/slim_ref).
 
-Used only in the development container (where the reference tree exists) to write the
-golden fixtures under tests/golden/. GPU runs only read the committed fixtures.
+Used to write the golden fixtures under tests/golden/ in the development container (where the
+reference tree exists), and by bench.py's CPU-baseline leg, which times the built driver where
+it travelled with the tree. GPU tests only read the committed fixtures.
 """
 import os
 import struct
