@@ -1,4 +1,4 @@
-"""The C ABI library loads and exports every function include/hexopt.h declares (CPU only:
+"""The C ABI library loads and exports every function include/*.h declares (CPU only:
 no compute call is made)."""
 import ctypes
 import os
@@ -34,8 +34,18 @@ def test_library_exports_every_file_format_symbol():
     assert sorted(names) == sorted(meshio.EXPORTS)
 
 
+def test_library_exports_every_cleaning_symbol():
+    from pyhexopt import hexclean
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    names = declared("hexclean.h", "hexclean")
+    assert len(names) == 6
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(names) == sorted(hexclean.EXPORTS)
+
+
 def test_every_header_is_checked():
-    assert sorted(os.listdir(os.path.join(REPO, "include"))) == ["hexio.h", "hexopt.h"]
+    assert sorted(os.listdir(os.path.join(REPO, "include"))) == ["hexclean.h", "hexio.h", "hexopt.h"]
 
 
 def test_create_fails_cleanly_on_bad_mesh():
