@@ -181,10 +181,10 @@ def manifold_and_connected(nv, H, flag):
     assert len(seen) == len(m.Hvs)
 
 
-@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("seed", range(8))
 def test_random_blobs_match_the_oracle(seed):
     rng = np.random.default_rng(seed)
-    n = 6 if seed < 4 else 8
+    n = 6 if seed < 4 else 8 if seed < 6 else 12
     V, H = grid(n, n, n)
     # smoothed noise gives blobs with spikes, pits, thin bridges and separate pieces
     occ = rng.random((n, n, n)) < 0.5
