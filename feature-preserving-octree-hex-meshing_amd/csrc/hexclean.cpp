@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <thread>
 #include <vector>
@@ -369,6 +370,9 @@ int64_t guarded(int64_t nV, int64_t nH, const int32_t* H, uint8_t* H_flag, F&& b
     return -1;
   } catch (const std::bad_alloc&) {
     g_err = "out of host memory";
+    return -1;
+  } catch (const std::exception& e) {  // e.g. a sort thread that could not start
+    g_err = e.what();
     return -1;
   }
 }
