@@ -1,25 +1,33 @@
 """Benchmark: 1M-hex optimiser iterations/s, FP32+adjoint vs FP64 plain, on one MI355X.
 
-Workload: the torus R=1, r=0.35 cut by the planes z=+-0.2, triangulated
-with 40,000 triangles (two tube bands and two flat annuli meeting at four sharp rim
-circles), and a uniform-level octree hex mesh of ~1.0M hexes of its inside. Boundary
-vertices near a rim are LINE vertices on that rim (feature curve); the other boundary
-vertices are tangent-plane vertices.
+Workload (pyhexopt.workload): the torus R=1, r=0.35 cut by the planes z=+-0.2, 40,000
+triangles in four patches meeting at four sharp rim circles; the hex mesh is the graded,
+paired octree of that surface (leaves of 8 sizes, finest at the surface) restricted to the
+leaves inside it, ~1.0M hexes. Boundary vertices near a rim are LINE vertices of that rim; the
+others are REGULAR vertices of the patch of their nearest triangle.
 
-One step = one iteration of the reference's feature-alignment loop
-(grid_hex_meshing.cpp:3855-3869) on the device: rest <- current, compute_referenceMesh,
-tangent-plane projection onto the triangle surface (closest point + pseudonormal), feature
-projection onto the rims, one SLIM iteration (local step, 20-step CG global step,
-flip-avoiding line search), then lamda_T = lamda_L = min(max(E_q/E_s lamda, 1), 1e8).
-`value` is the FP32+adjoint optimiser's iterations/s; the FP64 plain (reference-faithful)
-path runs the same loop from the same start and is reported beside it, with the min scaled
-Jacobian and the largest boundary-vertex distance to the surface each reaches.
+One step = one iteration of the reference's feature-alignment loop (feature_alignment,
+grid_hex_meshing.cpp:3783-3912, with dirty_graph_projection :3954-4082) on the device:
+rest <- current, compute_referenceMesh (cube shapes, H_flag), tangent planes from each
+REGULAR vertex's own patch tree, LINE vertices projected onto their rim, one SLIM iteration
+(local step, 20-step CG global step, flip-avoiding line search), then
+lamda = min(0.05 max(E_q / E_s lamda, 1), 1e16) with lamda starting at args.feature_weight
+= 0.05 and opt.weight_opt = 0.001.
+
+`value` is the FP32+adjoint optimiser's iterations/s (FP32 element arithmetic, FP64 sign
+test, step bound, energy sums and an FP64-residual iterative refinement of the direction).
+The FP64 plain (reference-faithful) path runs the same loop from the same start. Both then
+continue, untimed, to `--iterations` loop iterations in all, and the final min scaled
+Jacobian and metro Hausdorff distance of each are reported with their differences. The
+per-result errors of the FP32 path against FP64 (energy, gradient, SLIM weights, step,
+positions) are measured on the first iteration from the shared start.
 
 Prints one JSON line.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -30,9 +38,8 @@ sys.path.insert(0, os.path.join(REPO, "feature-preserving-octree-hex-meshing_amd
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "cg32_apply_pipe_torus1m_hbm_traffic.md")
-R_MAJ, R_MIN, Z_CUT = 1.0, 0.35, 0.2
 WEIGHT_OPT = 0.001          # feature_alignment's opt.weight_opt = weight_opt * 0.001
-LAMDA0, LAMDA_BOUND = 1.0, 1e8
+LAMDA0, LAMDA_BOUND = 0.05, 1e16   # args.feature_weight, LAMDA_FEATURE_PROJECTION_BOUND
 
 
 def measured_traffic(nH):
@@ -46,109 +53,168 @@ def measured_traffic(nH):
         return None
 
 
-def build_workload(target):
-    from pyhexopt import meshgen as mg
-    V, H, bnd = mg.torus_hex_mesh(target, R_MAJ, R_MIN, Z_CUT)
-    SV, SF, rims = mg.clipped_torus_surface(R_MAJ, R_MIN, Z_CUT, nu=200, nband=30, nring=20)
-    h = np.linalg.norm(V[H[0, 1]] - V[H[0, 0]])
-    w = np.sqrt(R_MIN ** 2 - Z_CUT ** 2)
-    rim_geom = [(R_MAJ - w, -Z_CUT), (R_MAJ - w, Z_CUT), (R_MAJ + w, -Z_CUT), (R_MAJ + w, Z_CUT)]
-    P = V[bnd]
-    rho = np.sqrt(P[:, 0] ** 2 + P[:, 1] ** 2)
-    d = np.stack([np.hypot(rho - rr, P[:, 2] - zz) for rr, zz in rim_geom], axis=1)
-    k = d.argmin(axis=1)
-    line = d[np.arange(k.size), k] < 0.75 * h
-    return dict(V=V, H=H, idsT=bnd[~line].astype(np.int32), idsL=bnd[line].astype(np.int32),
-                curveL=k[line].astype(np.int32), SV=SV, SF=SF, curves=[list(r) for r in rims], h=h)
-
-
 def make_ctx(w):
-    from pyhexopt import capi
+    from pyhexopt import capi, workload
     h = capi.HexOpt(w["V"], w["H"])
     nT, nL = w["idsT"].size, w["idsL"].size
     h.set_constraints(idsT=w["idsT"], normalT=np.zeros((nT, 3)), disT=np.zeros(nT), lamT=LAMDA0,
                       idsL=w["idsL"], axaL=np.zeros((nL, 3)), originL=np.zeros((nL, 3)), lamL=LAMDA0)
-    surf = capi.Surface(w["SV"], w["SF"])
+    patches = workload.patch_surfaces(w)
     curves = capi.Curves(w["SV"], w["curves"], np.ones(len(w["curves"]), dtype=np.uint8))
-    return h, surf, curves
+    return h, patches, curves
 
 
-def run(h, surf, curves, w, mode, steps, lam):
+def prepare(h, patches, curves, w, lam):
+    """The loop body up to the SLIM iteration: reference operator at the current positions,
+    patch planes and rim lines at weight lam."""
+    h.rest_from_current()              # ts.V = ts.UV
+    h.reference()                      # compute_referenceMesh(ts.UV, ..., md.H_flag, cube): every hex inside
+    h.project_patches(patches, w["patchT"], lam)
+    h.refresh_lines(curves, w["curveL"], lam)
+
+
+def iterate(h, mode, refine, measure=False):
+    if mode == "fp64":
+        E, Eq, Es, log, _ = h.slim_solve(1, weight_opt=WEIGHT_OPT)
+        return Eq, Es, log[0]
+    E, log = h.opt32_solve(1, weight_opt=WEIGHT_OPT, refine_iters=refine, residual_every=1 if measure else 0)
+    return log[0, 1], log[0, 2], log[0]
+
+
+def run(h, patches, curves, w, mode, steps, lam, refine):
     """`steps` loop iterations; returns seconds, the per-iteration logs and the final lamda."""
     logs = []
     h.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
-        h.rest_from_current()              # ts.V = ts.UV
-        h.reference()                      # compute_referenceMesh at the current positions
-        h.project_planes(surf, lam)        # tangent planes (dirty_graph_projection, REGULAR vertices)
-        h.refresh_lines(curves, w["curveL"], lam)   # feature lines (LINE vertices)
-        if mode == "fp64":
-            E, Eq, Es, log, _ = h.slim_solve(1, weight_opt=WEIGHT_OPT)
-            row = log[0]
-        else:
-            # the FP64 residual of the direction solve is measured on every 10th step
-            E, log = h.opt32_solve(1, weight_opt=WEIGHT_OPT, residual_every=1 if k % 10 == 0 else 0)
-            row = log[0]
-            Eq, Es = row[1], row[2]
+        prepare(h, patches, curves, w, lam)
+        Eq, Es, row = iterate(h, mode, refine, measure=(k % 10 == 0))
         logs.append(row)
         lam = min(LAMDA0 * max(Eq / Es * lam, 1.0), LAMDA_BOUND) if Es > 0 else LAMDA_BOUND
     h.synchronize()
-    return time.perf_counter() - t0, np.array(logs), lam
+    return time.perf_counter() - t0, logs, lam
 
 
-def cpu_baseline(target_small):
+def first_iteration_errors(w, refine):
+    """FP32 path against FP64 from the shared start: quality energy of the same positions, dE/dx,
+    SLIM weight blocks, the accepted step, the positions after one iteration."""
+    from pyhexopt import capi
+    res = {}
+    ctx = {m: make_ctx(w) for m in ("fp64", "fp32")}
+    try:
+        h64 = ctx["fp64"][0]
+        prepare(*ctx["fp64"], w, LAMDA0)
+        prepare(*ctx["fp32"], w, LAMDA0)
+        X0 = h64.get_positions()
+        # the FP32 adjoint's arithmetic error: both gradients at the same (FP32) positions
+        X32 = X0.astype(np.float32)
+        g64 = h64.energy_grad(X32.astype(np.float64), precision=64, weight_opt=WEIGHT_OPT)
+        g32 = h64.energy_grad(X32, precision=32, weight_opt=WEIGHT_OPT).astype(np.float64)
+        K64 = h64.debug_operator(64, weight_opt=WEIGHT_OPT)
+        K32 = h64.debug_operator(32, weight_opt=WEIGHT_OPT)
+        _, _, r64 = iterate(h64, "fp64", refine)
+        h32 = ctx["fp32"][0]
+        _, _, r32 = iterate(h32, "fp32", refine, measure=True)
+        X64, X32 = h64.get_positions(), h32.get_positions()
+        Eq32_at64, _ = h32.energy(weight_opt=WEIGHT_OPT)    # FP64 energy of the FP32 iterate
+        diag = float(np.linalg.norm(w["SV"].max(axis=0) - w["SV"].min(axis=0)))
+        gscale = np.abs(g64).max()
+        kscale = np.abs(K64).max()
+        res = {
+            # the quality energy of the FP32 iterate summed from FP32 element energies, against FP64
+            "energy_quality_rel": abs(r32[1] - Eq32_at64) / abs(Eq32_at64),
+            "grad_rel_max": float(np.abs(g32 - g64).max() / gscale),
+            "grad_rel_l2": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+            "weights_rel_max": float(np.abs(K32 - K64).max() / kscale),
+            "weights_rel_l2": float(np.linalg.norm(K32 - K64) / np.linalg.norm(K64)),
+            "step_fp32": float(r32[4]), "step_fp64": float(r64[4]),
+            "step_bound_fp32": float(r32[3]), "step_bound_fp64": float(r64[3]),
+            "positions_max_over_diag": float(np.abs(X32 - X64).max() / diag),
+            "fp64_residual_before_refinement": float(r32[7]),
+            "fp64_residual_after_refinement": float(r32[9]),
+            "fp64_cg_error": float(r64[6]),
+        }
+    finally:
+        for h, patches, curves in ctx.values():
+            h.close()
+            for p in patches:
+                p.close()
+            curves.close()
+    return res
+
+
+def boundary_triangles(quads):
+    return np.concatenate([quads[:, [0, 1, 2]], quads[:, [2, 3, 0]]]).astype(np.int32)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(num_voxels):
     """The reference's own CPU loop iteration where its compiled build exists
     (oracle/_ref/slim_ref, built from the reference's sources; driven through the golden
-    helper tests/golden/refio.py): projection_smooth's plane projection through igl's tree,
-    then slim_m_opt (slim_precompute + one slim_solve2 iteration), single-threaded, on a
-    bounded sample, a smaller mesh of the same workload; iterations/s scaled to 1M hexes.
-    The whole driver process is timed (input, tree set-up and the final scaled-Jacobian pass
-    included), and its loop leaves the feature lines out. Without that build: the CPU oracle's
-    serial FP64 restatement of the same iteration (lines included)."""
-    from pyhexopt import capi
-    w = build_workload(target_small)
+    helper tests/golden/refio.py), on a bounded sample: the same workload at a coarser
+    octree (num_voxels), single-threaded. The driver times the loop body alone
+    (compute_referenceMesh, projection_smooth's plane projection through igl's tree,
+    slim_m_opt: slim_precompute + one slim_solve2 iteration, the rim lines as constraints)
+    with std::chrono; iterations/s are scaled linearly to the 1M-hex mesh by hex count.
+    Without that build: the CPU oracle's serial FP64 restatement of the same iteration."""
+    from pyhexopt import capi, workload
+    w = workload.clipped_torus_octree(num_voxels)
     V, H = w["V"], w["H"]
+    nL = w["idsL"].size
+    circles = np.ones(len(w["curves"]), dtype=np.uint8)
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py as O
+    o, a = O.line_projection(w["SV"], w["curves"], circles, V[w["idsL"]], w["curveL"])
     ref_bin = os.path.join(REPO, "oracle", "_ref", "slim_ref")
     if os.path.exists(ref_bin):
         sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
         import refio
         ids = np.asarray(w["idsT"], np.int32)
         t0 = time.perf_counter()
-        refio.run_slim(1, V, H, energy=capi.SYMMETRIC_DIRICHLET, n_outer=1, n_inner=1, lamT=LAMDA0,
-                       lamT_bound=LAMDA_BOUND, idsT=ids, normalT=np.zeros((ids.size, 3)), disT=np.zeros(ids.size),
-                       surfV=w["SV"], surfF=w["SF"], weight_opt=WEIGHT_OPT)
-        dt = time.perf_counter() - t0
+        r = refio.run_slim(1, V, H, energy=capi.SYMMETRIC_DIRICHLET, n_outer=1, n_inner=1, lamT=LAMDA0,
+                           lamT_bound=LAMDA_BOUND, lamL=LAMDA0, idsT=ids, normalT=np.zeros((ids.size, 3)),
+                           disT=np.zeros(ids.size), idsL=w["idsL"], axaL=a, originL=o, surfV=w["SV"],
+                           surfF=w["SF"], weight_opt=WEIGHT_OPT)
+        whole = time.perf_counter() - t0
+        dt = float(r["t_iter"][0])
         return {"value": (1.0 / dt) * H.shape[0] / 1e6, "unit": "iterations/s (scaled to 1M hexes)", "cores": 1,
-                "kind": "reference",
-                "sample": "1 loop iteration of the compiled reference (projection_smooth + slim_m_opt: "
-                          "compute_referenceMesh, slim_precompute, one slim_solve2 iteration; %d plane vertices, "
-                          "no feature lines) on the %d-hex clipped torus (%.1f s for the whole driver run), "
-                          "scaled by %d/1e6" % (ids.size, H.shape[0], dt, H.shape[0])}
-    sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import oracle_py as O
+                "kind": "reference", "cpu": cpu_model(),
+                "sample": "1 loop iteration of the compiled reference (compute_referenceMesh, projection_smooth, "
+                          "slim_m_opt: slim_precompute + one slim_solve2 iteration; %d plane vertices against the "
+                          "whole surface, %d rim vertices as line constraints) on the %d-hex octree mesh of the "
+                          "same torus at num_voxels %d: %.2f s for the loop body (%.1f s for the whole driver "
+                          "process), scaled by %d/1e6" % (ids.size, nL, H.shape[0], num_voxels, dt, whole,
+                                                           H.shape[0])}
     FN, VN, FEN = capi.surface_normals(w["SV"], w["SF"])     # host-only, part of the surface set-up
-    circles = np.ones(len(w["curves"]), dtype=np.uint8)
     t0 = time.perf_counter()
     G = O.grad_ref(V, H)
     _, _, C, N, _ = O.signed_distance_pseudonormal(V[w["idsT"]], w["SV"], w["SF"], FN, VN, FEN)
-    o, a = O.line_projection(w["SV"], w["curves"], circles, V[w["idsL"]], w["curveL"])
     O.slim_m_opt(V, H, G, V, 1, weight_opt=WEIGHT_OPT, lam_T=LAMDA0, lam_L=LAMDA0, idsT=w["idsT"], normalT=N,
                  disT=(N * C).sum(axis=1), idsL=w["idsL"], axaL=a, originL=o)
     dt = time.perf_counter() - t0
     return {"value": (1.0 / dt) * H.shape[0] / 1e6, "unit": "iterations/s (scaled to 1M hexes)", "cores": 1,
-            "kind": "port",
+            "kind": "port", "cpu": cpu_model(),
             "sample": "1 loop iteration (reference operator, projections, FP64 SLIM iteration) of the CPU oracle "
-                      "on the %d-hex clipped torus (%.1f s), scaled by %d/1e6" % (H.shape[0], dt, H.shape[0])}
+                      "on the %d-hex octree mesh at num_voxels %d (%.1f s), scaled by %d/1e6" %
+                      (H.shape[0], num_voxels, dt, H.shape[0])}
 
 
 def glue_measure(target, steps):
     """The padding untangle (hexopt_glue_setup / hexopt_glue_solve's SLIM iteration, FP64,
-    CONFORMAL) on the exploded clipped-torus mesh: boundary vertices pinned, interior target
-    jittered by 0.3 edge lengths, start copies from meshgen.exploded_start. Fixed iteration
-    count; seconds per iteration and the FP64 element-operator kernel on the exploded mesh."""
+    CONFORMAL) on an exploded uniform clipped-torus mesh: boundary vertices pinned, interior
+    target jittered by 0.3 edge lengths, start copies from meshgen.exploded_start. Fixed
+    iteration count; seconds per iteration and the FP64 element-operator kernel."""
     from pyhexopt import capi, meshgen as mg
-    V, H, bnd = mg.torus_hex_mesh(target, R_MAJ, R_MIN, Z_CUT)
+    V, H, bnd = mg.torus_hex_mesh(target, 1.0, 0.35, 0.2)
     fixed = np.zeros(V.shape[0], dtype=bool)
     fixed[bnd] = True
     edge = np.linalg.norm(V[H[0, 1]] - V[H[0, 0]])
@@ -175,78 +241,76 @@ def octree_measure(reps=5):
     pairing cascades, leaf-to-hex export) around the 40,000-triangle clipped torus at
     num_voxels 4096 and STOP extent 8 (3,717,184 leaves). Device time of the build, best of
     `reps`, and the leaf and node counts."""
-    from pyhexopt import capi, meshgen as mg
-    V, F, _ = mg.clipped_torus_surface(R_MAJ, R_MIN, Z_CUT, nu=200, nband=30, nring=20)
-    lo, hi = V.min(0), V.max(0)
-    vs = float((hi - lo).max()) / 4096
-    grid = [1 << int(np.ceil(np.log2(np.ceil(e / vs)))) for e in (hi - lo)]
-    mt = (lo + hi) / 2 - (lo + (lo + vs * np.array(grid, np.float64))) * 0.5
+    from pyhexopt import capi, meshgen as mg, workload
+    V, F, _, _ = workload.torus_surface()
+    vs, lo, grid, mt = mg.octree_frame(V, 4096)
     s = capi.Surface(V, F)
     best = None
     for _ in range(reps):
         out = s.octree(grid, lo, vs, mt, 8)
         best = out["ms"] if best is None else min(best, out["ms"])
+    s.close()
     return {"surface_triangles": int(F.shape[0]), "grid": [int(g) for g in grid], "leaves": int(len(out["leaves"])),
             "nodes": int(len(out["nodes"])), "levels": out["levels"], "predicate_evaluations": out["evaluated"],
             "ms_build": round(best, 3)}
 
 
-def metro_measure(target):
-    """Metro Hausdorff (hexopt_metro_hausdorff: vertex sampling both ways, closest points on
-    the device) between the 40,000-triangle clipped torus and the boundary of its hex mesh
-    (each boundary quad as two triangles), as the feature-alignment loop's Hausdorff check
-    measures it. Wall time of the whole call: both surfaces' trees are built on the host."""
-    from pyhexopt import capi, meshgen as mg
-    V, H, _ = mg.torus_hex_mesh(target, R_MAJ, R_MIN, Z_CUT)
-    T = mg.hex_boundary_triangles(H).astype(np.int32)
-    SV, SF, _ = mg.clipped_torus_surface(R_MAJ, R_MIN, Z_CUT, nu=200, nband=30, nring=20)
-    capi.metro_hausdorff(SV, SF, V, T)    # warm-up (device code loaded)
-    t0 = time.perf_counter()
-    diag, mx, mean = capi.metro_hausdorff(SV, SF, V, T)
-    secs = time.perf_counter() - t0
-    return {"surface_triangles": int(SF.shape[0]), "hex_boundary_triangles": int(T.shape[0]),
-            "bbox_diagonal": diag, "max": mx, "mean": mean, "ms": round(1e3 * secs, 2)}
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--hexes", type=int, default=1_000_000)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--iterations", type=int, default=100, help="loop iterations of the quality comparison")
+    ap.add_argument("--num-voxels", type=int, default=512)
+    ap.add_argument("--stop-extent", type=int, default=1)
+    ap.add_argument("--refine", type=int, default=4, help="FP32 CG iterations of the FP64-residual refinement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-glue", action="store_true")
     ap.add_argument("--no-octree", action="store_true")
-    ap.add_argument("--no-metro", action="store_true")
+    ap.add_argument("--no-errors", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see the design notes on "
               "how the mesh would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    w = build_workload(args.hexes)
+    from pyhexopt import capi, workload
+    t0 = time.perf_counter()
+    w = workload.clipped_torus_octree(args.num_voxels, args.stop_extent)
     nV, nH = w["V"].shape[0], w["H"].shape[0]
-    print("workload: %d hexes, %d vertices, %d plane + %d line vertices, %d surface triangles" %
-          (nH, nV, w["idsT"].size, w["idsL"].size, w["SF"].shape[0]), file=sys.stderr, flush=True)
+    ext, cnt = w["extents"]
+    print("workload: %d hexes (%s), %d vertices, %d plane + %d line vertices, %d surface triangles, "
+          "built in %.1f s" % (nH, ", ".join("%d of extent %d" % (c, e) for e, c in zip(ext, cnt)), nV,
+                                w["idsT"].size, w["idsL"].size, w["SF"].shape[0], time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+
+    errors = None if args.no_errors else first_iteration_errors(w, args.refine)
+    if errors:
+        print("first-iteration errors: %s" % json.dumps(errors), file=sys.stderr, flush=True)
 
     res = {}
+    T = boundary_triangles(w["quads"])
+    total = max(args.iterations, args.warmup + args.steps)
     for mode in ("fp64", "fp32"):
-        h, surf, curves = make_ctx(w)
-        _, _, lam = run(h, surf, curves, w, mode, args.warmup, LAMDA0)
-        secs, logs, lam = run(h, surf, curves, w, mode, args.steps, lam)
+        h, patches, curves = make_ctx(w)
+        _, _, lam = run(h, patches, curves, w, mode, args.warmup, LAMDA0, args.refine)
+        secs, logs, lam = run(h, patches, curves, w, mode, args.steps, lam, args.refine)
+        _, rest, lam = run(h, patches, curves, w, mode, total - args.warmup - args.steps, lam, args.refine)
+        logs = np.array(logs + rest)
         mn, avg = h.scaled_jacobian()
         X = h.get_positions()
-        bnd = np.concatenate([w["idsT"], w["idsL"]])
-        S, _, _, _ = surf.closest(X[bnd])
+        diag, hmax, hmean = capi.metro_hausdorff(w["SV"], w["SF"], X, T)
         which = 0 if mode == "fp64" else 1
         ms_apply = h.time_kernel(which, 20)
-        res[mode] = dict(secs=secs, logs=logs, minsj=mn, avgsj=avg, ms_apply=ms_apply, maxdist=float(np.abs(S).max()),
-                         lam=lam, X=X)
-        print("%s: %.3f ms/iter, min SJ %.6f, max boundary distance %.3e, apply tile kernel %.1f us, "
-              "final energy %.9g, lamda %.3g, steps %s" %
-              (mode, 1e3 * secs / args.steps, mn, res[mode]["maxdist"], 1e3 * ms_apply, logs[-1][0], lam,
-               np.array2string(logs[:, 4], precision=3)), file=sys.stderr, flush=True)
+        ms_grad = h.time_kernel(5 if mode == "fp64" else 4, 20)
+        res[mode] = dict(secs=secs, logs=logs, minsj=mn, avgsj=avg, ms_apply=ms_apply, ms_grad=ms_grad,
+                         hmax=hmax, hmean=hmean, diag=diag, lam=lam, X=X)
+        print("%s: %.3f ms/iter, after %d iterations min SJ %.6f, metro max %.6e mean %.6e, operator kernel "
+              "%.1f us, energy+gradient kernel %.1f us, final energy %.9g, lamda %.3g" %
+              (mode, 1e3 * secs / args.steps, total, mn, hmax, hmean, 1e3 * ms_apply, 1e3 * ms_grad, logs[-1][0],
+               lam), file=sys.stderr, flush=True)
         h.close()
-        surf.close()
+        for p in patches:
+            p.close()
         curves.close()
 
     r32, r64 = res["fp32"], res["fp64"]
@@ -256,9 +320,12 @@ def main():
     # each) z and the previous direction read, the new direction and L p written
     bytes_per_hex32 = 192 + 16 + 4 * 12 * rV
     achieved = bytes_per_hex32 * nH / (r32["ms_apply"] * 1e-3) / 1e9
+    # the FP32 energy + gradient pass: 8 corner ids (32 B) and 8 reference factors (32 B) per
+    # hex, positions read and gradient written (12 B each per vertex)
+    grad_bytes_per_hex = 64 + 24 * rV
+    grad_achieved = grad_bytes_per_hex * nH / (r32["ms_grad"] * 1e-3) / 1e9
     traffic = measured_traffic(nH)
     value = args.steps / r32["secs"]
-    diag = float(np.linalg.norm(w["SV"].max(axis=0) - w["SV"].min(axis=0)))
     out = {
         "metric": "1M-hex optimiser iters/s: FP32+adjoint vs FP64 plain, equal min scaled-Jac",
         "value": round(value, 3),
@@ -270,25 +337,36 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (FP64 step bound, sign test, energy sums, CG dot products and residual)",
+        "dtype": "f32 (FP64 sign test, step bound, energy sums, CG dot products, residual refinement)",
         "data": "synthetic",
-        "config": {"workload": "clipped torus R=1 r=0.35 z=+-0.2 (40k triangles, 4 rim feature curves), "
-                               "uniform-level octree hex mesh, feature-alignment loop",
+        "config": {"workload": "clipped torus R=1 r=0.35 z=+-0.2 (40k triangles, 4 patches, 4 rim feature curves), "
+                               "graded paired octree hex mesh (leaves inside the surface), feature-alignment loop",
                    "hexes": nH, "vertices": nV, "plane_vertices": int(w["idsT"].size),
                    "line_vertices": int(w["idsL"].size), "surface_triangles": int(w["SF"].shape[0]),
-                   "iteration": "compute_referenceMesh + plane/line projection + 1 SLIM iteration "
-                                "(20-step CG, flip-avoiding line search), weight_opt 0.001"},
+                   "num_voxels": args.num_voxels, "stop_extent": args.stop_extent,
+                   "leaf_extents": {str(int(e)): int(c) for e, c in zip(ext, cnt)},
+                   "iteration": "compute_referenceMesh + patch plane / rim line projection + 1 SLIM iteration "
+                                "(20-step CG, flip-avoiding line search), weight_opt 0.001, lamda 0.05 -> 1e16",
+                   "fp32_refinement_cg_iters": args.refine, "quality_iterations": total},
         "fp64_plain": {"iterations_per_s": round(args.steps / r64["secs"], 3),
                        "ms_per_step": round(1e3 * r64["secs"] / args.steps, 4), "min_scaled_jacobian": r64["minsj"],
-                       "max_boundary_distance": r64["maxdist"], "apply_tile_kernel_us": round(1e3 * r64["ms_apply"], 2)},
+                       "avg_scaled_jacobian": r64["avgsj"], "hausdorff_max": r64["hmax"],
+                       "hausdorff_mean": r64["hmean"], "operator_kernel_us": round(1e3 * r64["ms_apply"], 2),
+                       "energy_gradient_kernel_us": round(1e3 * r64["ms_grad"], 2)},
         "fp32_adjoint": {"min_scaled_jacobian": r32["minsj"], "avg_scaled_jacobian": r32["avgsj"],
-                         "max_fp64_relative_residual": float(r32["logs"][:, 7].max()),
-                         "max_boundary_distance": r32["maxdist"],
+                         "hausdorff_max": r32["hmax"], "hausdorff_mean": r32["hmean"],
                          "max_vertex_diff_vs_fp64": float(np.abs(r32["X"] - r64["X"]).max())},
         "speedup_fp32_vs_fp64": round((args.steps / r32["secs"]) / (args.steps / r64["secs"]), 3),
         "min_sj_abs_diff": abs(r32["minsj"] - r64["minsj"]),
-        "max_boundary_distance_diff_over_bbox_diag": abs(r32["maxdist"] - r64["maxdist"]) / diag,
-        "element_evals_per_s": round(value * nH, 1),
+        "hausdorff_diff_over_bbox_diag": abs(r32["hmax"] - r64["hmax"]) / r64["diag"],
+        "bbox_diagonal": r64["diag"],
+        "first_iteration_errors_fp32_vs_fp64": errors,
+        "element_evals_per_s": round(nH / (r32["ms_grad"] * 1e-3), 1),
+        "element_evals_note": "one element-eval = energy + gradient of one hex (8 corner tets): the standalone "
+                              "FP32 adjoint pass (hexopt_energy_grad's kernel), device time",
+        "energy_gradient_roofline": {"bound": "hbm", "achieved": round(grad_achieved, 1), "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": round(grad_achieved / HBM_PEAK_GBS, 4),
+                                     "bytes_per_hex": round(grad_bytes_per_hex, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "cg32_apply_pipe_kernel (persistent FP32 CG operator pass)",
@@ -296,13 +374,11 @@ def main():
                      "traffic_source": "profiles/cg32_apply_pipe_torus1m_hbm_traffic.md" if traffic else None},
     }
     if not args.no_glue:
-        out["padding_untangle"] = glue_measure(max(args.hexes // 8, 20000), 5)
+        out["padding_untangle"] = glue_measure(max(nH // 8, 20000), 5)
     if not args.no_octree:
         out["octree_build"] = octree_measure()
-    if not args.no_metro:
-        out["metro_hausdorff"] = metro_measure(args.hexes)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(max(args.hexes // 16, 20000))
+        out["cpu_baseline"] = cpu_baseline(max(args.num_voxels // 4, 64))
     print(json.dumps(out))
 
 

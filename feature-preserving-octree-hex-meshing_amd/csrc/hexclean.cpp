@@ -359,22 +359,44 @@ void drop_small(int64_t nV, int64_t nH, const int32_t* H, uint8_t* flag) {
     if (piece[i] == which) flag[hmap[i]] = 1;
 }
 
+// Sets the error message without letting an allocation failure escape (the message is then
+// left empty).
+void set_err(const char* msg) noexcept {
+  try {
+    g_err = msg;
+  } catch (...) {
+    g_err.clear();
+  }
+}
+
+// Runs body and turns every exception into a -1 return with the message in g_err, so that
+// nothing is thrown across the C ABI.
+template <class F>
+int64_t catching(F&& body) noexcept {
+  try {
+    return body();
+  } catch (const Fail&) {
+    return -1;
+  } catch (const std::bad_alloc&) {
+    set_err("out of host memory");
+    return -1;
+  } catch (const std::exception& e) {  // e.g. a sort thread that could not start
+    set_err(e.what());
+    return -1;
+  } catch (...) {
+    set_err("unknown exception");
+    return -1;
+  }
+}
+
 template <class F>
 int64_t guarded(int64_t nV, int64_t nH, const int32_t* H, uint8_t* H_flag, F&& body) {
-  try {
+  return catching([&]() -> int64_t {
     check_hexes(nV, nH, H);
     if (nH > 0 && !H_flag) fail("H_flag is null");
     body();
     return count(nH, H_flag);
-  } catch (const Fail&) {
-    return -1;
-  } catch (const std::bad_alloc&) {
-    g_err = "out of host memory";
-    return -1;
-  } catch (const std::exception& e) {  // e.g. a sort thread that could not start
-    g_err = e.what();
-    return -1;
-  }
+  });
 }
 
 }  // namespace
@@ -384,7 +406,7 @@ extern "C" {
 const char* hexclean_error(void) { return g_err.c_str(); }
 
 int64_t hexclean_reorder_hex_mesh(int64_t nV, const double* V, int64_t nH, int32_t* H, int64_t* flipped) {
-  try {
+  return catching([&]() -> int64_t {
     check_hexes(nV, nH, H);
     if (nH > 0 && !V) fail("V is null");
     int64_t n = 0;
@@ -402,9 +424,7 @@ int64_t hexclean_reorder_hex_mesh(int64_t nV, const double* V, int64_t nH, int32
     }
     if (flipped) *flipped = n;
     return nH;
-  } catch (const Fail&) {
-    return -1;
-  }
+  });
 }
 
 int64_t hexclean_tagging_uneven_element(int64_t nV, int64_t nH, const int32_t* H, uint8_t* H_flag) {

@@ -86,9 +86,15 @@ struct hexopt_ctx {
   // tangent-plane refresh: folded rows without the plane terms
   DevBuf d_S0, d_b0;
   bool planes_refreshable = false;
+  std::vector<int> h_tids;          // plane rows' vertices (Morton numbering), ids_T order
+  // per-patch projection: the last patch assignment, its rows grouped by patch (each group in
+  // vertex order) and the groups' offsets
+  std::vector<int32_t> h_patch;
+  std::vector<int> patch_off;
+  DevBuf d_porder, d_pC, d_pstat;
   bool lines_refreshable = false;
   // FP32 optimiser state
-  DevBuf f_K, f_rhs, f_invd, f_d, f_r, f_z, f_p, f_p2, f_tmp, f_e, f_halo, d_Ld64, d_halo64b, d_ninv;
+  DevBuf f_K, f_rhs, f_invd, f_d, f_r, f_z, f_p, f_p2, f_tmp, f_e, f_halo, d_Ld64, d_halo64b, d_ninv, d_b64;
 };
 
 namespace {
@@ -431,6 +437,8 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
   rc |= upload(c, c->d_cids, cids);
   rc |= upload(c, c->d_cpos, vec(cs->C, (size_t)cs->nC * 3));
   rc |= upload(c, c->d_tids, tids);
+  c->h_tids = tids;
+  c->h_patch.clear();
   {
     // plane queries in the vertices' Morton order (for coherent closest-point traversals)
     std::vector<int> order(cs->nT);
@@ -758,6 +766,7 @@ static int ensure_state32(hexopt_ctx* c) {
   const size_t nh32 = std::max((size_t)6 * (c->L.nSlots + 1), (size_t)4 * (c->L.lv_gid.size() + 1));
   HIPCHK(c, c->f_halo.ensure(nh32 * sizeof(float)));
   HIPCHK(c, c->d_Ld64.ensure(n * sizeof(double)));
+  HIPCHK(c, c->d_b64.ensure(n * sizeof(double)));
   HIPCHK(c, c->d_ninv.ensure(sizeof(int)));
   HIPCHK(c, c->d_cgsave.ensure(cg_state_bytes()));
   return HEXOPT_OK;
@@ -831,21 +840,33 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
     if ((rc = cg32(c, sd, prox, c->f_rhs.as<float>(), d, opt->cg_iters, nullptr, nullptr))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->d_cgsave.p, c->d_cg.p, cg_state_bytes(), hipMemcpyDeviceToDevice, st));
     const int nb = nblocks(n);
-    const bool measure = opt->refine_iters > 0 ||
-                         (opt->residual_every > 0 && (it % opt->residual_every == 0 || it == iters - 1));
+    const bool every = opt->residual_every > 0 && (it % opt->residual_every == 0 || it == iters - 1);
+    const bool measure = opt->refine_iters > 0 || every;
+    double* b64 = c->d_b64.as<double>();
     if (measure) {
+      // FP64 right-hand side -dE/dx / 2 from the FP64 adjoint gradient at CURRENT (a_L = 0),
+      // then r = b64 - L d with FP64 sums over the stored operator
+      if (nL > 0) HIPCHK(c, hipMemsetAsync(c->d_aL.p, 0, (size_t)nL * sizeof(double), st));
+      launch_grad(64, c->tm, ro, cur, b64, M, sd, c->d_aL.as<double>(), c->d_halo.p, st);
+      launch_rhs64_from_grad(nV, b64, sd, cur, st);
       launch_apply32_f64(c->tm, ro, c->f_K.as<float>(), d, c->d_Ld64.as<double>(), prox, sd, c->d_halo.as<double>(),
                          st);
-      launch_residual64(n, c->f_rhs.as<float>(), c->d_Ld64.as<double>(), c->f_r.as<float>(),
-                        c->d_part[1].as<double>(), nb, scal + 6, st);
+      launch_residual64b(n, b64, c->d_Ld64.as<double>(), c->f_r.as<float>(), c->d_part[1].as<double>(),
+                         c->d_part[2].as<double>(), nb, scal + 6, st);
     }
     if (opt->refine_iters > 0) {
-      // FP32 copy of the FP64 residual -> correction e, d += e (the copy keeps the right-hand
-      // side apart from the CG's own r, z and p)
+      // FP32 copy of the FP64 residual -> correction e by FP32 CG, d += e (the copy keeps the
+      // right-hand side apart from the CG's own r, z and p)
       HIPCHK(c, hipMemcpyAsync(c->d_Ld64.p, c->f_r.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
       float* e = c->f_e.as<float>();
       if ((rc = cg32(c, sd, prox, (const float*)c->d_Ld64.p, e, opt->refine_iters, nullptr, nullptr))) return rc;
       launch_add_inplace(n, d, e, nb, st);
+      if (every) {
+        launch_apply32_f64(c->tm, ro, c->f_K.as<float>(), d, c->d_Ld64.as<double>(), prox, sd,
+                           c->d_halo.as<double>(), st);
+        launch_residual64b(n, b64, c->d_Ld64.as<double>(), nullptr, c->d_part[1].as<double>(),
+                           c->d_part[2].as<double>(), nb, scal + 8, st);
+      }
     }
     // a_L is the line unknowns' solution (as slim_m.cpp:501-503), used by the soft energy
     double* aL = c->d_aL.as<double>();
@@ -854,10 +875,10 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
     launch_max_step_dir32(cur, d, nV, c->tm.conn, nH, c->d_part[0].as<double>(), scal + 4, st);
     launch_energy_dir32(ro, cur, d, 0.0, nV, c->tm.conn, nH, M, sd, aL, c->d_part[0].as<double>(),
                         c->d_part[1].as<double>(), nsb, scal, st);
-    double hs[8];
+    double hs[10];
     int ninv = 0;
     std::vector<unsigned char> cgs(cg_state_bytes());
-    HIPCHK(c, hipMemcpyAsync(hs, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(hs, scal, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipMemcpyAsync(&ninv, c->d_ninv.p, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipMemcpyAsync(cgs.data(), c->d_cgsave.p, cgs.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipGetLastError());
@@ -865,8 +886,8 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
     int cg_it = 0, cg_done = 0;
     double cg_err = 0;
     read_cg_state(cgs.data(), &cg_it, &cg_err, &cg_done);
-    // |b| of the direction system: the CG state's rhs norm (FP64 sum of the FP32 rhs)
-    const double rhs_norm = std::sqrt(*(const double*)cgs.data());
+    // |b64| of the direction system, from the FP64 right-hand side
+    const double rhs_norm = std::sqrt(hs[7]);
     const double res64 = hs[6];
     const double tmin = hs[4];
     double h2[2] = {hs[0], hs[1]};
@@ -900,12 +921,53 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
       lg.energy = energy; lg.e_quality = eq; lg.e_soft = es; lg.step_bound = bound; lg.step = step;
       lg.cg_iters = cg_it; lg.cg_error = cg_err;
       lg.residual64 = !measure ? -1.0 : (rhs_norm > 0 ? std::sqrt(res64) / rhs_norm : 0.0);
+      lg.residual64_refined = !(every && opt->refine_iters > 0) ? -1.0
+                              : (rhs_norm > 0 ? std::sqrt(hs[8]) / rhs_norm : 0.0);
       lg.inverted_tets = ninv;
     }
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(st));
   if (E_out) *E_out = energy;
+  return HEXOPT_OK;
+}
+
+int hexopt_debug_operator(hexopt_ctx* c, const hexopt_params* prm, int precision, double* K_out) {
+  if (!c || !prm || !K_out || (precision != 32 && precision != 64)) return HEXOPT_EINVAL;
+  if (c->ref_mode < 0) return fail(c, HEXOPT_EINVAL, "hexopt_reference has not been called");
+  if (prm->energy != HEXOPT_SYMMETRIC_DIRICHLET) return fail(c, HEXOPT_EUNSUPPORTED, "SYMMETRIC_DIRICHLET only");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = ensure_state(c);
+  if (rc) return rc;
+  if ((rc = ensure_state32(c))) return rc;
+  const int nH = c->L.nH;
+  const double M = tet_mass(prm), prox = 0.00001 * prm->weight_opt;
+  hipStream_t st = c->stream;
+  SoftDev sd = soft_with(c, prm);
+  std::vector<double> K64;
+  std::vector<float> K32;
+  if (precision == 64) {
+    launch_local_step(c->tm, refop(c), c->d_cur.as<double>(), c->d_K.as<double>(), prm->energy, M, prm->exp_factor,
+                      prox, sd, c->d_rhs.as<double>(), c->d_invd.as<double>(), c->d_halo.as<double>(), st);
+    K64.resize((size_t)48 * nH);
+    HIPCHK(c, hipMemcpyAsync(K64.data(), c->d_K.p, K64.size() * 8, hipMemcpyDeviceToHost, st));
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->d_ninv.p, 0, sizeof(int), st));
+    launch_adj_local32(c->tm, refop(c), c->d_cur.as<double>(), c->f_K.as<float>(), M, prox, sd, c->f_rhs.as<float>(),
+                       c->f_invd.as<float>(), c->f_halo.as<float>(), c->d_part[0].as<double>(), c->d_ninv.as<int>(),
+                       st);
+    K32.resize((size_t)48 * nH);
+    HIPCHK(c, hipMemcpyAsync(K32.data(), c->f_K.p, K32.size() * 4, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(st));
+  // value m of hex h sits at (m / W * nH + h) * W + m % W with W = 16 / sizeof(value)
+  const int W = precision == 64 ? 2 : 4;
+  for (int h = 0; h < nH; ++h)
+    for (int m = 0; m < 48; ++m) {
+      const size_t at = ((size_t)(m / W) * nH + h) * W + m % W;
+      K_out[(size_t)c->L.hperm[h] * 48 + m] = precision == 64 ? K64[at] : (double)K32[at];
+    }
   return HEXOPT_OK;
 }
 
@@ -943,6 +1005,14 @@ int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch
         launch_local_step(tm, ro, c->d_cur.as<double>(), c->d_K.as<double>(), HEXOPT_SYMMETRIC_DIRICHLET, 1.0, 1.0,
                           1e-5, sd, c->d_rhs.as<double>(), c->d_invd.as<double>(), c->d_halo.as<double>(), st);
         break;
+      case 4:
+        // the adjoint energy + gradient pass alone, FP32 positions in, FP32 dE/dx out
+        launch_grad(32, tm, ro, c->d_X.p, c->d_G.p, 1.0, sd, c->d_aL.as<double>(), c->f_halo.p, st);
+        break;
+      case 5:
+        // the same in FP64, on the FP64 positions
+        launch_grad(64, tm, ro, c->d_cur.p, c->d_b64.p, 1.0, sd, c->d_aL.as<double>(), c->d_halo.p, st);
+        break;
       default:
         launch_adj_local32(tm, ro, c->d_cur.as<double>(), c->f_K.as<float>(), 1.0, 1e-5, sd, c->f_rhs.as<float>(),
                            c->f_invd.as<float>(), c->f_halo.as<float>(), c->d_part[0].as<double>(),
@@ -950,6 +1020,11 @@ int hexopt_time_kernel(hexopt_ctx* c, int which, int reps, double* ms_per_launch
     }
   };
   sd.nL = 0;
+  if (which == 4) {
+    HIPCHK(c, c->d_X.ensure((size_t)3 * c->L.nV * sizeof(float)));
+    HIPCHK(c, c->d_G.ensure((size_t)3 * c->L.nV * sizeof(float)));
+    launch_d2f(3 * c->L.nV, c->d_cur.as<double>(), c->d_X.as<float>(), st);
+  }
   if (which == 1) {
     // a fresh CG state (not converged) on the current FP32 system, so the pass runs in full
     HIPCHK(c, hipMemsetAsync(c->d_cg.p, 0, cg_state_bytes(), st));
@@ -1129,6 +1204,56 @@ int hexopt_project_planes(hexopt_ctx* c, hexopt_surface* s, double lamda_T) {
   c->sd.lamT = lamda_T;
   // no host round trip: the walk's stack holds the tree's depth plus two frames, and trees
   // deeper than the stack limit are refused at creation, so its overflow flag stays clear
+  HIPCHK(c, hipGetLastError());
+  return HEXOPT_OK;
+}
+
+int hexopt_project_patches(hexopt_ctx* c, int npatch, hexopt_surface* const* patches, const int32_t* patch_of_T,
+                           double lamda_T) {
+  if (!c || npatch <= 0 || !patches) return HEXOPT_EINVAL;
+  for (int p = 0; p < npatch; ++p) {
+    if (!patches[p]) return fail(c, HEXOPT_EINVAL, "null patch surface");
+    if (patches[p]->device != c->device) return fail(c, HEXOPT_EINVAL, "surface and context on different devices");
+  }
+  if (!c->planes_refreshable) return fail(c, HEXOPT_EUNSUPPORTED, "duplicate plane vertices");
+  const int nT = c->sd.nT;
+  if (nT == 0) return HEXOPT_OK;
+  if (!patch_of_T) return fail(c, HEXOPT_EINVAL, "null patch_of_T");
+  hipStream_t st = c->stream;
+  if ((int)c->h_patch.size() != nT || (int)c->patch_off.size() != npatch + 1 ||
+      std::memcmp(c->h_patch.data(), patch_of_T, (size_t)nT * sizeof(int32_t)) != 0) {
+    // rows grouped by patch, each group in the vertices' Morton order (coherent walks)
+    for (int i = 0; i < nT; ++i)
+      if (patch_of_T[i] < 0 || patch_of_T[i] >= npatch) return fail(c, HEXOPT_EINVAL, "patch index out of range");
+    std::vector<int> order(nT);
+    for (int i = 0; i < nT; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return patch_of_T[a] != patch_of_T[b] ? patch_of_T[a] < patch_of_T[b] : c->h_tids[a] < c->h_tids[b];
+    });
+    std::vector<int> off(npatch + 1, 0);
+    for (int i = 0; i < nT; ++i) off[patch_of_T[i] + 1]++;
+    for (int p = 0; p < npatch; ++p) off[p + 1] += off[p];
+    int rc = upload(c, c->d_porder, order);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(st));
+    c->h_patch.assign(patch_of_T, patch_of_T + nT);
+    c->patch_off = off;
+  }
+  HIPCHK(c, c->d_pC.ensure((size_t)nT * 24));
+  HIPCHK(c, c->d_pstat.ensure(sizeof(int)));
+  HIPCHK(c, hipMemsetAsync(c->d_pstat.p, 0, sizeof(int), st));
+  // each patch's rows against that patch's tree (dirty_graph_projection's loop over Qfg.Rs):
+  // normal_T = pseudonormal N, dis_T = N . closest point
+  for (int p = 0; p < npatch; ++p) {
+    const int n = c->patch_off[p + 1] - c->patch_off[p];
+    if (n > 0)
+      launch_closest(patches[p]->view, n, c->d_cur.as<double>(), 1, c->L.nV, c->d_tids.as<int>(), nullptr, nullptr,
+                     c->d_pC.as<double>(), const_cast<double*>(c->sd.t_n), nullptr, c->d_pstat.as<int>(), st,
+                     c->d_porder.as<int>() + c->patch_off[p]);
+  }
+  launch_refold_planes(c->sd, nT, c->d_pC.as<double>(), const_cast<double*>(c->sd.t_d), c->d_S0.as<double>(),
+                       c->d_b0.as<double>(), const_cast<double*>(c->sd.S), const_cast<double*>(c->sd.b), lamda_T, st);
+  c->sd.lamT = lamda_T;
   HIPCHK(c, hipGetLastError());
   return HEXOPT_OK;
 }

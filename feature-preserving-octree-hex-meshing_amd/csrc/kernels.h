@@ -153,6 +153,13 @@ void launch_cg32_iteration(const TileMesh& tm, const RefOp& ro, const float* K, 
 void launch_residual64(int n, const float* b, const double* Ld, float* r32, double* part, int nb, double* out,
                        hipStream_t st);
 void launch_f2d(int n, const float* a, double* b, hipStream_t st);
+void launch_d2f(int n, const double* a, float* b, hipStream_t st);
+// b = -g / 2 on the vertex rows of g (the FP64 adjoint gradient, in place) and the a_L rows at
+// a_L = 0: the FP64 right-hand side of the direction system (3 nV + nL entries)
+void launch_rhs64_from_grad(int nV, double* g, const SoftDev& sd, const double* X, hipStream_t st);
+// out2 = { |b - L d|^2, |b|^2 } in FP64; r32 (nullable) = FP32 copy of b - L d
+void launch_residual64b(int n, const double* b, const double* Ld, float* r32, double* part_r, double* part_b, int nb,
+                        double* out2, hipStream_t st);
 void launch_add_inplace(int n, float* d, const float* e, int nb, hipStream_t st);
 void launch_max_step_dir32(const double* cur, const float* d, int nV, const int* conn, int nH, double* part,
                            double* out, hipStream_t st);

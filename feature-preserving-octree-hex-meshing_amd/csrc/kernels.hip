@@ -751,28 +751,39 @@ struct AdjLocalOp32 {
     int vid[8];
     double pd[8][3];
     gather8(X, nV, conn, nH, h, vid, pd);
+    // FP32 coordinates relative to corner 0, differenced in FP64: the edges keep FP32's
+    // relative precision (rounding the absolute positions first would leave them only
+    // |x| / h times less precise)
     float p[8][3];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) p[k][d] = (float)pd[k][d];
+      for (int d = 0; d < 3; ++d) p[k][d] = (float)(pd[k][d] - pd[0][d]);
     double e = 0.0;
     int inv = 0;
-    // FP64 orientation test of the corner tets first, so the FP64 corners are dead before the
-    // FP32 loop: sign of det J = det(E B). For a cube reference B = invs * (a permutation
-    // with determinant +1 for every corner) and invs > 0, so the sign is that of det E.
-#pragma unroll
+    unsigned iso = 0;
+    // FP64 tests of the corner tets first, so the FP64 corners are dead before the FP32 loop:
+    // the orientation, sign of det J = det(E B) (for a cube reference B = invs * (a
+    // permutation with determinant +1 for every corner) and invs > 0, so the sign is that of
+    // det E), and the reference's m = 1 branch of the weights (slim_m.cpp:351-353,
+    // |s_i - 1| < 1e-8), taken for a tet when |J J^T - I|_F < 2e-8 puts every singular value
+    // inside it -- FP32 cannot resolve that threshold.
+#pragma unroll 1
     for (int c = 0; c < 8; ++c) {
-      double E64[3][3];
+      double E64[3][3], B64[3][3], J64[3][3];
       tet_edges(pd, c, E64);
-      if (ro.mode == REF_CUBE) {
-        inv += det3(E64) < 0.0;
-      } else {
-        double B64[3][3], J64[3][3];
-        tet_B(ro, nH, h, c, B64);
-        matmul3(E64, B64, J64);
-        inv += det3(J64) < 0.0;
-      }
+      tet_B(ro, nH, h, c, B64);
+      matmul3(E64, B64, J64);
+      inv += (ro.mode == REF_CUBE ? det3(E64) : det3(J64)) < 0.0;
+      double f2 = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double g = J64[i][0] * J64[j][0] + J64[i][1] * J64[j][1] + J64[i][2] * J64[j][2] - (i == j ? 1.0 : 0.0);
+          f2 += g * g;
+        }
+      iso |= (f2 < 4e-16 ? 1u : 0u) << c;
     }
 #pragma unroll 1
     for (int c = 0; c < 8; ++c) {
@@ -780,7 +791,14 @@ struct AdjLocalOp32 {
       tet_B(ro, nH, h, c, B);
       tet_edges(p, c, E);
       matmul3(E, B, J);
-      slim_weights_sd_stable(J, W);
+      if ((iso >> c) & 1u) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) W[i][j] = i == j ? 1.0f : 0.0f;
+      } else {
+        slim_weights_sd_stable(J, W);
+      }
       e += M * (double)sd_energy_grad(J, P);
       float Kt[3][3];
 #pragma unroll
@@ -1627,6 +1645,43 @@ __global__ void residual64_kernel(int n, const float* b, const double* Ld, float
   if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 
+// FP64 right-hand side of the direction system from the FP64 adjoint gradient g (in place):
+// b = -g / 2 on the 3 nV vertex rows, and the a_L rows at a_L = 0, lamL t . (x_v - o).
+__global__ void rhs64_from_grad_kernel(int n3, double* g, SoftDev sd, int nV, const double* X) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n3 + sd.nL; i += gridDim.x * blockDim.x) {
+    if (i < n3) {
+      g[i] *= -0.5;
+    } else {
+      const int l = i - n3, v = sd.lv[l];
+      const double* t = sd.lt + (size_t)l * 3;
+      const double* o = sd.lo + (size_t)l * 3;
+      double s = 0;
+      for (int d = 0; d < 3; ++d) s += t[d] * (X[(size_t)d * nV + v] - o[d]);
+      g[i] = sd.lamL * s;
+    }
+  }
+}
+
+// r = b - L d in FP64 with an FP64 right-hand side; FP32 copy of r; partial sums of r^2 and b^2
+__global__ void residual64b_kernel(int n, const double* b, const double* Ld, float* r32, double* part_r,
+                                   double* part_b) {
+  __shared__ double red[16];
+  double s = 0, sb = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double r = b[i] - Ld[i];
+    if (r32) r32[i] = (float)r;
+    s += r * r;
+    sb += b[i] * b[i];
+  }
+  double a = block_sum(s, red);
+  double c = block_sum(sb, red);
+  if (threadIdx.x == 0) { part_r[blockIdx.x] = a; part_b[blockIdx.x] = c; }
+}
+
+__global__ void d2f_kernel(int n, const double* a, float* b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+
 __global__ void f2d_kernel(int n, const float* a, double* b) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (double)a[i];
 }
@@ -2247,6 +2302,19 @@ void launch_residual64(int n, const float* b, const double* Ld, float* r32, doub
   hipLaunchKernelGGL(sum_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part, nb, out);
 }
 
+void launch_rhs64_from_grad(int nV, double* g, const SoftDev& sd, const double* X, hipStream_t st) {
+  const int n = 3 * nV + sd.nL;
+  hipLaunchKernelGGL(rhs64_from_grad_kernel, dim3(std::max(1, std::min((n + 255) / 256, 4096))), dim3(256), 0, st,
+                     3 * nV, g, sd, nV, X);
+}
+
+void launch_residual64b(int n, const double* b, const double* Ld, float* r32, double* part_r, double* part_b, int nb,
+                        double* out2, hipStream_t st) {
+  hipLaunchKernelGGL(residual64b_kernel, dim3(nb), dim3(256), 0, st, n, b, Ld, r32, part_r, part_b);
+  hipLaunchKernelGGL(energy_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part_r, nb, (const double*)part_b,
+                     nb, out2);
+}
+
 void launch_add_inplace(int n, float* d, const float* e, int nb, hipStream_t st) {
   hipLaunchKernelGGL(add_inplace_kernel, dim3(nb), dim3(256), 0, st, n, d, e);
 }
@@ -2262,6 +2330,11 @@ void read_cg_state(const void* host_state, int* iters, double* tol_error, int* d
   *iters = S->iters;
   *tol_error = S->tol_error;
   *done = S->done;
+}
+
+void launch_d2f(int n, const double* a, float* b, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(d2f_kernel, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, st, n, a, b);
 }
 
 void launch_f2d(int n, const float* a, double* b, hipStream_t st) {

@@ -24,7 +24,7 @@ EXPORTS = ["hexopt_create", "hexopt_create_error", "hexopt_destroy", "hexopt_las
            "hexopt_curves_create", "hexopt_curves_error", "hexopt_curves_destroy", "hexopt_project_lines",
            "hexopt_refresh_lines", "hexopt_metro_hausdorff", "hexopt_glue_setup", "hexopt_glue_solve",
            "hexopt_octree_build", "hexopt_octree_error", "hexopt_octree_destroy", "hexopt_octree_sizes",
-           "hexopt_octree_copy", "hexopt_octree_refine"]
+           "hexopt_octree_copy", "hexopt_octree_refine", "hexopt_project_patches", "hexopt_debug_operator"]
 
 _LIB = None
 
@@ -61,7 +61,7 @@ class Opt32Options(C.Structure):
 class Opt32Log(C.Structure):
     _fields_ = [("energy", C.c_double), ("e_quality", C.c_double), ("e_soft", C.c_double),
                 ("step_bound", C.c_double), ("step", C.c_double), ("cg_iters", C.c_int), ("cg_error", C.c_double),
-                ("residual64", C.c_double), ("inverted_tets", C.c_int)]
+                ("residual64", C.c_double), ("inverted_tets", C.c_int), ("residual64_refined", C.c_double)]
 
 
 def lib():
@@ -110,6 +110,9 @@ def lib():
         L.hexopt_surface_last_error.restype = C.c_char_p
         L.hexopt_surface_last_error.argtypes = [C.c_void_p]
         L.hexopt_surface_destroy.argtypes = [C.c_void_p]
+        L.hexopt_debug_operator.restype = C.c_int
+        L.hexopt_project_patches.restype = C.c_int
+        L.hexopt_project_patches.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double]
         _LIB = L
     return _LIB
 
@@ -273,6 +276,14 @@ class HexOpt:
                                             _ptr(Lp)), "debug_system")
         return rhs, invd, Lp
 
+    def debug_operator(self, precision=64, weight_opt=1.0):
+        """Per-hex, per-corner-tet SLIM weight blocks (nH, 8, 6) from the FP64 or FP32 local step."""
+        prm = self.params(SYMMETRIC_DIRICHLET, weight_opt)
+        K = np.zeros((self.nH, 8, 6))
+        self._chk(lib().hexopt_debug_operator(C.c_void_p(self.h), C.byref(prm), precision, _ptr(K)),
+                  "debug_operator")
+        return K
+
     def opt32_solve(self, iters, cg_iters=20, refine_iters=0, weight_opt=1.0, residual_every=1):
         """FP32 + adjoint optimiser iterations from CURRENT (include/hexopt.h)."""
         prm = self.params(SYMMETRIC_DIRICHLET, weight_opt)
@@ -282,13 +293,21 @@ class HexOpt:
         self._chk(lib().hexopt_opt32_solve(C.c_void_p(self.h), C.byref(prm), C.byref(opt), iters, logs, C.byref(E)),
                   "opt32_solve")
         cols = ["energy", "e_quality", "e_soft", "step_bound", "step", "cg_iters", "cg_error", "residual64",
-                "inverted_tets"]
+                "inverted_tets", "residual64_refined"]
         log = np.array([[getattr(l, k) for k in cols] for l in logs[:iters]]).reshape(iters, len(cols))
         return E.value, log
 
     def project_planes(self, surface, lamT):
         self._chk(lib().hexopt_project_planes(C.c_void_p(self.h), C.c_void_p(surface.h), C.c_double(lamT)),
                   "project_planes")
+
+    def project_patches(self, patches, patch_of_T, lamT):
+        """Tangent planes of the plane vertices from their patches' surfaces (a list of
+        Surface); patch_of_T indexes that list, one entry per ids_T row."""
+        arr = (C.c_void_p * len(patches))(*[p.h for p in patches])
+        pt = _i(patch_of_T)
+        self._chk(lib().hexopt_project_patches(C.c_void_p(self.h), len(patches), arr, _ptr(pt, C.c_int32),
+                                               C.c_double(lamT)), "project_patches")
 
     def project_lines(self, curves, ids, curve_of, which=CURRENT):
         """origin_L and Axa_L (n x 3) of LINE vertices ids on curves curve_of."""

@@ -364,3 +364,140 @@ def hex_boundary_triangles(H):
     _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
     B = Q[cnt[inv.ravel()] == 1]
     return np.concatenate([B[:, [0, 1, 2]], B[:, [2, 3, 0]]]).astype(np.int32)
+
+
+def octree_frame(SV, num_voxels):
+    """octree_mesh's frame for a surface (grid_hex_meshing.cpp:463-491, no scaffold layers):
+    voxel size (largest bounding-box extent / num_voxels), origin (the box's min corner),
+    grid (next power of two of each extent in voxels) and the mesh_transform that centres the
+    grid on the box. Returns (voxel_size, origin, grid, mesh_transform)."""
+    lo, hi = SV.min(axis=0), SV.max(axis=0)
+    vs = float((hi - lo).max()) / num_voxels
+    grid = np.array([1 << int(np.ceil(np.log2(np.ceil(e / vs)))) for e in (hi - lo)], dtype=np.int64)
+    mt = (lo + hi) / 2 - (lo + (lo + vs * grid.astype(np.float64))) * 0.5
+    return vs, lo, grid, mt
+
+
+def _leaf_keys(x, y, z, level):
+    return (x.astype(np.int64) | (y.astype(np.int64) << 16) | (z.astype(np.int64) << 32) |
+            (level.astype(np.int64) << 48))
+
+
+# per face of HEX_FACE_TABLE: the axis it is normal to and whether it lies on the far side
+_FACE_AXIS = np.array([2, 2, 1, 0, 1, 0])
+_FACE_FAR = np.array([0, 1, 0, 0, 1, 1])
+
+
+def octree_leaf_boundary(leaves, H):
+    """Boundary quads of a hex mesh made of octree leaves (n x 4 rows x, y, z, extent in grid
+    units; H their corner node ids in Hybrid::vs order), which may hold hanging nodes where a
+    leaf meets leaves of another size. A leaf face is a boundary face when the point just
+    outside it at in-face fractions (0.3, 0.4) lies in no leaf of the mesh: with 2:1 graded
+    leaves that point is interior to whichever leaf covers it, so faces shared with smaller
+    or larger leaves count as inner faces. Returns the boundary quads (m, 4) in
+    HEX_FACE_TABLE vertex order."""
+    leaves = np.asarray(leaves, dtype=np.int64)
+    ext = leaves[:, 3]
+    lev = np.round(np.log2(ext)).astype(np.int64)
+    keys = np.sort(_leaf_keys(leaves[:, 0], leaves[:, 1], leaves[:, 2], lev))
+    levels = np.unique(lev)
+    out = []
+    for f in range(6):
+        a, far = _FACE_AXIS[f], _FACE_FAR[f]
+        u, v = [d for d in range(3) if d != a]
+        p = leaves[:, :3].astype(np.float64)
+        p[:, u] += 0.3 * ext
+        p[:, v] += 0.4 * ext
+        p[:, a] += ext + 0.25 if far else -0.25
+        covered = np.zeros(leaves.shape[0], dtype=bool)
+        for L in levels:
+            e = float(1 << int(L))
+            c = np.floor(p / e) * e
+            ok = (c >= 0).all(axis=1)
+            k = _leaf_keys(c[:, 0].astype(np.int64), c[:, 1].astype(np.int64), c[:, 2].astype(np.int64),
+                           np.full(c.shape[0], L))
+            i = np.searchsorted(keys, k)
+            hit = ok & (i < keys.size) & (keys[np.minimum(i, keys.size - 1)] == k)
+            covered |= hit
+        out.append(H[~covered][:, HEX_FACE_TABLE[f]])
+    return np.concatenate(out).astype(np.int32)
+
+
+def box_surface(n):
+    """The unit cube [-0.5,0.5]^3 as a closed, outward-oriented triangle mesh: each face an
+    n x n quad grid split along one diagonal (12 n^2 triangles, shared edge vertices)."""
+    t = np.linspace(-0.5, 0.5, n + 1)
+    V, F, index = [], [], {}
+
+    def vid(p):
+        key = tuple(np.round(np.asarray(p) * 2 * n).astype(np.int64))
+        if key not in index:
+            index[key] = len(V)
+            V.append(p)
+        return index[key]
+
+    for axis in range(3):
+        u, v = [d for d in range(3) if d != axis]
+        for side in (-0.5, 0.5):
+            ids = np.empty((n + 1, n + 1), dtype=np.int64)
+            for i in range(n + 1):
+                for j in range(n + 1):
+                    p = [0.0, 0.0, 0.0]
+                    p[axis], p[u], p[v] = side, t[i], t[j]
+                    ids[i, j] = vid(p)
+            for i in range(n):
+                for j in range(n):
+                    a, b, c, d = ids[i, j], ids[i + 1, j], ids[i + 1, j + 1], ids[i, j + 1]
+                    F.append([a, b, c])
+                    F.append([a, c, d])
+    V = np.array(V, dtype=np.float64)
+    F = np.array(F, dtype=np.int32)
+    # outward orientation: the normal points away from the centre
+    nrm = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    flip = (nrm * V[F].mean(axis=1)).sum(axis=1) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return V, F
+
+
+def random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def box_union_surface(nboxes=8, n=46, seed=0):
+    """The union of `nboxes` seeded, randomly rotated boxes (sides U[0.3, 0.8], centres
+    U[-0.5, 0.5]^3) as the soup of their closed surfaces (12 n^2 triangles each; ~203k at
+    n = 46). The soup is the union for the reference's inside tests: compute_sign counts
+    signed +z crossings (voxelization.h:221-272), so a point inside k boxes gets a count of
+    -k, negative exactly when it lies in at least one box. Returns V, F and each box's
+    (rotation, sides, centre)."""
+    rng = np.random.default_rng(seed)
+    BV, BF = box_surface(n)
+    Vs, Fs, boxes = [], [], []
+    off = 0
+    for _ in range(nboxes):
+        sides = rng.uniform(0.3, 0.8, size=3)
+        centre = rng.uniform(-0.5, 0.5, size=3)
+        Rm = random_rotation(rng)
+        Vs.append((BV * sides) @ Rm.T + centre)
+        Fs.append(BF + off)
+        off += BV.shape[0]
+        boxes.append((Rm, sides, centre))
+    return np.concatenate(Vs), np.concatenate(Fs).astype(np.int32), boxes
+
+
+def surface_samples(V, F, n, seed=0):
+    """n seeded points distributed uniformly by area over the triangles (V, F): a triangle
+    drawn with probability proportional to its area, then a uniform point in it
+    (r1, r2 uniform: sqrt(r1) folds the square onto the triangle)."""
+    rng = np.random.default_rng(seed)
+    a = 0.5 * np.linalg.norm(np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]]), axis=1)
+    t = rng.choice(F.shape[0], size=n, p=a / a.sum())
+    r1, r2 = rng.random(n), rng.random(n)
+    s = np.sqrt(r1)
+    A, B, C = V[F[t, 0]], V[F[t, 1]], V[F[t, 2]]
+    return (1 - s)[:, None] * A + (s * (1 - r2))[:, None] * B + (s * r2)[:, None] * C

@@ -134,34 +134,46 @@ int hexopt_glue_solve(hexopt_ctx* ctx, const hexopt_params* prm, int max_iters, 
  * SLIM iteration as hexopt_slim_solve in its adjoint form). Per iteration: one FP32 pass
  * over the elements gives the energy, -dE/dx / 2 (the hand-written adjoint), the SLIM
  * weights K and the Jacobi diagonal; the direction solves L d = -dE/dx / 2 (equal to the
- * SLIM global step taken from CURRENT) by FP32 CG with FP64 dot products, whose residual is
- * recomputed in FP64 and corrected by a second FP32 solve; the step bound (FP64 cubic), the
+ * SLIM global step taken from CURRENT) by FP32 CG with FP64 dot products. Iterative
+ * refinement (refine_iters > 0): the right-hand side b64 = -dE/dx / 2 is recomputed by the
+ * FP64 adjoint, the residual b64 - L d is formed with FP64 sums over the stored operator, and
+ * an FP32 CG solve of L e = that residual gives d += e. The step bound (FP64 cubic), the
  * corner-determinant sign test and the energy sums are FP64; the line search is the
  * reference's bisection. Symmetric Dirichlet only. */
 typedef struct {
   int cg_iters;        /* FP32 CG iterations of the direction solve (the reference uses 20) */
   int refine_iters;    /* FP32 CG iterations of the FP64-residual correction (0: none) */
-  int residual_every;  /* without correction, the FP64 residual is still measured on every
-                          residual_every-th iteration and the last (0: never); a correction
-                          always measures it */
+  int residual_every;  /* the FP64 residual is measured on every residual_every-th iteration and
+                          the last (0: never); a correction always measures the one it starts
+                          from */
 } hexopt_opt32_options;
 
 typedef struct {
   double energy, e_quality, e_soft, step_bound, step;
   int cg_iters;
   double cg_error;     /* relative residual estimate of the FP32 solve */
-  double residual64;   /* |b - L d| / |b| computed in FP64 before the correction (-1: not measured) */
+  double residual64;   /* |b64 - L d| / |b64| in FP64 before the correction (-1: not measured) */
   int inverted_tets;   /* corner tets with negative determinant at the iterate (FP64 test) */
+  double residual64_refined; /* the same after the correction (-1: no correction, or not an
+                                iteration residual_every measures) */
 } hexopt_opt32_log;
 
 int hexopt_opt32_solve(hexopt_ctx* ctx, const hexopt_params* prm, const hexopt_opt32_options* opt, int iters,
                        hexopt_opt32_log* log, double* E);
 
+/* Diagnostic view of the SLIM weights (no reference counterpart; used to state the FP32
+ * path's error): runs the local step at CURRENT in FP64 (update_weights_and_closest_rotations,
+ * slim_m.cpp:229-377) or in FP32 (hexopt_opt32_solve's element pass) and returns, per hex in
+ * caller order and per corner tet c, the 6 entries M W^T W (xx, yy, zz, xy, xz, yz) scaled by
+ * the tet's squared reference factor: K[h*48 + c*6 + j]. Symmetric Dirichlet only. */
+int hexopt_debug_operator(hexopt_ctx* ctx, const hexopt_params* prm, int precision, double* K);
+
 /* Average duration (ms, HIP events on the context's stream) of one launch of a kernel:
  * 0 FP64 CG operator tile pass, 1 FP32 fused CG operator pass (cg32_apply, forming the previous
  * iteration's beta step as it does inside the CG loop), 2 FP64 local step
- * tile pass, 3 FP32 adjoint/local tile pass. Needs a prior solve of the same precision
- * (operands resident). */
+ * tile pass, 3 FP32 adjoint/local tile pass, 4 the FP32 energy + gradient pass of
+ * hexopt_energy_grad (CURRENT rounded to FP32), 5 the same in FP64. Modes 0-3 need a prior
+ * solve of the same precision (operands resident). */
 int hexopt_time_kernel(hexopt_ctx* ctx, int which, int reps, double* ms_per_launch);
 
 /* Unnormalised SLIM energy of CURRENT: quality part and soft part (compute_energy_total,
@@ -212,6 +224,15 @@ int hexopt_surface_closest(hexopt_surface* s, int nq, const double* P, double* S
  * context's constraints (ids_T) are projected at CURRENT; normal_T = pseudonormal,
  * dis_T = normal_T . closest point, and the plane weight becomes lamda_T. */
 int hexopt_project_planes(hexopt_ctx* ctx, hexopt_surface* s, double lamda_T);
+
+/* The patch loop of dirty_graph_projection (grid_hex_meshing.cpp:4034-4081) on the device:
+ * plane vertex ids_T[i] (of hexopt_set_constraints, at CURRENT) is projected onto surface
+ * patches[patch_of_T[i]] -- its patch's own tree (md.tri_Trees[i] for Qfg.Rs[i]), or the whole
+ * input surface for a dirty vertex -- giving normal_T = pseudonormal and dis_T = normal_T .
+ * closest point; the plane weight becomes lamda_T. patch_of_T has nT entries in ids_T order,
+ * each in [0, npatch); it is checked and grouped again only when it changes between calls. */
+int hexopt_project_patches(hexopt_ctx* ctx, int npatch, hexopt_surface* const* patches, const int32_t* patch_of_T,
+                           double lamda_T);
 
 /* Metro Hausdorff distance between two triangle meshes with vertex sampling only, as the
  * reference's compute(const Mesh&, const Mesh&, bbox_diagonal, max, mean)

@@ -29,6 +29,7 @@
 #include "igl/per_vertex_normals.h"
 #include "igl/per_edge_normals.h"
 #include "igl/signed_distance.h"
+#include <chrono>
 #include <cstdio>
 #include <cstdint>
 #include <vector>
@@ -527,7 +528,10 @@ int main(int argc, char** argv) {
   } else if (mode == 1) {
     // ts.fc.ids_T lists the vertices to project (projection_smooth, grid_hex_meshing.cpp:3760-3781)
     double LAMDA0 = lamT;
+    std::vector<double> t_iter;
     for (int it = 0; it < n_outer; ++it) {
+      // wall time of the loop body alone: reference mesh, projection, slim_m_opt
+      const auto t0 = std::chrono::steady_clock::now();
       compute_referenceMesh(ts.V, m.Hs, H_flag, Hids, ts.RT, cube != 0);
       if (nsv > 0 && nTc > 0) {
         Eigen::MatrixXd Ps(nTc, 3);
@@ -549,12 +553,14 @@ int main(int argc, char** argv) {
       for (int k = 0; k < n_inner; ++k) slim_solve2(data, 1, -1, false);
       ts.UV = data.V_o;
       ts.V = ts.UV;
+      t_iter.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
       log.push_back(data.energy); log.push_back(data.engery_quality); log.push_back(data.engery_soft);
       log.push_back(fc.lamda_T);
       fc.lamda_T = std::min(LAMDA0 * std::max(data.engery_quality / data.engery_soft * fc.lamda_T, 1.0), lamT_bound);
       auto f = flat3(ts.UV);
       Vtrace.insert(Vtrace.end(), f.begin(), f.end());
     }
+    out.putd("t_iter", t_iter);
   }
   out.putd("log", log);
   out.putd("Vtrace", Vtrace);

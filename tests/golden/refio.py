@@ -42,31 +42,36 @@ def read_out(path):
     return out
 
 
-def run_slim(mode, V, H, energy=2, n_outer=1, n_inner=1, cube=True, projection=False, weight_opt=1.0,
-             lamT=0.0, lamT_bound=1e8, lamC=0.0, lamL=0.0, lamProj=1e5, hflag=None, b=None, bc=None,
-             idsT=None, normalT=None, disT=None, idsC=None, Cpos=None, idsL=None, axaL=None, originL=None,
-             surfV=None, surfF=None):
+def write_slim_input(path, mode, V, H, energy=2, n_outer=1, n_inner=1, cube=True, projection=False, weight_opt=1.0,
+                     lamT=0.0, lamT_bound=1e8, lamC=0.0, lamL=0.0, lamProj=1e5, hflag=None, b=None, bc=None,
+                     idsT=None, normalT=None, disT=None, idsC=None, Cpos=None, idsL=None, axaL=None, originL=None,
+                     surfV=None, surfF=None):
+    """The driver's input file (slim_ref_driver.cpp's main, modes 0, 1 and 3)."""
     nV, nH = V.shape[0], H.shape[0]
     e3 = np.zeros((0, 3)); ei = np.zeros(0, dtype=np.int32)
     hflag = np.ones(nH, dtype=np.int32) if hflag is None else hflag
+    with open(path, "wb") as f:
+        _w(f, "i", mode)
+        _w(f, "iiiiiii", nV, nH, energy, n_outer, n_inner, int(cube), int(projection))
+        _w(f, "dddddd", weight_opt, lamT, lamT_bound, lamC, lamL, lamProj)
+        _arr(f, V, "<f8"); _arr(f, H, "<i4"); _arr(f, hflag, "<i4")
+        for ids, arrs in ((b, (bc,)), (idsT, (normalT, disT)), (idsC, (Cpos,)), (idsL, (axaL, originL))):
+            ids = ei if ids is None else ids
+            _w(f, "i", len(ids)); _arr(f, ids, "<i4")
+            for a in arrs:
+                if a is None:
+                    a = e3[:0] if ids is None else np.zeros((len(ids), 3))
+                _arr(f, a, "<f8")
+        if surfV is None:
+            _w(f, "ii", 0, 0)
+        else:
+            _w(f, "ii", surfV.shape[0], surfF.shape[0]); _arr(f, surfV, "<f8"); _arr(f, surfF, "<i4")
+
+
+def run_slim(mode, V, H, **kw):
     with tempfile.TemporaryDirectory() as td:
         pin, pout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-        with open(pin, "wb") as f:
-            _w(f, "i", mode)
-            _w(f, "iiiiiii", nV, nH, energy, n_outer, n_inner, int(cube), int(projection))
-            _w(f, "dddddd", weight_opt, lamT, lamT_bound, lamC, lamL, lamProj)
-            _arr(f, V, "<f8"); _arr(f, H, "<i4"); _arr(f, hflag, "<i4")
-            for ids, arrs in ((b, (bc,)), (idsT, (normalT, disT)), (idsC, (Cpos,)), (idsL, (axaL, originL))):
-                ids = ei if ids is None else ids
-                _w(f, "i", len(ids)); _arr(f, ids, "<i4")
-                for a in arrs:
-                    if a is None:
-                        a = e3[:0] if ids is None else np.zeros((len(ids), 3))
-                    _arr(f, a, "<f8")
-            if surfV is None:
-                _w(f, "ii", 0, 0)
-            else:
-                _w(f, "ii", surfV.shape[0], surfF.shape[0]); _arr(f, surfV, "<f8"); _arr(f, surfF, "<i4")
+        write_slim_input(pin, mode, V, H, **kw)
         subprocess.run([REF_BIN, pin, pout], check=True, stdout=subprocess.DEVNULL)
         return read_out(pout)
 

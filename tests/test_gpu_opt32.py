@@ -39,8 +39,9 @@ def _compare_first_iteration(h64, h32, wo, disp_tol=None):
     assert int(log32[0, 5]) == int(log64[0, 5])                    # CG iterations
     # same energy: to 5e-5, or to 1% of the iteration's decrease on strongly inverted meshes
     assert abs(E32 - E64) < max(5e-5 * abs(E64), 1e-2 * (E0 - E64))
-    # the FP32 CG's recurrence residual is the true residual, recomputed in FP64 ...
-    assert log32[0, 7] == pytest.approx(log32[0, 6], rel=1e-2)
+    # the FP32 CG's recurrence residual is the true residual, recomputed in FP64 against the
+    # FP64 adjoint's right-hand side (which differs from the FP32 one by FP32 rounding) ...
+    assert abs(log32[0, 7] - log32[0, 6]) <= 1e-2 * log32[0, 6] + 3e-5
     # ... and of the size the FP64 CG reaches in as many steps
     assert log32[0, 7] < 5 * log64[0, 6] + 1e-6
     if disp_tol is not None:
@@ -104,3 +105,46 @@ def test_vertex_without_hexes():
     kw = dict(idsC=np.array([V.shape[0] - 1], dtype=np.int32), Cpos=np.array([[2.5, 2.0, 1.5]]), lamC=10.0)
     wo = float(c["weight_opt"])
     _compare_first_iteration(make(c, V=V, kw=kw), make(c, V=V, kw=kw), wo, 1e-4)
+
+
+def _direction(h, X0, log):
+    """The iteration's direction d from x' = x + step d (vertex rows, flattened [x y z])."""
+    step = float(log[0, 4])
+    assert step > 0
+    return ((h.get_positions() - X0) / step).T.reshape(-1)
+
+
+def test_refinement_moves_toward_fp64_solve():
+    """Iterative refinement: the residual against the FP64 right-hand side shrinks after the
+    correction, and the direction moves toward the converged solve of the FP64 system
+    (formed by hexopt_debug_system with the FP64 weights, solved here by CG to 1e-12)."""
+    from scipy.sparse.linalg import LinearOperator, cg
+    c = load("slim_grid6_features")
+    wo = float(c["weight_opt"])
+    h64, h0, h1 = make(c), make(c), make(c)
+    X0 = h64.get_positions()
+    nV = X0.shape[0]
+    nL = int(c["idsL"].size)
+    n = 3 * nV + nL
+    x0 = np.concatenate([X0.T.reshape(-1), np.zeros(nL)])
+    rhs, invd, Lx = h64.debug_system(x0, weight_opt=wo)
+    b = rhs - Lx                                      # the direction system's right-hand side
+    # the FP64 adjoint gives the same right-hand side: -dE/dx / 2 at a_L = 0
+    g = h64.energy_grad(X0, weight_opt=wo, aL=np.zeros(nL))
+    assert np.abs(-0.5 * g.T.reshape(-1) - b[:3 * nV]).max() < 1e-9 * np.abs(b).max()
+    op = LinearOperator((n, n), matvec=lambda p: h64.debug_system(p, weight_opt=wo)[2], dtype=np.float64)
+    dstar, info = cg(op, b, rtol=1e-12, atol=0.0, maxiter=5000, M=LinearOperator((n, n), matvec=lambda r: invd * r))
+    assert info == 0
+    _, log0 = h0.opt32_solve(1, weight_opt=wo, refine_iters=0, residual_every=1)
+    _, log1 = h1.opt32_solve(1, weight_opt=wo, refine_iters=10, residual_every=1)
+    assert log0[0, 9] == -1.0 and log1[0, 9] >= 0.0
+    d0, d1 = _direction(h0, X0, log0), _direction(h1, X0, log1)
+    ds = dstar[:3 * nV]
+    e0, e1 = np.linalg.norm(d0 - ds), np.linalg.norm(d1 - ds)
+    print("refinement: residual %.3e -> %.3e, |d - d*| / |d*| %.3e -> %.3e" %
+          (log1[0, 7], log1[0, 9], e0 / np.linalg.norm(ds), e1 / np.linalg.norm(ds)), flush=True)
+    # same starting residual (same FP32 solve), smaller after the correction: a restarted
+    # Jacobi-CG of 10 steps removes part of what the 20-step solve left
+    assert log1[0, 7] == pytest.approx(log0[0, 7], rel=1e-6)
+    assert log1[0, 9] < 0.9 * log1[0, 7]
+    assert e1 < 0.9 * e0
