@@ -255,6 +255,67 @@ def octree_measure(reps=5):
             "ms_build": round(best, 3)}
 
 
+def scale_measure(num_voxels, iterations, refine):
+    """The scale workload (pyhexopt.workload.box_union_octree): the graded octree hex mesh of
+    the union of 8 seeded rotated boxes (~203k triangles), its boundary vertices projected
+    onto the union's outer triangles each iteration, FP64 plain and FP32+adjoint from the same
+    start for `iterations` loop iterations each (the feature-alignment loop's constants, no
+    feature curves). Reports both paths' ms per iteration, min scaled Jacobian, metro
+    Hausdorff (vertex sampling) and the Hausdorff distance over 1M area-uniform samples of
+    each surface, and the 256^3 voxel occupancy of the union."""
+    from pyhexopt import capi, workload
+    t0 = time.perf_counter()
+    w = workload.box_union_octree(num_voxels, 1)
+    nH = w["H"].shape[0]
+    print("scale workload: %d hexes, %d vertices, %d boundary vertices, %d outer of %d triangles, built in %.1f s" %
+          (nH, w["V"].shape[0], w["idsT"].size, w["SF"].shape[0], w["SF_soup"].shape[0], time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    T = boundary_triangles(w["quads"])
+    out = {"hexes": int(nH), "vertices": int(w["V"].shape[0]), "boundary_vertices": int(w["idsT"].size),
+           "surface_triangles": int(w["SF_soup"].shape[0]), "outer_triangles": int(w["SF"].shape[0]),
+           "num_voxels": num_voxels, "iterations": iterations}
+    for mode in ("fp64", "fp32"):
+        h = capi.HexOpt(w["V"], w["H"])
+        nT = w["idsT"].size
+        h.set_constraints(idsT=w["idsT"], normalT=np.zeros((nT, 3)), disT=np.zeros(nT), lamT=LAMDA0)
+        surf = capi.Surface(w["SV"], w["SF"])
+        lam = LAMDA0
+        h.synchronize()
+        t0 = time.perf_counter()
+        for k in range(iterations):
+            h.rest_from_current()
+            h.reference()
+            h.project_patches([surf], w["patchT"], lam)
+            Eq, Es, _ = iterate(h, mode, refine)
+            lam = min(LAMDA0 * max(Eq / Es * lam, 1.0), LAMDA_BOUND) if Es > 0 else LAMDA_BOUND
+        h.synchronize()
+        secs = time.perf_counter() - t0
+        mn, avg = h.scaled_jacobian()
+        X = h.get_positions()
+        _, hmax, hmean = capi.metro_hausdorff(w["SV"], w["SF"], X, T)
+        sampled = workload.sampled_hausdorff(w["SV"], w["SF"], X, T, n=1_000_000, seed=0)
+        out[mode] = {"ms_per_iteration": round(1e3 * secs / iterations, 3), "min_scaled_jacobian": mn,
+                     "avg_scaled_jacobian": avg, "metro_max": hmax, "metro_mean": hmean,
+                     "sampled_hausdorff_1M": sampled}
+        print("scale %s: %.3f ms/iter, min SJ %.6f, metro max %.4e, sampled max %.4e" %
+              (mode, 1e3 * secs / iterations, mn, hmax, sampled["max"]), file=sys.stderr, flush=True)
+        h.close()
+        surf.close()
+    out["speedup_fp32_vs_fp64"] = round(out["fp64"]["ms_per_iteration"] / out["fp32"]["ms_per_iteration"], 3)
+    out["min_sj_abs_diff"] = abs(out["fp32"]["min_scaled_jacobian"] - out["fp64"]["min_scaled_jacobian"])
+    # the 256^3 inside bitmap of the union (compute_sign(VoxelGrid), voxelization.h:221-272)
+    SV = w["SV"]
+    lo = SV.min(axis=0)
+    sp = float((SV.max(axis=0) - lo).max()) / 256 * (1 + 1e-4)
+    s = capi.Surface(SV, w["SF_soup"])
+    s.voxel_parity(256, 256, 256, lo, sp)       # warm-up: column bins built on first use
+    t0 = time.perf_counter()
+    inside = s.voxel_parity(256, 256, 256, lo, sp)
+    out["voxel_parity_256"] = {"ms": round(1e3 * (time.perf_counter() - t0), 2), "inside": int(inside.sum())}
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,6 +329,9 @@ def main():
     ap.add_argument("--no-glue", action="store_true")
     ap.add_argument("--no-octree", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--no-scale", action="store_true")
+    ap.add_argument("--scale-voxels", type=int, default=512)
+    ap.add_argument("--scale-iterations", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see the design notes on "
@@ -377,6 +441,8 @@ def main():
         out["padding_untangle"] = glue_measure(max(nH // 8, 20000), 5)
     if not args.no_octree:
         out["octree_build"] = octree_measure()
+    if not args.no_scale:
+        out["scale_box_union"] = scale_measure(args.scale_voxels, args.scale_iterations, args.refine)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(max(args.num_voxels // 4, 64))
     print(json.dumps(out))

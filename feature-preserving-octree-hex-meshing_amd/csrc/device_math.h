@@ -319,6 +319,75 @@ __device__ __forceinline__ T sd_energy_grad(const T J[3][3], T P[3][3]) {
   return e;
 }
 
+// Symmetric Dirichlet density from the deviation from isometry D = J J^T - I, given to its own
+// relative precision (formed from an FP64 Jacobian):
+//   e = |J|^2 + |J^-1|^2 = tr(S) + tr(S^-1) = 6 + tr((I + D)^-1 D^2),            S = I + D
+//   dE/dJ = 2 (J - J^-T J^-1 J^-T) = 2 (S - S^-1) J^-T = 2 (I + D)^-1 (2 D + D^2) J^-T.
+// Near a rotation both are small and come out to FP32's relative precision, where the
+// direct forms lose it to cancellation. Returns e - 6; P = dE/dJ.
+template <class T>
+__device__ __forceinline__ T sd_energy_grad_dev(const T J[3][3], const T D[3][3], T P[3][3]) {
+  T A[3][3], dJ;
+  inverse3(J, A, dJ);  // A = J^-1
+  T S[3][3], Si[3][3], dS;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) S[i][j] = D[i][j] + (i == j ? T(1) : T(0));
+  inverse3(S, Si, dS);
+  T DD[3][3], N[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      DD[i][j] = D[i][0] * D[0][j] + D[i][1] * D[1][j] + D[i][2] * D[2][j];
+      N[i][j] = T(2) * D[i][j] + DD[i][j];
+    }
+  T ex = 0, Q[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      ex += Si[i][j] * DD[j][i];
+      Q[i][j] = Si[i][0] * N[0][j] + Si[i][1] * N[1][j] + Si[i][2] * N[2][j];
+    }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) P[i][j] = T(2) * (Q[i][0] * A[j][0] + Q[i][1] * A[j][1] + Q[i][2] * A[j][2]);
+  return ex;
+}
+
+// e - 6 = tr((I + D)^-1 D^2) alone (sd_energy_grad_dev without the gradient)
+template <class T>
+__device__ __forceinline__ T sd_excess_dev(const T D[3][3]) {
+  T S[3][3], Si[3][3], dS;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) S[i][j] = D[i][j] + (i == j ? T(1) : T(0));
+  inverse3(S, Si, dS);
+  T ex = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ex += Si[i][j] * (D[j][0] * D[0][i] + D[j][1] * D[1][i] + D[j][2] * D[2][i]);
+  return ex;
+}
+
+// D = J J^T - I in FP64 and its squared Frobenius norm
+__device__ __forceinline__ double isometry_deviation(const double J[3][3], double D[3][3]) {
+  double f2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      D[i][j] = J[i][0] * J[j][0] + J[i][1] * J[j][1] + J[i][2] * J[j][2] - (i == j ? 1.0 : 0.0);
+      f2 += D[i][j] * D[i][j];
+    }
+  return f2;
+}
+
 // Second derivative of the symmetric Dirichlet density along dJ:
 // d/de [2 (J - A^T A A^T)] = 2 (dJ - (dA^T A A^T + A^T dA A^T + A^T A dA^T)), dA = -A dJ A.
 template <class T>

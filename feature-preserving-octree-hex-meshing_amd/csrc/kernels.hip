@@ -744,6 +744,13 @@ struct ApplyOp {
 // its SVD, the SLIM weights K = M W^T W (stored FP32), the energy, and dE/dJ; scatters the
 // direction rhs -dE/dx / 2 and the Jacobi diagonal. The corner-determinant sign is tested
 // in FP64 (inverted-tet count); the energy is summed in FP64.
+struct AdjLocalOp32;
+// three waves per SIMD: the FP64 corners stay live through the corner-tet loop
+template <>
+struct TileWaves<AdjLocalOp32> {
+  static constexpr int value = 3;
+};
+
 struct AdjLocalOp32 {
   RefOp ro; int nV, nH; const int* conn; const double* X; float* K; double M, prox;
   SoftDev sd; float* rhs; float* invd; int* n_inverted;
@@ -751,55 +758,46 @@ struct AdjLocalOp32 {
     int vid[8];
     double pd[8][3];
     gather8(X, nV, conn, nH, h, vid, pd);
-    // FP32 coordinates relative to corner 0, differenced in FP64: the edges keep FP32's
-    // relative precision (rounding the absolute positions first would leave them only
-    // |x| / h times less precise)
-    float p[8][3];
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) p[k][d] = (float)(pd[k][d] - pd[0][d]);
     double e = 0.0;
     int inv = 0;
-    unsigned iso = 0;
-    // FP64 tests of the corner tets first, so the FP64 corners are dead before the FP32 loop:
-    // the orientation, sign of det J = det(E B) (for a cube reference B = invs * (a
-    // permutation with determinant +1 for every corner) and invs > 0, so the sign is that of
-    // det E), and the reference's m = 1 branch of the weights (slim_m.cpp:351-353,
-    // |s_i - 1| < 1e-8), taken for a tet when |J J^T - I|_F < 2e-8 puts every singular value
-    // inside it -- FP32 cannot resolve that threshold.
+    // Per corner tet, FP64 first: the Jacobian J = E B from the FP64 corners, its orientation
+    // (sign of det J; for a cube reference B = invs * (a permutation with determinant +1 for
+    // every corner) and invs > 0, so the sign is that of det E), and its deviation from
+    // isometry D = J J^T - I. Then FP32 for the rest: the weights (the reference's m = 1
+    // branch, |s_i - 1| < 1e-8 (slim_m.cpp:351-353), is taken for the tet when
+    // |D|_F < 2e-8 puts every singular value inside it, a threshold FP32 cannot resolve),
+    // and the energy and dE/dJ in their deviation form, which keeps FP32's relative
+    // precision where J is near a rotation.
 #pragma unroll 1
     for (int c = 0; c < 8; ++c) {
-      double E64[3][3], B64[3][3], J64[3][3];
+      double E64[3][3], B64[3][3], J64[3][3], D64[3][3];
       tet_edges(pd, c, E64);
       tet_B(ro, nH, h, c, B64);
       matmul3(E64, B64, J64);
       inv += (ro.mode == REF_CUBE ? det3(E64) : det3(J64)) < 0.0;
-      double f2 = 0.0;
+      const double dev2 = isometry_deviation(J64, D64);
+      const bool iso = dev2 < 4e-16;
+      float B[3][3], J[3][3], D[3][3], W[3][3], P[3][3];
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          const double g = J64[i][0] * J64[j][0] + J64[i][1] * J64[j][1] + J64[i][2] * J64[j][2] - (i == j ? 1.0 : 0.0);
-          f2 += g * g;
+        for (int j2 = 0; j2 < 3; ++j2) {
+          B[i][j2] = (float)B64[i][j2];
+          J[i][j2] = (float)J64[i][j2];
+          D[i][j2] = (float)D64[i][j2];
         }
-      iso |= (f2 < 4e-16 ? 1u : 0u) << c;
-    }
-#pragma unroll 1
-    for (int c = 0; c < 8; ++c) {
-      float B[3][3], E[3][3], J[3][3], W[3][3], R[3][3], P[3][3];
-      tet_B(ro, nH, h, c, B);
-      tet_edges(p, c, E);
-      matmul3(E, B, J);
-      if ((iso >> c) & 1u) {
+      if (iso) {
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j) W[i][j] = i == j ? 1.0f : 0.0f;
+          for (int j2 = 0; j2 < 3; ++j2) W[i][j2] = i == j2 ? 1.0f : 0.0f;
       } else {
         slim_weights_sd_stable(J, W);
       }
-      e += M * (double)sd_energy_grad(J, P);
+      // deviation form while I + D is well conditioned (|D|_F < 0.5); the direct form else,
+      // where inverting J J^T would square J's condition number
+      if (dev2 < 0.25) e += M * (6.0 + (double)sd_energy_grad_dev(J, D, P));
+      else e += M * (double)sd_energy_grad(J, P);
       float Kt[3][3];
 #pragma unroll
       for (int i = 0; i < 3; ++i)
@@ -837,7 +835,6 @@ struct AdjLocalOp32 {
           add_row<3, 3>(out, kHexTet[c][k + 1], v);
         }
       }
-      (void)R;
     }
     if (inv) atomicAdd(n_inverted, inv);
     return e;
@@ -1865,16 +1862,45 @@ __global__ __launch_bounds__(256) void energy_kernel(RefOp ro, const double* cur
       for (int k = 0; k < 8; ++k)
         for (int d = 0; d < 3; ++d) p[k][d] = line_point<DIFF>(p[k][d], dst[(size_t)d * nV + vid[k]], step);
     }
-    T pt[8][3];
-    for (int k = 0; k < 8; ++k)
-      for (int d = 0; d < 3; ++d) pt[k][d] = (T)p[k][d];
+    if (sizeof(T) == 4 && energy == E_SYMMETRIC_DIRICHLET) {
+      // FP32 symmetric Dirichlet: J and its deviation from isometry D = J J^T - I in FP64,
+      // e - 6 = tr((I + D)^-1 D^2) in FP32 (to FP32's relative precision of the excess,
+      // which is what the line search compares)
+#pragma unroll 1
+      for (int c = 0; c < 8; ++c) {
+        double B64[3][3], E64[3][3], J64[3][3], D64[3][3];
+        tet_B(ro, nH, h, c, B64);
+        tet_edges(p, c, E64);
+        matmul3(E64, B64, J64);
+        const double dev2 = isometry_deviation(J64, D64);
+        if (dev2 < 0.25) {   // well conditioned I + D (see AdjLocalOp32)
+          T D[3][3];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      T B[3][3], E[3][3], J[3][3];
-      tet_B(ro, nH, h, c, B);
-      tet_edges(pt, c, E);
-      matmul3(E, B, J);
-      e += M * (double)tet_energy(J, energy, (T)exp_f);
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) D[i][j] = (T)D64[i][j];
+          e += M * (6.0 + (double)sd_excess_dev(D));
+        } else {
+          T J[3][3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) J[i][j] = (T)J64[i][j];
+          e += M * (double)tet_energy(J, energy, (T)exp_f);
+        }
+      }
+    } else {
+      T pt[8][3];
+      for (int k = 0; k < 8; ++k)
+        for (int d = 0; d < 3; ++d) pt[k][d] = (T)p[k][d];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        T B[3][3], E[3][3], J[3][3];
+        tet_B(ro, nH, h, c, B);
+        tet_edges(pt, c, E);
+        matmul3(E, B, J);
+        e += M * (double)tet_energy(J, energy, (T)exp_f);
+      }
     }
   }
   double s = block_sum(e, red);
@@ -1948,17 +1974,30 @@ struct GradOp {
     int vid[8];
     double pd[8][3];
     gather8(X, nV, conn, nH, h, vid, pd);
-    T p[8][3];
-    for (int k = 0; k < 8; ++k)
-      for (int d = 0; d < 3; ++d) p[k][d] = (T)pd[k][d];
     double e = 0.0;
 #pragma unroll 1
     for (int c = 0; c < 8; ++c) {
-      T B[3][3], E[3][3], J[3][3], Pm[3][3];
-      tet_B(ro, nH, h, c, B);
-      tet_edges(p, c, E);
-      matmul3(E, B, J);
-      e += M * (double)sd_energy_grad(J, Pm);
+      T B[3][3], J[3][3], Pm[3][3];
+      if constexpr (sizeof(T) == 4) {
+        // FP32: J and D = J J^T - I formed in FP64 from the corners, the energy and dE/dJ in
+        // the deviation form (FP32 keeps its relative precision near a rotation)
+        double B64[3][3], E64[3][3], J64[3][3], D64[3][3];
+        tet_B(ro, nH, h, c, B64);
+        tet_edges(pd, c, E64);
+        matmul3(E64, B64, J64);
+        const double dev2 = isometry_deviation(J64, D64);
+        T D[3][3];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) { B[i][j] = (T)B64[i][j]; J[i][j] = (T)J64[i][j]; D[i][j] = (T)D64[i][j]; }
+        if (dev2 < 0.25) e += M * (6.0 + (double)sd_energy_grad_dev(J, D, Pm));   // see AdjLocalOp32
+        else e += M * (double)sd_energy_grad(J, Pm);
+      } else {
+        T E[3][3];
+        tet_B(ro, nH, h, c, B);
+        tet_edges(pd, c, E);
+        matmul3(E, B, J);
+        e += M * (double)sd_energy_grad(J, Pm);
+      }
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) Pm[i][j] *= (T)M;
       scatter_tet(Pm, B, c, o);

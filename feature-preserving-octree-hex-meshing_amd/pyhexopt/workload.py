@@ -90,3 +90,67 @@ def patch_surfaces(w, device=0):
         remap[used] = np.arange(used.size)
         out.append(capi.Surface(w["SV"][used], remap[F].astype(np.int32), device=device))
     return out
+
+
+def box_union_outer(V, F, boxes, nper):
+    """The triangles of the box-union soup on the union's outer surface: those whose centroid
+    lies in no other box (a box's own faces have their centroids on its boundary)."""
+    cen = V[F].mean(axis=1)
+    owner = np.arange(F.shape[0]) // nper
+    keep = np.ones(F.shape[0], dtype=bool)
+    for b, (Rm, sides, centre) in enumerate(boxes):
+        q = np.abs((cen - centre) @ Rm) <= sides / 2 + 1e-12
+        keep &= ~(q.all(axis=1) & (owner != b))
+    return F[keep]
+
+
+def box_union_octree(num_voxels=512, stop_extent=1, device=0, nboxes=8, n=46, seed=0):
+    """The scale workload: the union of `nboxes` seeded rotated boxes (meshgen.box_union_surface,
+    ~203k triangles as the soup of the closed box surfaces), its graded, paired octree, and the
+    leaves inside the union by the signed +z crossing count of compute_sign (voxelization.cpp:
+    101-163, hexopt_occ_cell_parity), which makes the soup the union. Boundary vertices are
+    REGULAR vertices projected onto the union's outer triangles (box_union_outer); there are
+    no feature curves in this workload."""
+    SV, SF, boxes = mg.box_union_surface(nboxes, n, seed)
+    nper = SF.shape[0] // nboxes
+    vs, origin, grid, mt = mg.octree_frame(SV, num_voxels)
+    surf = capi.Surface(SV, SF, device=device)
+    try:
+        out = surf.octree(grid, origin, vs, mt, stop_extent)
+        leaves, V, H = out["leaves"], out["V"], out["H"]
+        # compute_sign's octree variant on the leaves, in the frame the octree was built in
+        inside = surf.cell_parity(leaves, origin + mt, vs) > 0.5
+    finally:
+        surf.close()
+    leaves, H = leaves[inside], H[inside]
+    used = np.unique(H)
+    remap = np.full(V.shape[0], -1, dtype=np.int64)
+    remap[used] = np.arange(used.size)
+    V, H = V[used], remap[H].astype(np.int32)
+    quads = mg.octree_leaf_boundary(leaves, H)
+    bnd = np.unique(quads).astype(np.int32)
+    SFo = box_union_outer(SV, SF, boxes, nper)
+    return dict(V=V, H=H, leaves=leaves, quads=quads, idsT=bnd, patchT=np.zeros(bnd.size, np.int32),
+                idsL=np.zeros(0, np.int32), curveL=np.zeros(0, np.int32), SV=SV, SF=SFo, SF_soup=SF,
+                fpatch=np.zeros(SFo.shape[0], np.int32), curves=[], voxel_size=vs, grid=grid,
+                octree_leaves=int(out["leaves"].shape[0]), inside=int(inside.sum()),
+                extents=np.unique(leaves[:, 3], return_counts=True))
+
+
+def sampled_hausdorff(V0, F0, V1, F1, n=1_000_000, seed=0, device=0):
+    """Hausdorff distance by area-uniform sampling: n seeded points on each triangle mesh
+    (meshgen.surface_samples), each measured against the other mesh by the device closest
+    point (hexopt_surface_closest). Returns dict(max, mean) per direction ("0to1", "1to0")
+    and the symmetric maximum."""
+    out = {}
+    for tag, (Va, Fa, Vb, Fb, sd) in (("0to1", (V0, F0, V1, F1, seed)), ("1to0", (V1, F1, V0, F0, seed + 1))):
+        P = mg.surface_samples(Va, Fa, n, sd)
+        s = capi.Surface(Vb, Fb, device=device)
+        try:
+            S, _, _, _ = s.closest(P)
+        finally:
+            s.close()
+        d = np.abs(S)
+        out[tag] = dict(max=float(d.max()), mean=float(d.mean()))
+    out["max"] = max(out["0to1"]["max"], out["1to0"]["max"])
+    return out
