@@ -24,7 +24,7 @@ EXPORTS = ["hexopt_create", "hexopt_create_error", "hexopt_destroy", "hexopt_las
            "hexopt_curves_create", "hexopt_curves_error", "hexopt_curves_destroy", "hexopt_project_lines",
            "hexopt_refresh_lines", "hexopt_metro_hausdorff", "hexopt_glue_setup", "hexopt_glue_solve",
            "hexopt_octree_build", "hexopt_octree_error", "hexopt_octree_destroy", "hexopt_octree_sizes",
-           "hexopt_octree_copy", "hexopt_octree_refine", "hexopt_project_patches", "hexopt_debug_operator"]
+           "hexopt_octree_copy", "hexopt_octree_refine", "hexopt_scaled_jacobian_noalloc", "hexopt_noalloc_error", "hexopt_project_patches", "hexopt_debug_operator"]
 
 _LIB = None
 
@@ -110,6 +110,10 @@ def lib():
         L.hexopt_surface_last_error.restype = C.c_char_p
         L.hexopt_surface_last_error.argtypes = [C.c_void_p]
         L.hexopt_surface_destroy.argtypes = [C.c_void_p]
+        L.hexopt_scaled_jacobian_noalloc.restype = C.c_int
+        L.hexopt_scaled_jacobian_noalloc.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                                     C.c_void_p, C.c_void_p]
+        L.hexopt_noalloc_error.restype = C.c_char_p
         L.hexopt_debug_operator.restype = C.c_int
         L.hexopt_project_patches.restype = C.c_int
         L.hexopt_project_patches.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double]
@@ -344,6 +348,18 @@ class HexOpt:
         out = np.zeros(6, dtype=np.int32)
         self._chk(lib().hexopt_layout_info(C.c_void_p(self.h), _ptr(out, C.c_int32)), "layout_info")
         return dict(zip(["tiles", "shared", "slots", "max_colours", "ref_mode", "corner_tiles"], out.tolist()))
+
+
+def scaled_jacobian_noalloc(device, nV, X_ptr, nH, conn_ptr, per_hex_ptr, min_sum_ptr, stream=None):
+    """hexopt_scaled_jacobian_noalloc on caller-owned device arrays (integer device addresses:
+    X as x[d*nV+v], conn as conn[k*nH+h], per_hex nullable, min_sum two doubles). Queues the
+    work on `stream` (an integer hipStream_t handle, None for the default stream)."""
+    rc = lib().hexopt_scaled_jacobian_noalloc(device, nV, C.c_void_p(X_ptr), nH, C.c_void_p(conn_ptr),
+                                              C.c_void_p(per_hex_ptr) if per_hex_ptr else None,
+                                              C.c_void_p(min_sum_ptr), C.c_void_p(stream) if stream else None)
+    if rc != 0:
+        raise RuntimeError("hexopt_scaled_jacobian_noalloc failed (%d): %s" %
+                           (rc, lib().hexopt_noalloc_error().decode()))
 
 
 def surface_normals(V, F):

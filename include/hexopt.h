@@ -326,6 +326,22 @@ int hexopt_project_lines(hexopt_ctx* ctx, hexopt_curves* cv, int which, int nL, 
  * Line vertices must carry no corner or plane constraint (HEXOPT_EUNSUPPORTED otherwise). */
 int hexopt_refresh_lines(hexopt_ctx* ctx, hexopt_curves* cv, const int32_t* curve_L, double lamda_L);
 
+/* No counterpart in the reference: the one entry point that allocates no device memory, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). It evaluates hexopt_scaled_jacobian's scaled Jacobian with the same
+ * kernels on device arrays the caller owns, in the caller's numbering: X the nV positions as
+ * x[d*nV + v] (d = 0..2), conn the nH hexes as conn[k*nH + h] (corner k in the reference's
+ * Hybrid::vs order, every id in [0, nV)); per_hex (nullable, nH doubles) each hex's minimum
+ * corner scaled Jacobian; min_sum two doubles: the mesh minimum and the sum of the per-hex
+ * minima (the mean is min_sum[1] / nH). Every pointer is device memory; the work is queued on
+ * `stream` (a hipStream_t, NULL for the default stream) and nothing waits for it. The block
+ * partials live in a fixed device array of 1 MiB per GPU, held until the process ends, which
+ * caps nH at 16,777,216 (larger meshes: HEXOPT_EINVAL). Calls that overlap on one GPU share
+ * that array and race. Errors: hexopt_noalloc_error(). */
+int hexopt_scaled_jacobian_noalloc(int device, int nV, const double* X, int nH, const int32_t* conn, double* per_hex,
+                                   double* min_sum, void* stream);
+const char* hexopt_noalloc_error(void);
+
 /* Blocks until the context's stream is idle. */
 int hexopt_synchronize(hexopt_ctx* ctx);
 
