@@ -73,11 +73,15 @@ def prepare(h, patches, curves, w, lam):
     h.refresh_lines(curves, w["curveL"], lam)
 
 
+CG_ITERS = [20]   # FP32 direction solve: CG steps before the refinement (set from --cg-iters)
+
+
 def iterate(h, mode, refine, measure=False):
     if mode == "fp64":
         E, Eq, Es, log, _ = h.slim_solve(1, weight_opt=WEIGHT_OPT)
         return Eq, Es, log[0]
-    E, log = h.opt32_solve(1, weight_opt=WEIGHT_OPT, refine_iters=refine, residual_every=1 if measure else 0)
+    E, log = h.opt32_solve(1, cg_iters=CG_ITERS[0], weight_opt=WEIGHT_OPT, refine_iters=refine,
+                           residual_every=1 if measure else 0)
     return log[0, 1], log[0, 2], log[0]
 
 
@@ -130,6 +134,7 @@ def first_iteration_errors(w, refine):
             "step_fp32": float(r32[4]), "step_fp64": float(r64[4]),
             "step_bound_fp32": float(r32[3]), "step_bound_fp64": float(r64[3]),
             "positions_max_over_diag": float(np.abs(X32 - X64).max() / diag),
+            "fp32_cg_error": float(r32[6]),
             "fp64_residual_before_refinement": float(r32[7]),
             "fp64_residual_after_refinement": float(r32[9]),
             "fp64_cg_error": float(r64[6]),
@@ -322,15 +327,17 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--iterations", type=int, default=100, help="loop iterations of the quality comparison")
-    ap.add_argument("--num-voxels", type=int, default=512)
+    ap.add_argument("--num-voxels", type=int, default=448)
     ap.add_argument("--stop-extent", type=int, default=1)
-    ap.add_argument("--refine", type=int, default=4, help="FP32 CG iterations of the FP64-residual refinement")
+    ap.add_argument("--refine", type=int, default=2, help="FP32 CG iterations of the FP64-residual refinement")
+    ap.add_argument("--cg-iters", type=int, default=18,
+                    help="FP32 CG iterations before the refinement (with the default refinement, the reference's 20)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-glue", action="store_true")
     ap.add_argument("--no-octree", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-scale", action="store_true")
-    ap.add_argument("--scale-voxels", type=int, default=512)
+    ap.add_argument("--scale-voxels", type=int, default=448)
     ap.add_argument("--scale-iterations", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -338,6 +345,7 @@ def main():
               "how the mesh would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     from pyhexopt import capi, workload
+    CG_ITERS[0] = args.cg_iters
     t0 = time.perf_counter()
     w = workload.clipped_torus_octree(args.num_voxels, args.stop_extent)
     nV, nH = w["V"].shape[0], w["H"].shape[0]
@@ -354,6 +362,20 @@ def main():
     res = {}
     T = boundary_triangles(w["quads"])
     total = max(args.iterations, args.warmup + args.steps)
+    # the FP64 loop again from the start perturbed by 1e-12 of the bounding-box diagonal: how
+    # far the final min scaled Jacobian and Hausdorff distance move under last-bit changes
+    # alone, the yardstick for the FP32 path's differences
+    h, patches, curves = make_ctx(w)
+    diag0 = float(np.linalg.norm(w["V"].max(axis=0) - w["V"].min(axis=0)))
+    h.set_positions(capi.CURRENT, w["V"] + 1e-12 * diag0 * np.random.default_rng(9).standard_normal(w["V"].shape))
+    _, _, _ = run(h, patches, curves, w, "fp64", total, LAMDA0, args.refine)
+    mn_p, _ = h.scaled_jacobian()
+    _, hmax_p, _ = capi.metro_hausdorff(w["SV"], w["SF"], h.get_positions(), T)
+    h.close()
+    for p in patches:
+        p.close()
+    curves.close()
+    print("fp64 from the perturbed start: min SJ %.6f, metro max %.6e" % (mn_p, hmax_p), file=sys.stderr, flush=True)
     for mode in ("fp64", "fp32"):
         h, patches, curves = make_ctx(w)
         _, _, lam = run(h, patches, curves, w, mode, args.warmup, LAMDA0, args.refine)
@@ -377,6 +399,22 @@ def main():
             p.close()
         curves.close()
 
+    # the FP32 path once more without the FP64-residual refinement (and with the reference's 20
+    # CG steps in one solve): what the refinement changes
+    saved, CG_ITERS[0] = CG_ITERS[0], 20
+    h, patches, curves = make_ctx(w)
+    _, _, lam = run(h, patches, curves, w, "fp32", args.warmup, LAMDA0, 0)
+    secs0, _, lam = run(h, patches, curves, w, "fp32", args.steps, lam, 0)
+    _, _, lam = run(h, patches, curves, w, "fp32", total - args.warmup - args.steps, lam, 0)
+    mn0, _ = h.scaled_jacobian()
+    _, hmax0, _ = capi.metro_hausdorff(w["SV"], w["SF"], h.get_positions(), T)
+    CG_ITERS[0] = saved
+    h.close()
+    for p in patches:
+        p.close()
+    curves.close()
+    print("fp32 without refinement: %.3f ms/iter, min SJ %.6f, metro max %.6e" % (1e3 * secs0 / args.steps, mn0, hmax0),
+          file=sys.stderr, flush=True)
     r32, r64 = res["fp32"], res["fp64"]
     rV = nV / nH
     # algorithmic bytes per hex of the fused FP32 CG operator pass (cube reference operator):
@@ -411,7 +449,8 @@ def main():
                    "leaf_extents": {str(int(e)): int(c) for e, c in zip(ext, cnt)},
                    "iteration": "compute_referenceMesh + patch plane / rim line projection + 1 SLIM iteration "
                                 "(20-step CG, flip-avoiding line search), weight_opt 0.001, lamda 0.05 -> 1e16",
-                   "fp32_refinement_cg_iters": args.refine, "quality_iterations": total},
+                   "fp32_cg_iters": args.cg_iters, "fp32_refinement_cg_iters": args.refine,
+                   "quality_iterations": total},
         "fp64_plain": {"iterations_per_s": round(args.steps / r64["secs"], 3),
                        "ms_per_step": round(1e3 * r64["secs"] / args.steps, 4), "min_scaled_jacobian": r64["minsj"],
                        "avg_scaled_jacobian": r64["avgsj"], "hausdorff_max": r64["hmax"],
@@ -423,6 +462,13 @@ def main():
         "speedup_fp32_vs_fp64": round((args.steps / r32["secs"]) / (args.steps / r64["secs"]), 3),
         "min_sj_abs_diff": abs(r32["minsj"] - r64["minsj"]),
         "hausdorff_diff_over_bbox_diag": abs(r32["hmax"] - r64["hmax"]) / r64["diag"],
+        "fp32_without_refinement": {"ms_per_step": round(1e3 * secs0 / args.steps, 4), "min_scaled_jacobian": mn0,
+                                    "hausdorff_max": hmax0,
+                                    "min_sj_abs_diff_vs_fp64": abs(mn0 - r64["minsj"]),
+                                    "hausdorff_diff_over_bbox_diag": abs(hmax0 - r64["hmax"]) / r64["diag"]},
+        "fp64_sensitivity": {"note": "FP64 loop from the start perturbed by 1e-12 of the bbox diagonal",
+                             "min_sj_abs_diff": abs(mn_p - r64["minsj"]),
+                             "hausdorff_diff_over_bbox_diag": abs(hmax_p - r64["hmax"]) / r64["diag"]},
         "bbox_diagonal": r64["diag"],
         "first_iteration_errors_fp32_vs_fp64": errors,
         "element_evals_per_s": round(nH / (r32["ms_grad"] * 1e-3), 1),

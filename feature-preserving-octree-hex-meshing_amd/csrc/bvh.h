@@ -69,9 +69,6 @@ void build_ep_faces(int nv, const double* V, int nf, const int* F, std::vector<d
 // == upper when no face is closer.
 void launch_metro_ep(const SurfaceDev& s, const double* ep, int nq, const double* P, double upper, double* dist,
                      int* status, hipStream_t st);
-// (max, sum, count) of the distances below `upper`, summed one by one in query order as
-// AddSample accumulates them, into out3 (one thread: the order fixes the rounding)
-void launch_metro_seq(int n, const double* dist, double upper, double* out3, hipStream_t st);
 
 struct SoftDev;
 void launch_refold_planes(const SoftDev& sd, int nT, const double* Cp, double* t_d, const double* S0, const double* b0,

@@ -390,21 +390,6 @@ __global__ __launch_bounds__(kBvhBlock) void metro_ep_kernel(SurfaceDev s, const
   dist[k] = best;
 }
 
-__global__ void metro_seq_kernel(int n, const double* __restrict__ dist, double upper, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double mx = -HUGE_VAL, sm = 0, cn = 0;
-  for (int i = 0; i < n; ++i) {
-    const double d = dist[i];
-    if (d == upper) continue;
-    if (d > mx) mx = d;
-    sm += d;
-    cn += 1;
-  }
-  out[0] = mx;
-  out[1] = sm;
-  out[2] = cn;
-}
-
 void launch_metro_ep(const SurfaceDev& s, const double* ep, int nq, const double* P, double upper, double* dist,
                      int* status, hipStream_t st) {
   if (nq <= 0) return;
@@ -413,9 +398,6 @@ void launch_metro_ep(const SurfaceDev& s, const double* ep, int nq, const double
                      upper, 1e-9 * upper, dist, status);
 }
 
-void launch_metro_seq(int n, const double* dist, double upper, double* out3, hipStream_t st) {
-  hipLaunchKernelGGL(metro_seq_kernel, dim3(1), dim3(64), 0, st, n, dist, upper, out3);
-}
 
 void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int stride, const int* ids, double* S, int* I,
                     double* C, double* N, double* sqrD, int* status, hipStream_t st, const int* order) {

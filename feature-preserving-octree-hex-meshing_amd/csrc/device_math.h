@@ -358,23 +358,6 @@ __device__ __forceinline__ T sd_energy_grad_dev(const T J[3][3], const T D[3][3]
   return ex;
 }
 
-// e - 6 = tr((I + D)^-1 D^2) alone (sd_energy_grad_dev without the gradient)
-template <class T>
-__device__ __forceinline__ T sd_excess_dev(const T D[3][3]) {
-  T S[3][3], Si[3][3], dS;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) S[i][j] = D[i][j] + (i == j ? T(1) : T(0));
-  inverse3(S, Si, dS);
-  T ex = 0;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) ex += Si[i][j] * (D[j][0] * D[0][i] + D[j][1] * D[1][i] + D[j][2] * D[2][i]);
-  return ex;
-}
-
 // D = J J^T - I in FP64 and its squared Frobenius norm
 __device__ __forceinline__ double isometry_deviation(const double J[3][3], double D[3][3]) {
   double f2 = 0.0;

@@ -1588,19 +1588,32 @@ static int metro_direction(hexopt_surface* to, int nvt, const double* Vt, int nf
   HIPCHK(to, to->dep.ensure(ep.size() * 8 + 8));
   HIPCHK(to, to->dP.ensure(P.size() * 8 + 8));
   HIPCHK(to, to->dS.ensure((size_t)nq * 8 + 8));
-  HIPCHK(to, to->dC.ensure(3 * 8));
   HIPCHK(to, hipMemcpyAsync(to->dep.p, ep.data(), ep.size() * 8, hipMemcpyHostToDevice, st));
   HIPCHK(to, hipMemcpyAsync(to->dP.p, P.data(), P.size() * 8, hipMemcpyHostToDevice, st));
   HIPCHK(to, hipMemsetAsync(to->dstat.p, 0, sizeof(int), st));
   launch_metro_ep(to->view, to->dep.as<double>(), nq, to->dP.as<double>(), upper, to->dS.as<double>(),
                   to->dstat.as<int>(), st);
-  launch_metro_seq(nq, to->dS.as<double>(), upper, to->dC.as<double>(), st);
   HIPCHK(to, hipGetLastError());
   int stat = 0;
-  HIPCHK(to, hipMemcpyAsync(out3, to->dC.p, 3 * 8, hipMemcpyDeviceToHost, st));
+  std::vector<double> dist(nq);
+  if (nq) HIPCHK(to, hipMemcpyAsync(dist.data(), to->dS.p, (size_t)nq * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(to, hipMemcpyAsync(&stat, to->dstat.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(to, hipStreamSynchronize(st));
   if (stat) return (to->err = "traversal stack overflow", HEXOPT_EUNSUPPORTED);
+  // vcg's AddSample adds the samples one by one in vertex order (a serial chain of additions
+  // that no parallel sum reproduces bit for bit): done here on the host, where the chain runs
+  // at a few ns per sample; samples at the upper bound found no face and are skipped
+  double mx = -HUGE_VAL, sm = 0, cn = 0;
+  for (int i = 0; i < nq; ++i) {
+    const double d = dist[i];
+    if (d == upper) continue;
+    if (d > mx) mx = d;
+    sm += d;
+    cn += 1;
+  }
+  out3[0] = mx;
+  out3[1] = sm;
+  out3[2] = cn;
   return HEXOPT_OK;
 }
 

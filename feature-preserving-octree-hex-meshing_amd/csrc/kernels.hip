@@ -745,7 +745,8 @@ struct ApplyOp {
 // direction rhs -dE/dx / 2 and the Jacobi diagonal. The corner-determinant sign is tested
 // in FP64 (inverted-tet count); the energy is summed in FP64.
 struct AdjLocalOp32;
-// three waves per SIMD: the FP64 corners stay live through the corner-tet loop
+// three waves per SIMD (a few spilled registers): the FP64 corners stay live through the
+// corner-tet loop
 template <>
 struct TileWaves<AdjLocalOp32> {
   static constexpr int value = 3;
@@ -794,8 +795,8 @@ struct AdjLocalOp32 {
       } else {
         slim_weights_sd_stable(J, W);
       }
-      // deviation form while I + D is well conditioned (|D|_F < 0.5); the direct form else,
-      // where inverting J J^T would square J's condition number
+      // the deviation form while I + D is well conditioned (|D|_F < 0.5); the direct form
+      // else, where inverting J J^T would square J's condition number
       if (dev2 < 0.25) e += M * (6.0 + (double)sd_energy_grad_dev(J, D, P));
       else e += M * (double)sd_energy_grad(J, P);
       float Kt[3][3];
@@ -1862,45 +1863,18 @@ __global__ __launch_bounds__(256) void energy_kernel(RefOp ro, const double* cur
       for (int k = 0; k < 8; ++k)
         for (int d = 0; d < 3; ++d) p[k][d] = line_point<DIFF>(p[k][d], dst[(size_t)d * nV + vid[k]], step);
     }
-    if (sizeof(T) == 4 && energy == E_SYMMETRIC_DIRICHLET) {
-      // FP32 symmetric Dirichlet: J and its deviation from isometry D = J J^T - I in FP64,
-      // e - 6 = tr((I + D)^-1 D^2) in FP32 (to FP32's relative precision of the excess,
-      // which is what the line search compares)
-#pragma unroll 1
-      for (int c = 0; c < 8; ++c) {
-        double B64[3][3], E64[3][3], J64[3][3], D64[3][3];
-        tet_B(ro, nH, h, c, B64);
-        tet_edges(p, c, E64);
-        matmul3(E64, B64, J64);
-        const double dev2 = isometry_deviation(J64, D64);
-        if (dev2 < 0.25) {   // well conditioned I + D (see AdjLocalOp32)
-          T D[3][3];
+    // FP32: corners relative to corner 0 before rounding, so the edges keep FP32's relative
+    // precision (FP64 keeps the reference's own differences of positions)
+    T pt[8][3];
+    for (int k = 0; k < 8; ++k)
+      for (int d = 0; d < 3; ++d) pt[k][d] = sizeof(T) == 4 ? (T)(p[k][d] - p[0][d]) : (T)p[k][d];
 #pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) D[i][j] = (T)D64[i][j];
-          e += M * (6.0 + (double)sd_excess_dev(D));
-        } else {
-          T J[3][3];
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) J[i][j] = (T)J64[i][j];
-          e += M * (double)tet_energy(J, energy, (T)exp_f);
-        }
-      }
-    } else {
-      T pt[8][3];
-      for (int k = 0; k < 8; ++k)
-        for (int d = 0; d < 3; ++d) pt[k][d] = (T)p[k][d];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        T B[3][3], E[3][3], J[3][3];
-        tet_B(ro, nH, h, c, B);
-        tet_edges(pt, c, E);
-        matmul3(E, B, J);
-        e += M * (double)tet_energy(J, energy, (T)exp_f);
-      }
+    for (int c = 0; c < 8; ++c) {
+      T B[3][3], E[3][3], J[3][3];
+      tet_B(ro, nH, h, c, B);
+      tet_edges(pt, c, E);
+      matmul3(E, B, J);
+      e += M * (double)tet_energy(J, energy, (T)exp_f);
     }
   }
   double s = block_sum(e, red);
