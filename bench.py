@@ -14,13 +14,17 @@ REGULAR vertex's own patch tree, LINE vertices projected onto their rim, one SLI
 lamda = min(0.05 max(E_q / E_s lamda, 1), 1e16) with lamda starting at args.feature_weight
 = 0.05 and opt.weight_opt = 0.001.
 
-`value` is the FP32+adjoint optimiser's iterations/s (FP32 element arithmetic, FP64 sign
-test, step bound, energy sums and an FP64-residual iterative refinement of the direction).
-The FP64 plain (reference-faithful) path runs the same loop from the same start. Both then
-continue, untimed, to `--iterations` loop iterations in all, and the final min scaled
-Jacobian and metro Hausdorff distance of each are reported with their differences. The
-per-result errors of the FP32 path against FP64 (energy, gradient, SLIM weights, step,
-positions) are measured on the first iteration from the shared start.
+`value` is the FP32+adjoint optimiser's iterations/s: FP32 element arithmetic and CG, FP64
+for the sign test, the step bound, the line-search energies and the CG dot products, and the
+reference's 20 CG steps, which reproduce the FP64 path's truncated solve (the directions agree
+to ~1e-7). The variant with the FP64-residual iterative refinement (18 CG steps, then 2 on the
+FP64 residual) is timed and reported beside it. The FP64 plain (reference-faithful) path runs
+the same loop from the same start. All three then continue, untimed, to `--iterations` loop
+iterations in all, and the final min scaled Jacobian and metro Hausdorff distance of each are
+reported with their differences, next to how far the FP64 loop itself moves from a start
+perturbed by 1e-12 of the bounding-box diagonal. The per-result errors of the FP32 path
+against FP64 (gradient, SLIM weights, step, direction, positions, energy) are measured on the
+first iteration from the shared start.
 
 Prints one JSON line.
 """
@@ -37,7 +41,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "feature-preserving-octree-hex-meshing_amd"))
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "cg32_apply_pipe_torus1m_hbm_traffic.md")
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "cg32_apply_pipe_torus_octree1m_hbm_traffic.md")
 WEIGHT_OPT = 0.001          # feature_alignment's opt.weight_opt = weight_opt * 0.001
 LAMDA0, LAMDA_BOUND = 0.05, 1e16   # args.feature_weight, LAMDA_FEATURE_PROJECTION_BOUND
 
@@ -73,42 +77,47 @@ def prepare(h, patches, curves, w, lam):
     h.refresh_lines(curves, w["curveL"], lam)
 
 
-CG_ITERS = [20]   # FP32 direction solve: CG steps before the refinement (set from --cg-iters)
-
-
-def iterate(h, mode, refine, measure=False):
+def iterate(h, mode, solve, measure=False):
+    """One SLIM iteration. mode "fp64": the plain FP64 path; "fp32": the FP32+adjoint path with
+    solve = (CG steps, refinement CG steps). measure: the FP32 path also forms the FP64
+    residual of its direction (log columns 7 and 9)."""
     if mode == "fp64":
         E, Eq, Es, log, _ = h.slim_solve(1, weight_opt=WEIGHT_OPT)
         return Eq, Es, log[0]
-    E, log = h.opt32_solve(1, cg_iters=CG_ITERS[0], weight_opt=WEIGHT_OPT, refine_iters=refine,
+    E, log = h.opt32_solve(1, cg_iters=solve[0], weight_opt=WEIGHT_OPT, refine_iters=solve[1],
                            residual_every=1 if measure else 0)
     return log[0, 1], log[0, 2], log[0]
 
 
-def run(h, patches, curves, w, mode, steps, lam, refine):
-    """`steps` loop iterations; returns seconds, the per-iteration logs and the final lamda."""
+def run(h, patches, curves, w, mode, steps, lam, solve, measure_every=0):
+    """`steps` loop iterations; returns seconds, the per-iteration logs and the final lamda.
+    measure_every > 0: the FP64 residual of the FP32 direction on every that-many-th iteration
+    (an extra FP64 gradient and operator pass; left out of timed runs)."""
     logs = []
     h.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
         prepare(h, patches, curves, w, lam)
-        Eq, Es, row = iterate(h, mode, refine, measure=(k % 10 == 0))
+        Eq, Es, row = iterate(h, mode, solve, measure=measure_every > 0 and k % measure_every == 0)
         logs.append(row)
         lam = min(LAMDA0 * max(Eq / Es * lam, 1.0), LAMDA_BOUND) if Es > 0 else LAMDA_BOUND
     h.synchronize()
     return time.perf_counter() - t0, logs, lam
 
 
-def first_iteration_errors(w, refine):
-    """FP32 path against FP64 from the shared start: quality energy of the same positions, dE/dx,
-    SLIM weight blocks, the accepted step, the positions after one iteration."""
+def first_iteration_errors(w, solve, refined):
+    """FP32 path against FP64 from the shared start: dE/dx and the SLIM weight blocks at the
+    same positions, then one iteration of each: the accepted step, the direction, the positions
+    and the quality energy after it, and the FP64 residual of the FP32 direction. The same for
+    the FP32 variant with the FP64-residual refinement (`refined` = (CG steps, refinement CG
+    steps))."""
     from pyhexopt import capi
     res = {}
-    ctx = {m: make_ctx(w) for m in ("fp64", "fp32")}
+    ctx = {m: make_ctx(w) for m in ("fp64", "fp32", "fp32r")}
     try:
         h64 = ctx["fp64"][0]
-        prepare(*ctx["fp64"], w, LAMDA0)
-        prepare(*ctx["fp32"], w, LAMDA0)
+        for m in ctx:
+            prepare(*ctx[m], w, LAMDA0)
         X0 = h64.get_positions()
         # the FP32 adjoint's arithmetic error: both gradients at the same (FP32) positions
         X32 = X0.astype(np.float32)
@@ -116,28 +125,46 @@ def first_iteration_errors(w, refine):
         g32 = h64.energy_grad(X32, precision=32, weight_opt=WEIGHT_OPT).astype(np.float64)
         K64 = h64.debug_operator(64, weight_opt=WEIGHT_OPT)
         K32 = h64.debug_operator(32, weight_opt=WEIGHT_OPT)
-        _, _, r64 = iterate(h64, "fp64", refine)
-        h32 = ctx["fp32"][0]
-        _, _, r32 = iterate(h32, "fp32", refine, measure=True)
-        X64, X32 = h64.get_positions(), h32.get_positions()
-        Eq32_at64, _ = h32.energy(weight_opt=WEIGHT_OPT)    # FP64 energy of the FP32 iterate
+        _, _, r64 = iterate(h64, "fp64", None)
+        h32, h32r = ctx["fp32"][0], ctx["fp32r"][0]
+        _, _, r32 = iterate(h32, "fp32", solve, measure=True)
+        _, _, r32r = iterate(h32r, "fp32", refined, measure=True)
+        X64, X32, X32r = h64.get_positions(), h32.get_positions(), h32r.get_positions()
+        # directions from x' = x + step d
+        d64 = (X64 - X0) / r64[4] if r64[4] > 0 else None
+
+        def ddiff(X, r):
+            if d64 is None or r[4] <= 0:
+                return None
+            return float(np.linalg.norm((X - X0) / r[4] - d64) / np.linalg.norm(d64))
         diag = float(np.linalg.norm(w["SV"].max(axis=0) - w["SV"].min(axis=0)))
         gscale = np.abs(g64).max()
         kscale = np.abs(K64).max()
         res = {
-            # the quality energy of the FP32 iterate summed from FP32 element energies, against FP64
-            "energy_quality_rel": abs(r32[1] - Eq32_at64) / abs(Eq32_at64),
+            # the quality energy after the FP32 path's iteration against the FP64 path's
+            "energy_quality_rel": abs(r32[1] - r64[1]) / abs(r64[1]),
             "grad_rel_max": float(np.abs(g32 - g64).max() / gscale),
             "grad_rel_l2": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
             "weights_rel_max": float(np.abs(K32 - K64).max() / kscale),
             "weights_rel_l2": float(np.linalg.norm(K32 - K64) / np.linalg.norm(K64)),
             "step_fp32": float(r32[4]), "step_fp64": float(r64[4]),
             "step_bound_fp32": float(r32[3]), "step_bound_fp64": float(r64[3]),
+            "direction_rel_diff": ddiff(X32, r32),
             "positions_max_over_diag": float(np.abs(X32 - X64).max() / diag),
+            # |b64 - L d| / |b64| of the direction system L d = -dE/dx / 2 (FP64 sums), and the
+            # FP32 CG's own estimate of it; both CGs stop at the same truncation, so this is
+            # the truncation, not FP32 rounding (the directions agree to direction_rel_diff)
+            "fp64_residual_of_fp32_direction": float(r32[7]),
             "fp32_cg_error": float(r32[6]),
-            "fp64_residual_before_refinement": float(r32[7]),
-            "fp64_residual_after_refinement": float(r32[9]),
-            "fp64_cg_error": float(r64[6]),
+            # the FP64 path's Eigen error |rhs - L x| / |rhs| of the position form
+            "fp64_cg_error_position_form": float(r64[6]),
+            "with_refinement": {
+                "cg_iters": refined[0], "refinement_cg_iters": refined[1],
+                "step": float(r32r[4]), "direction_rel_diff": ddiff(X32r, r32r),
+                "positions_max_over_diag": float(np.abs(X32r - X64).max() / diag),
+                "energy_quality_rel": abs(r32r[1] - r64[1]) / abs(r64[1]),
+                "fp64_residual_before_refinement": float(r32r[7]),
+                "fp64_residual_after_refinement": float(r32r[9])},
         }
     finally:
         for h, patches, curves in ctx.values():
@@ -260,7 +287,7 @@ def octree_measure(reps=5):
             "ms_build": round(best, 3)}
 
 
-def scale_measure(num_voxels, iterations, refine):
+def scale_measure(num_voxels, iterations, solve):
     """The scale workload (pyhexopt.workload.box_union_octree): the graded octree hex mesh of
     the union of 8 seeded rotated boxes (~203k triangles), its boundary vertices projected
     onto the union's outer triangles each iteration, FP64 plain and FP32+adjoint from the same
@@ -291,7 +318,7 @@ def scale_measure(num_voxels, iterations, refine):
             h.rest_from_current()
             h.reference()
             h.project_patches([surf], w["patchT"], lam)
-            Eq, Es, _ = iterate(h, mode, refine)
+            Eq, Es, _ = iterate(h, mode, solve)
             lam = min(LAMDA0 * max(Eq / Es * lam, 1.0), LAMDA_BOUND) if Es > 0 else LAMDA_BOUND
         h.synchronize()
         secs = time.perf_counter() - t0
@@ -329,9 +356,12 @@ def main():
     ap.add_argument("--iterations", type=int, default=100, help="loop iterations of the quality comparison")
     ap.add_argument("--num-voxels", type=int, default=448)
     ap.add_argument("--stop-extent", type=int, default=1)
-    ap.add_argument("--refine", type=int, default=2, help="FP32 CG iterations of the FP64-residual refinement")
-    ap.add_argument("--cg-iters", type=int, default=18,
-                    help="FP32 CG iterations before the refinement (with the default refinement, the reference's 20)")
+    ap.add_argument("--cg-iters", type=int, default=20, help="FP32 CG steps of the direction solve (the reference's 20)")
+    ap.add_argument("--refine", type=int, default=0, help="FP32 CG steps of the FP64-residual refinement")
+    ap.add_argument("--refined-cg-iters", type=int, default=18,
+                    help="the refinement variant reported beside: CG steps before its refinement")
+    ap.add_argument("--refined-refine", type=int, default=2,
+                    help="the refinement variant reported beside: its refinement CG steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-glue", action="store_true")
     ap.add_argument("--no-octree", action="store_true")
@@ -345,7 +375,8 @@ def main():
               "how the mesh would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     from pyhexopt import capi, workload
-    CG_ITERS[0] = args.cg_iters
+    solve = (args.cg_iters, args.refine)
+    refined = (args.refined_cg_iters, args.refined_refine)
     t0 = time.perf_counter()
     w = workload.clipped_torus_octree(args.num_voxels, args.stop_extent)
     nV, nH = w["V"].shape[0], w["H"].shape[0]
@@ -355,7 +386,7 @@ def main():
                                 w["idsT"].size, w["idsL"].size, w["SF"].shape[0], time.perf_counter() - t0),
           file=sys.stderr, flush=True)
 
-    errors = None if args.no_errors else first_iteration_errors(w, args.refine)
+    errors = None if args.no_errors else first_iteration_errors(w, solve, refined)
     if errors:
         print("first-iteration errors: %s" % json.dumps(errors), file=sys.stderr, flush=True)
 
@@ -368,7 +399,7 @@ def main():
     h, patches, curves = make_ctx(w)
     diag0 = float(np.linalg.norm(w["V"].max(axis=0) - w["V"].min(axis=0)))
     h.set_positions(capi.CURRENT, w["V"] + 1e-12 * diag0 * np.random.default_rng(9).standard_normal(w["V"].shape))
-    _, _, _ = run(h, patches, curves, w, "fp64", total, LAMDA0, args.refine)
+    _, _, _ = run(h, patches, curves, w, "fp64", total, LAMDA0, None)
     mn_p, _ = h.scaled_jacobian()
     _, hmax_p, _ = capi.metro_hausdorff(w["SV"], w["SF"], h.get_positions(), T)
     h.close()
@@ -378,9 +409,10 @@ def main():
     print("fp64 from the perturbed start: min SJ %.6f, metro max %.6e" % (mn_p, hmax_p), file=sys.stderr, flush=True)
     for mode in ("fp64", "fp32"):
         h, patches, curves = make_ctx(w)
-        _, _, lam = run(h, patches, curves, w, mode, args.warmup, LAMDA0, args.refine)
-        secs, logs, lam = run(h, patches, curves, w, mode, args.steps, lam, args.refine)
-        _, rest, lam = run(h, patches, curves, w, mode, total - args.warmup - args.steps, lam, args.refine)
+        _, _, lam = run(h, patches, curves, w, mode, args.warmup, LAMDA0, solve)
+        secs, logs, lam = run(h, patches, curves, w, mode, args.steps, lam, solve)
+        _, rest, lam = run(h, patches, curves, w, mode, total - args.warmup - args.steps, lam, solve,
+                           measure_every=10)
         logs = np.array(logs + rest)
         mn, avg = h.scaled_jacobian()
         X = h.get_positions()
@@ -399,22 +431,23 @@ def main():
             p.close()
         curves.close()
 
-    # the FP32 path once more without the FP64-residual refinement (and with the reference's 20
-    # CG steps in one solve): what the refinement changes
-    saved, CG_ITERS[0] = CG_ITERS[0], 20
+    # the FP32 path once more with the FP64-residual refinement (the refined variant's CG steps
+    # before it and its correction steps): what the refinement changes
     h, patches, curves = make_ctx(w)
-    _, _, lam = run(h, patches, curves, w, "fp32", args.warmup, LAMDA0, 0)
-    secs0, _, lam = run(h, patches, curves, w, "fp32", args.steps, lam, 0)
-    _, _, lam = run(h, patches, curves, w, "fp32", total - args.warmup - args.steps, lam, 0)
-    mn0, _ = h.scaled_jacobian()
-    _, hmax0, _ = capi.metro_hausdorff(w["SV"], w["SF"], h.get_positions(), T)
-    CG_ITERS[0] = saved
+    _, _, lam = run(h, patches, curves, w, "fp32", args.warmup, LAMDA0, refined)
+    secs_r, _, lam = run(h, patches, curves, w, "fp32", args.steps, lam, refined)
+    _, logs_r, lam = run(h, patches, curves, w, "fp32", total - args.warmup - args.steps, lam, refined,
+                         measure_every=10)
+    mn_r, _ = h.scaled_jacobian()
+    _, hmax_r, _ = capi.metro_hausdorff(w["SV"], w["SF"], h.get_positions(), T)
     h.close()
     for p in patches:
         p.close()
     curves.close()
-    print("fp32 without refinement: %.3f ms/iter, min SJ %.6f, metro max %.6e" % (1e3 * secs0 / args.steps, mn0, hmax0),
-          file=sys.stderr, flush=True)
+    print("fp32 with refinement (%d + %d CG steps): %.3f ms/iter, min SJ %.6f, metro max %.6e" %
+          (refined[0], refined[1], 1e3 * secs_r / args.steps, mn_r, hmax_r), file=sys.stderr, flush=True)
+    res32 = [l[7] for l in res["fp32"]["logs"] if l[7] >= 0]
+    res32r = [(l[7], l[9]) for l in logs_r if l[9] >= 0]
     r32, r64 = res["fp32"], res["fp64"]
     rV = nV / nH
     # algorithmic bytes per hex of the fused FP32 CG operator pass (cube reference operator):
@@ -439,7 +472,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (FP64 sign test, step bound, energy sums, CG dot products, residual refinement)",
+        "dtype": "f32 (FP64 sign test, step bound, line-search energies, CG dot products)",
         "data": "synthetic",
         "config": {"workload": "clipped torus R=1 r=0.35 z=+-0.2 (40k triangles, 4 patches, 4 rim feature curves), "
                                "graded paired octree hex mesh (leaves inside the surface), feature-alignment loop",
@@ -462,10 +495,14 @@ def main():
         "speedup_fp32_vs_fp64": round((args.steps / r32["secs"]) / (args.steps / r64["secs"]), 3),
         "min_sj_abs_diff": abs(r32["minsj"] - r64["minsj"]),
         "hausdorff_diff_over_bbox_diag": abs(r32["hmax"] - r64["hmax"]) / r64["diag"],
-        "fp32_without_refinement": {"ms_per_step": round(1e3 * secs0 / args.steps, 4), "min_scaled_jacobian": mn0,
-                                    "hausdorff_max": hmax0,
-                                    "min_sj_abs_diff_vs_fp64": abs(mn0 - r64["minsj"]),
-                                    "hausdorff_diff_over_bbox_diag": abs(hmax0 - r64["hmax"]) / r64["diag"]},
+        "fp32_fp64_residual_max": max(res32) if res32 else None,
+        "fp32_with_refinement": {"cg_iters": refined[0], "refinement_cg_iters": refined[1],
+                                 "ms_per_step": round(1e3 * secs_r / args.steps, 4), "min_scaled_jacobian": mn_r,
+                                 "hausdorff_max": hmax_r,
+                                 "min_sj_abs_diff_vs_fp64": abs(mn_r - r64["minsj"]),
+                                 "hausdorff_diff_over_bbox_diag": abs(hmax_r - r64["hmax"]) / r64["diag"],
+                                 "fp64_residual_before_after_max": [max(a for a, _ in res32r), max(b for _, b in res32r)]
+                                 if res32r else None},
         "fp64_sensitivity": {"note": "FP64 loop from the start perturbed by 1e-12 of the bbox diagonal",
                              "min_sj_abs_diff": abs(mn_p - r64["minsj"]),
                              "hausdorff_diff_over_bbox_diag": abs(hmax_p - r64["hmax"]) / r64["diag"]},
@@ -481,14 +518,14 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "cg32_apply_pipe_kernel (persistent FP32 CG operator pass)",
                      "bytes_per_hex": round(bytes_per_hex32, 2), "algorithmic_bytes": round(bytes_per_hex32 * nH),
-                     "traffic_source": "profiles/cg32_apply_pipe_torus1m_hbm_traffic.md" if traffic else None},
+                     "traffic_source": "profiles/cg32_apply_pipe_torus_octree1m_hbm_traffic.md" if traffic else None},
     }
     if not args.no_glue:
         out["padding_untangle"] = glue_measure(max(nH // 8, 20000), 5)
     if not args.no_octree:
         out["octree_build"] = octree_measure()
     if not args.no_scale:
-        out["scale_box_union"] = scale_measure(args.scale_voxels, args.scale_iterations, args.refine)
+        out["scale_box_union"] = scale_measure(args.scale_voxels, args.scale_iterations, solve)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(max(args.num_voxels // 4, 64))
     print(json.dumps(out))

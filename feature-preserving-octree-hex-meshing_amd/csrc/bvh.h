@@ -58,6 +58,17 @@ __host__ __device__ inline int stack_frames(int depth) {
 void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int stride, const int* ids, double* S, int* I,
                     double* C, double* N, double* sqrD, int* status, hipStream_t st, const int* order = nullptr);
 
+// Several surfaces in one launch: segment j's queries are order[q0 .. q0 + n) against surface
+// s, taken by blocks block0 .. block0 + ceil(n / kBvhBlock) (segments in increasing block0,
+// the first at block 0). Writes C and N (indexed by the query, as launch_closest does).
+struct PatchSeg {
+  SurfaceDev s;
+  int block0, q0, n, pad;
+};
+void launch_closest_segs(const PatchSeg* segs, int nseg, int nblocks, int max_depth, const double* P, int soa,
+                         int stride, const int* ids, const int* order, double* C, double* N, int* status,
+                         hipStream_t st);
+
 // Per-face edge-plane data of vcg's tri::UpdateEdges<>::Set (vcg/complex/algorithms/update/
 // component_ep.h:48-73), kEPRec doubles per face: the three edges scaled by 1 / the normal's
 // dominant component (9), the twice-normalised normal (3), the plane offset (1), the dominant

@@ -88,15 +88,32 @@ __device__ __forceinline__ unsigned fr_state(unsigned f) { return (f >> 26) & 7u
 __device__ __forceinline__ unsigned fr_seen(unsigned f) { return (f >> 29) & 3u; }
 __device__ __forceinline__ unsigned fr_make(int n, unsigned st, unsigned seen) { return (unsigned)n | (st << 26) | (seen << 29); }
 
-__global__ __launch_bounds__(kBvhBlock) void closest_kernel(SurfaceDev s, int nq, const double* P, int soa, int stride,
+__global__ __launch_bounds__(kBvhBlock) void closest_kernel(SurfaceDev s0, const PatchSeg* __restrict__ segs, int nseg,
+                                                            int nq, const double* P, int soa, int stride,
                                                             const int* ids, const int* order, double* S, int* I,
                                                             double* Cout, double* Nout, double* sqrDout, int* status) {
   // one frame per tree level: (depth + 1) x kBvhBlock entries of dynamic LDS (see
   // launch_closest), so shallow trees leave room for more resident blocks
   extern __shared__ unsigned stack[];
+  // with segments, each block belongs to one (the last segment starting at or before it) and
+  // walks that segment's tree for that segment's queries; the choice is the same in every lane
+  SurfaceDev s = s0;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nseg > 0) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (segs[mid].block0 <= (int)blockIdx.x) lo = mid;
+      else hi = mid - 1;
+    }
+    s = segs[lo].s;
+    const int local = ((int)blockIdx.x - segs[lo].block0) * blockDim.x + threadIdx.x;
+    if (local >= segs[lo].n) return;
+    k = segs[lo].q0 + local;
+  } else if (k >= nq) {
+    return;
+  }
   const int cap = stack_frames(s.depth);
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nq) return;
   const int q = order ? order[k] : k;
   const int pi = ids ? ids[q] : q;
   double p[3];
@@ -403,8 +420,18 @@ void launch_closest(const SurfaceDev& s, int nq, const double* P, int soa, int s
                     double* C, double* N, double* sqrD, int* status, hipStream_t st, const int* order) {
   if (nq <= 0) return;
   const size_t lds = (size_t)stack_frames(s.depth) * kBvhBlock * sizeof(unsigned);
-  hipLaunchKernelGGL(closest_kernel, dim3((nq + kBvhBlock - 1) / kBvhBlock), dim3(kBvhBlock), lds, st, s, nq, P,
-                     soa, stride, ids, order, S, I, C, N, sqrD, status);
+  hipLaunchKernelGGL(closest_kernel, dim3((nq + kBvhBlock - 1) / kBvhBlock), dim3(kBvhBlock), lds, st, s,
+                     (const PatchSeg*)nullptr, 0, nq, P, soa, stride, ids, order, S, I, C, N, sqrD, status);
+}
+
+void launch_closest_segs(const PatchSeg* segs, int nseg, int nblocks, int max_depth, const double* P, int soa,
+                         int stride, const int* ids, const int* order, double* C, double* N, int* status,
+                         hipStream_t st) {
+  if (nseg <= 0 || nblocks <= 0) return;
+  const size_t lds = (size_t)stack_frames(max_depth) * kBvhBlock * sizeof(unsigned);
+  SurfaceDev none{};
+  hipLaunchKernelGGL(closest_kernel, dim3(nblocks), dim3(kBvhBlock), lds, st, none, segs, nseg, 0, P, soa, stride,
+                     ids, order, (double*)nullptr, (int*)nullptr, C, N, (double*)nullptr, status);
 }
 
 }  // namespace hexopt

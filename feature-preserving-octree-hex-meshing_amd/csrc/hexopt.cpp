@@ -91,7 +91,8 @@ struct hexopt_ctx {
   // vertex order) and the groups' offsets
   std::vector<int32_t> h_patch;
   std::vector<int> patch_off;
-  DevBuf d_porder, d_pC, d_pstat;
+  std::vector<PatchSeg> h_segs;  // kept until the next call: the source of an async upload
+  DevBuf d_porder, d_pC, d_pstat, d_segs;
   bool lines_refreshable = false;
   // FP32 optimiser state
   DevBuf f_K, f_rhs, f_invd, f_d, f_r, f_z, f_p, f_p2, f_tmp, f_e, f_halo, d_Ld64, d_halo64b, d_ninv, d_b64;
@@ -1242,15 +1243,33 @@ int hexopt_project_patches(hexopt_ctx* c, int npatch, hexopt_surface* const* pat
   HIPCHK(c, c->d_pC.ensure((size_t)nT * 24));
   HIPCHK(c, c->d_pstat.ensure(sizeof(int)));
   HIPCHK(c, hipMemsetAsync(c->d_pstat.p, 0, sizeof(int), st));
-  // each patch's rows against that patch's tree (dirty_graph_projection's loop over Qfg.Rs):
-  // normal_T = pseudonormal N, dis_T = N . closest point
+  // each patch's rows against that patch's tree (dirty_graph_projection's loop over Qfg.Rs),
+  // all patches in one launch: normal_T = pseudonormal N, dis_T = N . closest point
+  std::vector<PatchSeg> segs;
+  int nblocks = 0, max_depth = 1;
   for (int p = 0; p < npatch; ++p) {
     const int n = c->patch_off[p + 1] - c->patch_off[p];
-    if (n > 0)
-      launch_closest(patches[p]->view, n, c->d_cur.as<double>(), 1, c->L.nV, c->d_tids.as<int>(), nullptr, nullptr,
-                     c->d_pC.as<double>(), const_cast<double*>(c->sd.t_n), nullptr, c->d_pstat.as<int>(), st,
-                     c->d_porder.as<int>() + c->patch_off[p]);
+    if (n == 0) continue;
+    PatchSeg g{};
+    g.s = patches[p]->view;
+    g.block0 = nblocks;
+    g.q0 = c->patch_off[p];
+    g.n = n;
+    segs.push_back(g);
+    nblocks += (n + kBvhBlock - 1) / kBvhBlock;
+    max_depth = std::max(max_depth, patches[p]->view.depth);
   }
+  if (segs.size() != c->h_segs.size() ||
+      std::memcmp(segs.data(), c->h_segs.data(), segs.size() * sizeof(PatchSeg)) != 0) {
+    HIPCHK(c, hipStreamSynchronize(st));  // the previous upload from h_segs has finished
+    c->h_segs = segs;
+    HIPCHK(c, c->d_segs.ensure(std::max<size_t>(1, segs.size()) * sizeof(PatchSeg)));
+    HIPCHK(c, hipMemcpyAsync(c->d_segs.p, c->h_segs.data(), segs.size() * sizeof(PatchSeg), hipMemcpyHostToDevice,
+                             st));
+  }
+  launch_closest_segs(c->d_segs.as<PatchSeg>(), (int)segs.size(), nblocks, max_depth, c->d_cur.as<double>(), 1,
+                      c->L.nV, c->d_tids.as<int>(), c->d_porder.as<int>(), c->d_pC.as<double>(),
+                      const_cast<double*>(c->sd.t_n), c->d_pstat.as<int>(), st);
   launch_refold_planes(c->sd, nT, c->d_pC.as<double>(), const_cast<double*>(c->sd.t_d), c->d_S0.as<double>(),
                        c->d_b0.as<double>(), const_cast<double*>(c->sd.S), const_cast<double*>(c->sd.b), lamda_T, st);
   c->sd.lamT = lamda_T;

@@ -1863,11 +1863,9 @@ __global__ __launch_bounds__(256) void energy_kernel(RefOp ro, const double* cur
       for (int k = 0; k < 8; ++k)
         for (int d = 0; d < 3; ++d) p[k][d] = line_point<DIFF>(p[k][d], dst[(size_t)d * nV + vid[k]], step);
     }
-    // FP32: corners relative to corner 0 before rounding, so the edges keep FP32's relative
-    // precision (FP64 keeps the reference's own differences of positions)
     T pt[8][3];
     for (int k = 0; k < 8; ++k)
-      for (int d = 0; d < 3; ++d) pt[k][d] = sizeof(T) == 4 ? (T)(p[k][d] - p[0][d]) : (T)p[k][d];
+      for (int d = 0; d < 3; ++d) pt[k][d] = (T)p[k][d];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       T B[3][3], E[3][3], J[3][3];
@@ -2491,12 +2489,17 @@ void launch_energy(const RefOp& ro, const double* cur, const double* dst, double
                      ns_blocks, out2);
 }
 
+// The line search of the FP32 path compares energies whose difference is ~1e-8 of the total
+// near isometry (e = 6 + O(|strain|^2) per tet), below what per-tet FP32 values of e resolve:
+// summed from FP32, the total drifts by ~1e-8 relative as soon as the corners leave the
+// lattice, which rejects good steps. The energy is therefore evaluated in FP64 along the FP32
+// direction; it is a streaming pass, so FP64 costs little more than FP32 here.
 void launch_energy_dir32(const RefOp& ro, const double* cur, const float* d, double step, int nV, const int* conn,
                          int nH, double M, const SoftDev& sd, const double* aL, double* part_q, double* part_s,
                          int ns_blocks, double* out2, hipStream_t st) {
   const int nb = (nH + 255) / 256;
   if (nb > 0)
-    hipLaunchKernelGGL((energy_kernel<float, false, float>), dim3(nb), dim3(256), 0, st, ro, cur, d, step, nV, conn, nH,
+    hipLaunchKernelGGL((energy_kernel<double, false, float>), dim3(nb), dim3(256), 0, st, ro, cur, d, step, nV, conn, nH,
                        (int)E_SYMMETRIC_DIRICHLET, M, 1.0, part_q);
   hipLaunchKernelGGL((soft_energy_kernel<false, float>), dim3(ns_blocks), dim3(256), 0, st, sd, cur, d, step, nV, aL,
                      part_s);
