@@ -137,8 +137,10 @@ int hexopt_glue_solve(hexopt_ctx* ctx, const hexopt_params* prm, int max_iters, 
  * SLIM global step taken from CURRENT) by FP32 CG with FP64 dot products. Iterative
  * refinement (refine_iters > 0): the right-hand side b64 = -dE/dx / 2 is recomputed by the
  * FP64 adjoint, the residual b64 - L d is formed with FP64 sums over the stored operator, and
- * an FP32 CG solve of L e = that residual gives d += e. The step bound (FP64 cubic), the
- * corner-determinant sign test and the energy sums are FP64; the line search is the
+ * an FP32 CG solve of L e = that residual gives d += e (it moves d toward the converged
+ * solve; with the reference's truncated 20-step CG and no refinement, d reproduces the FP64
+ * path's truncated iterate). The step bound (FP64 cubic), the corner-determinant sign test
+ * and the line-search energies (per tet and summed) are FP64; the line search is the
  * reference's bisection. Symmetric Dirichlet only. */
 typedef struct {
   int cg_iters;        /* FP32 CG iterations of the direction solve (the reference uses 20) */
