@@ -31,18 +31,20 @@ __device__ __forceinline__ double box_ext_dist(const double* b, const double p[3
 }
 
 __device__ __forceinline__ double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+// igl's Dot of 1x3 rows as Eigen 3.2.9 unrolls it: the first product plus the sum of the other two
+__device__ __forceinline__ double dotE(const double a[3], const double b[3]) { return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]); }
 
 // Ericson's closest point on a triangle (point_simplex_squared_distance.cpp:44-109)
 __device__ __forceinline__ void closest_on_tri(const double p[3], const double a[3], const double b[3], const double c[3], double r[3]) {
   double ab[3], ac[3], ap[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) { ab[k] = b[k] - a[k]; ac[k] = c[k] - a[k]; ap[k] = p[k] - a[k]; }
-  const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  const double d1 = dotE(ab, ap), d2 = dotE(ac, ap);
   if (d1 <= 0.0 && d2 <= 0.0) { r[0] = a[0]; r[1] = a[1]; r[2] = a[2]; return; }
   double bp[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) bp[k] = p[k] - b[k];
-  const double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  const double d3 = dotE(ab, bp), d4 = dotE(ac, bp);
   if (d3 >= 0.0 && d4 <= d3) { r[0] = b[0]; r[1] = b[1]; r[2] = b[2]; return; }
   const double vc = d1 * d4 - d3 * d2;
   if (!(a[0] == b[0] && a[1] == b[1] && a[2] == b[2])) {
@@ -56,7 +58,7 @@ __device__ __forceinline__ void closest_on_tri(const double p[3], const double a
   double cp[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) cp[k] = p[k] - c[k];
-  const double d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  const double d5 = dotE(ab, cp), d6 = dotE(ac, cp);
   if (d6 >= 0.0 && d5 <= d6) { r[0] = c[0]; r[1] = c[1]; r[2] = c[2]; return; }
   const double vb = d5 * d2 - d1 * d6;
   if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
@@ -144,7 +146,8 @@ __global__ __launch_bounds__(kBvhBlock) void closest_kernel(SurfaceDev s0, const
       const double a[3] = {x[0], x[1], x[2]}, b[3] = {x[3], x[4], x[5]}, c[3] = {x[6], x[7], x[8]};
       double r[3];
       closest_on_tri(p, a, b, c, r);
-      const double d = (p[0] - r[0]) * (p[0] - r[0]) + (p[1] - r[1]) * (p[1] - r[1]) + (p[2] - r[2]) * (p[2] - r[2]);
+      // (p - c).squaredNorm() in Eigen's order
+      const double d = (p[0] - r[0]) * (p[0] - r[0]) + ((p[1] - r[1]) * (p[1] - r[1]) + (p[2] - r[2]) * (p[2] - r[2]));
       if (d < best) { best = d; bf = pr; bc[0] = r[0]; bc[1] = r[1]; bc[2] = r[2]; }
       --top;
       continue;

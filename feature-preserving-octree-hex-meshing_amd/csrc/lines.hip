@@ -3,7 +3,9 @@
 // onto the segments of its curve (point_line_projection, global_functions.cpp:3454-3465) and
 // keeps the nearest, the lowest segment index on equal distances, which is the first entry
 // after the reference's sort of (distance, index) pairs. Built without fused multiply-add;
-// sums of three products add x, y, z in that order.
+// sums of three products add x + (y + z), the order of Eigen 3.2.9's unrolled reduction of a
+// 3-vector (the first coefficient, then the sum of the other two), which the reference's
+// dot() and squaredNorm() compile to.
 #include <hip/hip_runtime.h>
 #include "lines.h"
 #include "kernels.h"
@@ -11,7 +13,7 @@
 namespace hexopt {
 namespace {
 
-__device__ __forceinline__ double norm3(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
+__device__ __forceinline__ double norm3(double x, double y, double z) { return sqrt(x * x + (y * y + z * z)); }
 
 // (a - b).normalized(): Eigen 3.2.9 divides by the norm without a zero test
 __device__ __forceinline__ void normalized_diff(const double* a, const double* b, double o[3]) {
@@ -26,9 +28,9 @@ __device__ __forceinline__ void normalized_diff(const double* a, const double* b
 __device__ __forceinline__ double segment_point(const double v[3], const double* p0, const double* p1, double q3[3]) {
   const double a0 = v[0] - p0[0], a1 = v[1] - p0[1], a2 = v[2] - p0[2];
   const double e0 = p1[0] - p0[0], e1 = p1[1] - p0[1], e2 = p1[2] - p0[2];
-  const double l2 = e0 * e0 + e1 * e1 + e2 * e2;
+  const double l2 = e0 * e0 + (e1 * e1 + e2 * e2);
   double t = 0;
-  if (l2 >= 1.e-7) t = (a0 * e0 + a1 * e1 + a2 * e2) / l2;
+  if (l2 >= 1.e-7) t = (a0 * e0 + (a1 * e1 + a2 * e2)) / l2;
   if (t >= 0.0 && t <= 1.0) {
     for (int d = 0; d < 3; ++d) q3[d] = p0[d] + t * (p1[d] - p0[d]);
   } else if (t < 0.0) {

@@ -50,6 +50,9 @@ inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 inline V3 mul(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// a dot product of fixed-size Eigen 3-vectors as Eigen 3.2.9 unrolls it: the first product plus
+// the sum of the other two
+inline double dotE(V3 a, V3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
 inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
 inline double det3(V3 c0, V3 c1, V3 c2) { return dot(c0, cross(c1, c2)); }
@@ -1070,7 +1073,8 @@ struct Tree {
     }
     return s;
   }
-  // Ericson closest point on triangle (point_simplex_squared_distance.cpp:44-109)
+  // Ericson closest point on triangle (point_simplex_squared_distance.cpp:44-109); its Dot
+  // is Eigen's dot of 1x3 rows
   void closest_tri(const double* pp, int f, double c[3], double bary[3]) const {
     V3 p = {pp[0], pp[1], pp[2]};
     const double* A_ = &V[(size_t)F[(size_t)f * 3] * 3];
@@ -1078,12 +1082,12 @@ struct Tree {
     const double* C_ = &V[(size_t)F[(size_t)f * 3 + 2] * 3];
     V3 a = {A_[0], A_[1], A_[2]}, b = {B_[0], B_[1], B_[2]}, cc = {C_[0], C_[1], C_[2]};
     V3 ab = sub(b, a), ac = sub(cc, a), ap = sub(p, a);
-    double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    double d1 = dotE(ab, ap), d2 = dotE(ac, ap);
     V3 r;
     if (d1 <= 0.0 && d2 <= 0.0) { bary[0] = 1; bary[1] = 0; bary[2] = 0; r = a; goto done; }
     {
       V3 bp = sub(p, b);
-      double d3 = dot(ab, bp), d4 = dot(ac, bp);
+      double d3 = dotE(ab, bp), d4 = dotE(ac, bp);
       if (d3 >= 0.0 && d4 <= d3) { bary[0] = 0; bary[1] = 1; bary[2] = 0; r = b; goto done; }
       double vc = d1 * d4 - d3 * d2;
       if (!(a.x == b.x && a.y == b.y && a.z == b.z)) {
@@ -1093,7 +1097,7 @@ struct Tree {
         }
       }
       V3 cp = sub(p, cc);
-      double d5 = dot(ab, cp), d6 = dot(ac, cp);
+      double d5 = dotE(ab, cp), d6 = dotE(ac, cp);
       if (d6 >= 0.0 && d5 <= d6) { bary[0] = 0; bary[1] = 0; bary[2] = 1; r = cc; goto done; }
       double vb = d5 * d2 - d1 * d6;
       if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
@@ -1118,7 +1122,8 @@ struct Tree {
     if (nd.prim >= 0) {
       double cc[3], bb[3];
       closest_tri(p, nd.prim, cc, bb);
-      double d = (p[0] - cc[0]) * (p[0] - cc[0]) + (p[1] - cc[1]) * (p[1] - cc[1]) + (p[2] - cc[2]) * (p[2] - cc[2]);
+      // (p - c).squaredNorm() in Eigen's order
+      double d = (p[0] - cc[0]) * (p[0] - cc[0]) + ((p[1] - cc[1]) * (p[1] - cc[1]) + (p[2] - cc[2]) * (p[2] - cc[2]));
       if (d < sqr_d) { i = nd.prim; c[0] = cc[0]; c[1] = cc[1]; c[2] = cc[2]; sqr_d = d; }
       return sqr_d;
     }
@@ -1403,15 +1408,17 @@ extern "C" void orc_cell_parity(int nv, const double* V, int nf, const int* F, i
 // (grid_hex_meshing.cpp:3961-4026) with point_line_projection (global_functions.cpp:
 // 3454-3465, Precision = 1e-7 from global_types.h:23). Curves are CSR lists of surface
 // vertex ids (off[c]..off[c+1]); a circle also has the closing segment. Vector norms and
-// dot products sum x, y, z in that order.
+// dot products sum x + (y + z): Eigen 3.2.9 unrolls the reduction of a 3-vector as the first
+// coefficient plus the sum of the other two, which is what the reference's dot() and
+// squaredNorm() compile to (checked bit for bit against the compiled point_line_projection).
 namespace {
-double n3(const double* a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+double n3(const double* a) { return std::sqrt(a[0] * a[0] + (a[1] * a[1] + a[2] * a[2])); }
 void point_line_projection(const double* v1, const double* v2, const double* v, double* pv) {
   double vv1[3], v21[3];
   for (int d = 0; d < 3; ++d) { vv1[d] = v[d] - v1[d]; v21[d] = v2[d] - v1[d]; }
-  const double nv21_2 = v21[0] * v21[0] + v21[1] * v21[1] + v21[2] * v21[2];
+  const double nv21_2 = v21[0] * v21[0] + (v21[1] * v21[1] + v21[2] * v21[2]);
   double t = 0;
-  if (nv21_2 >= 1.e-7) t = (vv1[0] * v21[0] + vv1[1] * v21[1] + vv1[2] * v21[2]) / nv21_2;
+  if (nv21_2 >= 1.e-7) t = (vv1[0] * v21[0] + (vv1[1] * v21[1] + vv1[2] * v21[2])) / nv21_2;
   if (t >= 0.0 && t <= 1.0) for (int d = 0; d < 3; ++d) pv[d] = v1[d] + t * (v2[d] - v1[d]);
   else if (t < 0.0) for (int d = 0; d < 3; ++d) pv[d] = v1[d];
   else if (t > 1.0) for (int d = 0; d < 3; ++d) pv[d] = v2[d];
@@ -1422,6 +1429,11 @@ void normalized_diff(const double* a, const double* b, double* out) {  // (a - b
   for (int k = 0; k < 3; ++k) out[k] = d[k] / n;
 }
 }  // namespace
+
+// point_line_projection alone on n (segment, point) triples: the foot of each.
+extern "C" void orc_point_line_projection(int n, const double* A, const double* B, const double* P, double* PV) {
+  for (int i = 0; i < n; ++i) point_line_projection(A + 3 * i, B + 3 * i, P + 3 * i, PV + 3 * i);
+}
 
 extern "C" void orc_line_projection(int nv, const double* V, int ncurves, const int* off, const int* cvs,
                                     const uint8_t* circles, int n, const double* P, const int* curve_of,

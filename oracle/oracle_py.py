@@ -215,6 +215,15 @@ def sd_grad(H, G, X, M=1.0, lam_C=0.0, lam_T=0.0, lam_L=0.0, idsC=None, Cpos=Non
     return E.value, out
 
 
+def point_line_projection(A, B, P):
+    """Feet of the points P on the segments A-B (all n x 3), as point_line_projection."""
+    A, B, P = _d(A), _d(B), _d(P)
+    out = np.zeros_like(P)
+    lib().orc_point_line_projection(P.shape[0], _p(A, C.c_double), _p(B, C.c_double), _p(P, C.c_double),
+                                    _p(out, C.c_double))
+    return out
+
+
 def line_projection(V, curves, circles, P, curve_of):
     """origin_L, Axa_L of LINE vertices P (n x 3) on their curves (lists of vertex ids)."""
     V, P, curve_of = _d(V), _d(P), _i(curve_of)

@@ -21,7 +21,10 @@
 //      (optimization.cpp:552-628) and slim_solve2's gluing branch (slim_m.cpp:1191-1268),
 //      its loop written out here around the reference's own functions so that every
 //      iteration is logged, then checked against one slim_solve2(data, -1, 1) call.
+//   6  point_line_projection (global_functions.cpp:3454-3465), the segment projection of the
+//      feature-line branch, on given (segment, point) triples.
 #include "slim_m.h"
+#include "global_functions.h"
 #include "igl/flip_avoiding_line_search.h"
 #include "igl/slim.h"
 #include "igl/AABB.h"
@@ -382,6 +385,25 @@ int main(int argc, char** argv) {
 
   if (mode == 4) return glue_mode(in, out);
   if (mode == 5) return smooth_mode(in, out);
+  if (mode == 6) {
+    // point_line_projection (global_functions.cpp:3454-3465) on n (segment, point) triples:
+    // the foot pv and the parameter t
+    const int n = in.get<int32_t>();
+    auto A = in.arr<double>((size_t)n * 3), B = in.arr<double>((size_t)n * 3), P = in.arr<double>((size_t)n * 3);
+    std::vector<double> pv((size_t)n * 3), tv(n);
+    for (int i = 0; i < n; ++i) {
+      Eigen::Vector3d v1(A[i * 3], A[i * 3 + 1], A[i * 3 + 2]), v2(B[i * 3], B[i * 3 + 1], B[i * 3 + 2]);
+      Eigen::Vector3d v(P[i * 3], P[i * 3 + 1], P[i * 3 + 2]), f;
+      double t = 0;
+      point_line_projection(v1, v2, v, f, t);
+      for (int d = 0; d < 3; ++d) pv[(size_t)i * 3 + d] = f[d];
+      tv[i] = t;
+    }
+    out.putd("pv", pv);
+    out.putd("t", tv);
+    fclose(out.f);
+    return 0;
+  }
 
   int nV = in.get<int32_t>(), nH = in.get<int32_t>();
   int energy = in.get<int32_t>(), n_outer = in.get<int32_t>(), n_inner = in.get<int32_t>();

@@ -114,6 +114,18 @@ def run_aabb(V, F, P):
         return read_out(pout)
 
 
+def run_point_line(A, B, P):
+    """The reference's point_line_projection on (segment A-B, point P) triples (driver mode 6)."""
+    with tempfile.TemporaryDirectory() as td:
+        pin, pout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(pin, "wb") as f:
+            _w(f, "i", 6)
+            _w(f, "i", P.shape[0])
+            _arr(f, A, "<f8"); _arr(f, B, "<f8"); _arr(f, P, "<f8")
+        subprocess.run([REF_BIN, pin, pout], check=True, stdout=subprocess.DEVNULL)
+        return read_out(pout)
+
+
 METRO_BIN = os.path.join(REPO, "oracle", "_ref", "metro_ref")
 
 

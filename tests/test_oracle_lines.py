@@ -51,3 +51,19 @@ def test_single_vertex_fallback():
     o, a = O.line_projection(V, [[0]], [0], np.array([[0.0, 0.0, 0.0]]), [0])
     # nearest vertex; the tangent adds (0,0,0)/0 to (1,0,0), as the reference does
     assert (o[0] == V[0]).all() and np.isnan(a[0]).all()
+
+
+def test_point_line_projection_vs_compiled_reference():
+    """The oracle's point_line_projection against the reference's own, compiled from
+    global_functions.cpp (tests/golden/point_line_cases.npz, tests/golden/make_lines.py):
+    1,276 (segment, point) triples, including squared lengths around the 1e-7 threshold,
+    zero-length segments, points at and perpendicular to the ends, and all of it shifted
+    far from the origin. Bit for bit."""
+    import os
+    c = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "point_line_cases.npz"))
+    pv = O.point_line_projection(c["A"], c["B"], c["P"])
+    assert np.array_equal(pv, c["pv"])
+    # both sides of the threshold and both clamps occur
+    L2 = ((c["B"] - c["A"]) ** 2).sum(axis=1)
+    assert (L2 < 1e-7).any() and ((L2 >= 1e-7) & (L2 < 1.1e-7)).any()
+    assert (c["t"] < 0).any() and (c["t"] > 1).any() and (c["t"] == 0).any()
