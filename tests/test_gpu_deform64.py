@@ -8,9 +8,13 @@ Teacher-forced steps: from the reference's own iterate k (k = 25 and 49, rounded
 multiples of 2^-20 and stored exactly), one device step must give the reference's iterate
 k+1 from the same start: energies to 1e-10 relative, 1,000 seeded vertices and three
 checksums over all vertices (sum, sum of squares, a seeded random-weight sum) to 1e-10
-relative. Free run: the device's 50-step trajectory from the same start, every step's
-energies and sampled vertices against the reference's, to the bound the teacher-forced
-per-step agreement and the loop's growth of differences allow (stated at each assert)."""
+relative. Free run: the device's 50-step trajectory from the same start against the
+reference's; the first steps to 1e-10, and then, since the loop amplifies last-bit
+differences about tenfold per step once lambda_T grows, within the envelope of the device's
+own divergence from a start perturbed at the size of the per-step differences."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
@@ -23,12 +27,14 @@ NAME = "deform_grid64_superellipsoid"
 
 @pytest.fixture(scope="module")
 def case():
-    try:
-        c = load(NAME)
-    except FileNotFoundError:
-        pytest.skip("fixture not written yet (tests/golden/make_deform64.py)")
-    from pyhexopt import meshgen as mg
-    _, H = mg.grid_hex(int(c["n"]))
+    c = dict(load(NAME))
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_deform64 as gen
+    Vq0, H, ids, _, _ = gen.problem()
+    # the regenerated start is the one the reference ran from
+    assert np.array_equal(gen.start_sums(Vq0), c["Vq0_sums"])
+    assert np.array_equal(ids, c["idsT"])
+    c["Vq0"] = Vq0
     return c, H
 
 
@@ -81,32 +87,51 @@ def test_teacher_forced_step(case, k):
     assert (eC < 1e-10).all()
 
 
-def test_free_run_50_steps(case):
-    c, H = case
-    q = float(c["q"])
-    X = c["Vq0"].astype(np.float64) * q
+def _free_run(c, H, X, steps):
+    """The device loop from X: per step the energies, lambda and the sampled vertices."""
     h, surf = _context(X, H, c)
     lam0, bound = float(c["lamT0"]), float(c["lamT_bound"])
     lam = lam0
-    ref = c["ref_log"]
-    eE, eV = [], []
+    Es, lams, Vs = [], [], []
     try:
-        for k in range(int(c["steps"])):
+        for _ in range(steps):
             E = _step(h, surf, lam)
-            assert abs(lam - ref[k, 3]) <= 1e-6 * ref[k, 3], k
-            Y = h.get_positions()
-            eE.append(rel(E[:2], ref[k, :2]))
-            eV.append(rel(Y[c["sample"]], c["ref_Vsample"][k + 1]))
+            Es.append(E)
+            lams.append(lam)
+            Vs.append(h.get_positions()[c["sample"]])
             lam = min(lam0 * max(E[1] / E[2] * lam, 1.0), bound)
     finally:
         h.close()
         surf.close()
-    eE, eV = np.array(eE), np.array(eV)
-    print("free run: energy differences %s" % np.array2string(eE, precision=1), flush=True)
-    print("free run: sampled-vertex differences %s" % np.array2string(eV, precision=1), flush=True)
-    # the first step starts from the same positions: a teacher-forced step
-    assert eE[0] < 1e-10 and eV[0] < 1e-10
-    # then differences of 1e-12 per step grow through the truncated CG; the bound holds the
-    # whole run to the scale the reference's own reordering sensitivity reaches (1e-4 on the
-    # small fixtures, tests/test_gpu_slim.py)
-    assert eE.max() < 1e-6 and eV.max() < 1e-4
+    return np.array(Es), np.array(lams), np.array(Vs)
+
+
+def test_free_run_50_steps(case):
+    """The device's 50-step trajectory from the reference's start against the reference's.
+    The loop amplifies last-bit differences (a 20-step truncated CG, lambda_T multiplied by
+    E_q / E_s each step): the yardstick is the device's own trajectory from the start perturbed
+    by 1e-13 of the coordinates (the size of the teacher-forced steps' per-step differences),
+    which must diverge from the unperturbed one about as fast as the reference does."""
+    c, H = case
+    q = float(c["q"])
+    X = c["Vq0"].astype(np.float64) * q
+    n = int(c["steps"])
+    ref = c["ref_log"]
+    EA, lamA, VA = _free_run(c, H, X, n)
+    noise = np.random.default_rng(7).standard_normal(X.shape)
+    EB, _, VB = _free_run(c, H, X + 1e-13 * np.abs(X).max() * noise, n)
+    eA = np.array([rel(VA[k], c["ref_Vsample"][k + 1]) for k in range(n)])
+    eB = np.array([rel(VA[k], VB[k]) for k in range(n)])
+    fA = np.array([rel(EA[k, :2], ref[k, :2]) for k in range(n)])
+    fB = np.array([rel(EA[k, :2], EB[k, :2]) for k in range(n)])
+    for k in range(n):
+        print("step %2d: lambda %.9g (reference %.9g); vertices vs reference %.2e, vs perturbed start %.2e; "
+              "energies %.2e, %.2e" % (k, lamA[k], ref[k, 3], eA[k], eB[k], fA[k], fB[k]), flush=True)
+    # before the amplification sets in the device follows the reference to the per-step error
+    assert eA[:3].max() < 1e-10 and fA[:3].max() < 1e-10
+    # then the envelope of the differences stays within the loop's own sensitivity: the device
+    # follows the reference at least as closely as it follows itself from a start perturbed
+    # by 1e-13 (measured: 6e-13 against 1.5e-10 at step 1, 9e-5 against 3e-3 at step 8, both
+    # saturating at a few 1e-3 once lambda_T reaches its bound)
+    envA, envB = np.maximum.accumulate(eA), np.maximum.accumulate(eB)
+    assert (envA <= np.maximum(envB, 1e-12)).all()
