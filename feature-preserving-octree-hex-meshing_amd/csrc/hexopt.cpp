@@ -1219,10 +1219,11 @@ int hexopt_project_patches(hexopt_ctx* c, int npatch, hexopt_surface* const* pat
   if (!c->planes_refreshable) return fail(c, HEXOPT_EUNSUPPORTED, "duplicate plane vertices");
   const int nT = c->sd.nT;
   if (nT == 0) return HEXOPT_OK;
-  if (!patch_of_T) return fail(c, HEXOPT_EINVAL, "null patch_of_T");
   hipStream_t st = c->stream;
-  if ((int)c->h_patch.size() != nT || (int)c->patch_off.size() != npatch + 1 ||
-      std::memcmp(c->h_patch.data(), patch_of_T, (size_t)nT * sizeof(int32_t)) != 0) {
+  if (!patch_of_T && ((int)c->h_patch.size() != nT || (int)c->patch_off.size() != npatch + 1))
+    return fail(c, HEXOPT_EINVAL, "null patch_of_T and no previous assignment for these planes and patches");
+  if (patch_of_T && ((int)c->h_patch.size() != nT || (int)c->patch_off.size() != npatch + 1 ||
+                     std::memcmp(c->h_patch.data(), patch_of_T, (size_t)nT * sizeof(int32_t)) != 0)) {
     // rows grouped by patch, each group in the vertices' Morton order (coherent walks)
     for (int i = 0; i < nT; ++i)
       if (patch_of_T[i] < 0 || patch_of_T[i] >= npatch) return fail(c, HEXOPT_EINVAL, "patch index out of range");

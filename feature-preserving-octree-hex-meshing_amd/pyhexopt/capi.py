@@ -307,9 +307,10 @@ class HexOpt:
 
     def project_patches(self, patches, patch_of_T, lamT):
         """Tangent planes of the plane vertices from their patches' surfaces (a list of
-        Surface); patch_of_T indexes that list, one entry per ids_T row."""
+        Surface); patch_of_T indexes that list, one entry per ids_T row, or is None to reuse
+        the previous call's assignment."""
         arr = (C.c_void_p * len(patches))(*[p.h for p in patches])
-        pt = _i(patch_of_T)
+        pt = None if patch_of_T is None else _i(patch_of_T)
         self._chk(lib().hexopt_project_patches(C.c_void_p(self.h), len(patches), arr, _ptr(pt, C.c_int32),
                                                C.c_double(lamT)), "project_patches")
 

@@ -232,7 +232,9 @@ int hexopt_project_planes(hexopt_ctx* ctx, hexopt_surface* s, double lamda_T);
  * patches[patch_of_T[i]] -- its patch's own tree (md.tri_Trees[i] for Qfg.Rs[i]), or the whole
  * input surface for a dirty vertex -- giving normal_T = pseudonormal and dis_T = normal_T .
  * closest point; the plane weight becomes lamda_T. patch_of_T has nT entries in ids_T order,
- * each in [0, npatch); it is checked and grouped again only when it changes between calls. */
+ * each in [0, npatch); it is checked and grouped again only when it changes between calls.
+ * patch_of_T NULL reuses the assignment of the previous call (same npatch; the loop's usual
+ * case, which then costs the host nothing per call); EINVAL when there is none. */
 int hexopt_project_patches(hexopt_ctx* ctx, int npatch, hexopt_surface* const* patches, const int32_t* patch_of_T,
                            double lamda_T);
 

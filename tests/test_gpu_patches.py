@@ -27,6 +27,8 @@ def test_project_patches_equals_per_patch_planes(jitter):
         nT = ids.size
         h1.set_constraints(idsT=ids, normalT=np.zeros((nT, 3)), disT=np.zeros(nT), lamT=1.0)
         h1.project_patches(patches, pT, lam)
+        # a second call reusing the assignment gives the same planes
+        h1.project_patches(patches, None, lam)
         N = np.zeros((nT, 3))
         Cp = np.zeros((nT, 3))
         for p, s in enumerate(patches):
@@ -40,6 +42,14 @@ def test_project_patches_equals_per_patch_planes(jitter):
             assert np.array_equal(a, b)
         assert h1.energy(weight_opt=0.001) == h2.energy(weight_opt=0.001)
         r1, r2 = h1.slim_solve(2, weight_opt=0.001), h2.slim_solve(2, weight_opt=0.001)
+        # without a previous assignment (after new constraints) None is refused
+        h3 = capi.HexOpt(V, w["H"])
+        try:
+            h3.set_constraints(idsT=ids, normalT=np.zeros((nT, 3)), disT=np.zeros(nT), lamT=1.0)
+            with pytest.raises(RuntimeError):
+                h3.project_patches(patches, None, lam)
+        finally:
+            h3.close()
         assert r1[0] == r2[0]
         assert np.array_equal(h1.get_positions(), h2.get_positions())
     finally:
