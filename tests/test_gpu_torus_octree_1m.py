@@ -8,10 +8,10 @@ Bounds, from the errors the FP32 arithmetic leaves (measured values in brackets)
 - dE/dx at the same (FP32-representable) positions: 1e-6 of the largest component [3.6e-8];
 - SLIM weight blocks: 1e-6 of the largest entry [2.8e-8];
 - the FP32 CG (the reference's 20 steps, no refinement) reproduces the FP64 path's truncated
-  solve: directions within 1e-5 relative [2.7e-7], the same accepted step of the flip-free
+  solve: directions within 1e-5 relative [1.1e-7], the same accepted step of the flip-free
   line search (the first iteration on this mesh is the case whose FP32-summed line-search
   energies once rejected the FP64 path's step), quality energy after the step within 1e-12
-  [3e-15];
+  [8e-16];
 - the FP64 residual of the FP32 direction agrees with the FP32 CG's own estimate to 1e-6
   [1.6e-8]."""
 import numpy as np
