@@ -552,6 +552,21 @@ __device__ __forceinline__ void glue_rows(const SoftDev& sd, int g, const double
   for (int d = 0; d < 3; ++d) r[d] += c * ((double)k * pv[d] - sd.g_sum[(size_t)gr * 3 + d]);
 }
 
+// Per-hex corner arrays, each ring (corners 0-3, 4-7) turned one step: row i takes the row
+// that followed it. The corner tets of a ring are one pattern of rows turned round the ring
+// (kHexTet: tets 0-3 are {c, c-1, c+4, c+1}, tets 4-7 {c, c-1, c+1, c-4}, within each ring),
+// so a loop over a ring's tets that turns the arrays after each tet reads and writes the same
+// compile-time rows every time, with no selects over the 8 rows; four turns bring them back.
+template <class T, int N>
+__device__ __forceinline__ void rotate_rings(T a[8][N]) {
+#pragma unroll
+  for (int d = 0; d < N; ++d) {
+    const T b0 = a[0][d], t0 = a[4][d];
+    a[0][d] = a[1][d]; a[1][d] = a[2][d]; a[2][d] = a[3][d]; a[3][d] = b0;
+    a[4][d] = a[5][d]; a[5][d] = a[6][d]; a[6][d] = a[7][d]; a[7][d] = t0;
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Local step: per tet SVD -> W, R (slim_m.cpp:236-377); stores K = M W^T W (scaled by
 // invs^2 for the cube operator) and scatters the rhs A^T M (W R) (buildRhs, :1044-1092) and
@@ -560,60 +575,85 @@ struct LocalStepOp {
   RefOp ro; int nV, nH; const int* conn; const double* X; double* K;
   int energy; double M, exp_f, prox;
   SoftDev sd; double* rhs; double* invd;
+  // corner tet c on rows T0..T3 of p / out (compile-time indices, see rotate_rings); the sums
+  // are those of scatter_tet into a zeroed per-tet array added to out, in the same order
+  template <int T0, int T1, int T2, int T3>
+  __device__ __forceinline__ void tet(int h, int c, const double p[8][3], double out[8][6]) const {
+    constexpr int tc[4] = {T0, T1, T2, T3};
+    double B[3][3], E[3][3], J[3][3], W[3][3], R[3][3];
+    tet_B(ro, nH, h, c, B);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) E[d][k] = p[tc[k + 1]][d] - p[T0][d];
+    matmul3(E, B, J);
+    slim_weights(J, energy, exp_f, W, R);
+    double Kt[3][3];  // M W^T W
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Kt[i][j] = M * (W[0][i] * W[0][j] + W[1][i] * W[1][j] + W[2][i] * W[2][j]);
+    double s2 = 1.0;
+    if (ro.mode == REF_CUBE) { double s = ro.invs[(size_t)c * nH + h]; s2 = s * s; }
+    const double ks[6] = {Kt[0][0], Kt[1][1], Kt[2][2], Kt[0][1], Kt[0][2], Kt[1][2]};
+#pragma unroll
+    for (int j = 0; j < 6; ++j) K[k_at<double>(c * 6 + j, h, nH)] = ks[j] * s2;
+    // rhs: (W^T M W R) scattered through B
+    double Z[3][3];
+    {
+      double F[3][3];
+      matmul3(W, R, F);
+#pragma unroll
+      for (int b2 = 0; b2 < 3; ++b2)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) Z[b2][cc] = M * (W[0][b2] * F[0][cc] + W[1][b2] * F[1][cc] + W[2][b2] * F[2][cc]);
+    }
+    // row k + 1 gets Z B_k, row 0 the sum of their negatives (first from 0), each then added
+    // to out
+    double t0[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double f = Z[d][0] * B[k][0] + Z[d][1] * B[k][1] + Z[d][2] * B[k][2];
+        out[tc[k + 1]][d] += f;
+        t0[d] += -f;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[T0][d] += t0[d];
+    // diagonal: Kt_bb * |G[:,j]|^2; G[:,k] = B[k-1,:], G[:,0] = -sum_k B[k,:]
+    double g0[3] = {-(B[0][0] + B[1][0] + B[2][0]), -(B[0][1] + B[1][1] + B[2][1]), -(B[0][2] + B[1][2] + B[2][2])};
+    double n0 = g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2];
+    double nk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) nk[k] = B[k][0] * B[k][0] + B[k][1] * B[k][1] + B[k][2] * B[k][2];
+    {
+      // each product rounded before its sum (no fused multiply-add)
+#pragma clang fp contract(off)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) out[T0][3 + d] += Kt[d][d] * n0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[tc[k + 1]][3 + d] += Kt[d][d] * nk[k];
+    }
+  }
   __device__ double hex(int h, double out[8][6]) const {
     int vid[8];
     double p[8][3];
     gather8(X, nV, conn, nH, h, vid, p);
 #pragma unroll 1
-    for (int c = 0; c < 8; ++c) {
-      double B[3][3], E[3][3], J[3][3], W[3][3], R[3][3];
-      tet_B(ro, nH, h, c, B);
-      tet_edges(p, c, E);
-      matmul3(E, B, J);
-      slim_weights(J, energy, exp_f, W, R);
-      double Kt[3][3];  // M W^T W
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) Kt[i][j] = M * (W[0][i] * W[0][j] + W[1][i] * W[1][j] + W[2][i] * W[2][j]);
-      double s2 = 1.0;
-      if (ro.mode == REF_CUBE) { double s = ro.invs[(size_t)c * nH + h]; s2 = s * s; }
-      const double ks[6] = {Kt[0][0], Kt[1][1], Kt[2][2], Kt[0][1], Kt[0][2], Kt[1][2]};
-#pragma unroll
-      for (int j = 0; j < 6; ++j) K[k_at<double>(c * 6 + j, h, nH)] = ks[j] * s2;
-      // rhs: (W^T M W R) scattered through B
-      double Z[3][3];
-      {
-        double F[3][3];
-        matmul3(W, R, F);
-#pragma unroll
-        for (int b2 = 0; b2 < 3; ++b2)
-#pragma unroll
-          for (int cc = 0; cc < 3; ++cc) Z[b2][cc] = M * (W[0][b2] * F[0][cc] + W[1][b2] * F[1][cc] + W[2][b2] * F[2][cc]);
-      }
-      double tmp[8][3];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) tmp[k][0] = tmp[k][1] = tmp[k][2] = 0;
-      scatter_tet(Z, B, c, tmp);
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) out[k][d] += tmp[k][d];
-      // diagonal: Kt_bb * |G[:,j]|^2; G[:,k] = B[k-1,:], G[:,0] = -sum_k B[k,:]
-      double g0[3] = {-(B[0][0] + B[1][0] + B[2][0]), -(B[0][1] + B[1][1] + B[2][1]), -(B[0][2] + B[1][2] + B[2][2])};
-      double n0 = g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2];
-      {
-        std::remove_reference_t<decltype(Kt[0][0])> v[3] = {Kt[0][0] * n0, Kt[1][1] * n0, Kt[2][2] * n0};
-        add_row<3, 3>(out, kHexTet[c][0], v);
-      }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        double nk = B[k][0] * B[k][0] + B[k][1] * B[k][1] + B[k][2] * B[k][2];
-        {
-          std::remove_reference_t<decltype(Kt[0][0])> v[3] = {Kt[0][0] * nk, Kt[1][1] * nk, Kt[2][2] * nk};
-          add_row<3, 3>(out, kHexTet[c][k + 1], v);
-        }
-      }
+    for (int r = 0; r < 4; ++r) {
+      tet<0, 3, 4, 1>(h, r, p, out);
+      rotate_rings(p);
+      rotate_rings(out);
+    }
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+      tet<4, 7, 5, 0>(h, 4 + r, p, out);
+      rotate_rings(p);
+      rotate_rings(out);
     }
     return 0.0;
   }
@@ -647,7 +687,8 @@ struct LocalStepOp {
 };
 
 // the FP64 local step needs all 256 registers on its own (one wave per SIMD); two waves per
-// SIMD hide its FP64 latency better than the registers it then spills cost
+// SIMD hide its FP64 latency better than the registers it then spills cost (125 spilled
+// values: 2.13 ms against 2.63 ms per pass on the 1.1M-hex octree torus)
 template <>
 struct TileWaves<LocalStepOp> {
   static constexpr int value = 2;
@@ -745,8 +786,9 @@ struct ApplyOp {
 // direction rhs -dE/dx / 2 and the Jacobi diagonal. The corner-determinant sign is tested
 // in FP64 (inverted-tet count); the energy is summed in FP64.
 struct AdjLocalOp32;
-// three waves per SIMD (a few spilled registers): the FP64 corners stay live through the
-// corner-tet loop
+// three waves per SIMD: the FP64 corners and the hex's 8 x 6 sums stay live through the
+// corner-tet loops; 37 values spill, and it still runs faster than two waves without spills
+// (415 against 420 us on the 1.1M-hex octree torus)
 template <>
 struct TileWaves<AdjLocalOp32> {
   static constexpr int value = 3;
@@ -755,87 +797,103 @@ struct TileWaves<AdjLocalOp32> {
 struct AdjLocalOp32 {
   RefOp ro; int nV, nH; const int* conn; const double* X; float* K; double M, prox;
   SoftDev sd; float* rhs; float* invd; int* n_inverted;
+
+  // Corner tet c whose corners are rows T0, T1, T2, T3 of pd / out (compile-time indices: the
+  // caller rotates the rows so that every corner tet of a ring takes the same four rows).
+  // Per tet, FP64 first: the Jacobian J = E B from the FP64 corners, its orientation (sign of
+  // det J; for a cube reference B = invs * (a permutation with determinant +1 for every
+  // corner) and invs > 0, so the sign is that of det E), and its deviation from isometry
+  // D = J J^T - I. Then FP32 for the rest: the weights (the reference's m = 1 branch,
+  // |s_i - 1| < 1e-8 (slim_m.cpp:351-353), is taken for the tet when |D|_F < 2e-8 puts every
+  // singular value inside it, a threshold FP32 cannot resolve), and the energy and dE/dJ in
+  // their deviation form, which keeps FP32's relative precision where J is near a rotation.
+  template <int T0, int T1, int T2, int T3>
+  __device__ __forceinline__ void tet(int h, int c, const double pd[8][3], float out[8][6], double& e, int& inv) const {
+    constexpr int tc[4] = {T0, T1, T2, T3};
+    double E64[3][3], B64[3][3], J64[3][3], D64[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) E64[d][k] = pd[tc[k + 1]][d] - pd[T0][d];
+    tet_B(ro, nH, h, c, B64);
+    matmul3(E64, B64, J64);
+    inv += (ro.mode == REF_CUBE ? det3(E64) : det3(J64)) < 0.0;
+    const double dev2 = isometry_deviation(J64, D64);
+    const bool iso = dev2 < 4e-16;
+    float B[3][3], J[3][3], D[3][3], W[3][3], P[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j2 = 0; j2 < 3; ++j2) {
+        B[i][j2] = (float)B64[i][j2];
+        J[i][j2] = (float)J64[i][j2];
+        D[i][j2] = (float)D64[i][j2];
+      }
+    if (iso) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j2 = 0; j2 < 3; ++j2) W[i][j2] = i == j2 ? 1.0f : 0.0f;
+    } else {
+      slim_weights_sd_stable(J, W);
+    }
+    // the deviation form while I + D is well conditioned (|D|_F < 0.5); the direct form
+    // else, where inverting J J^T would square J's condition number
+    if (dev2 < 0.25) e += M * (6.0 + (double)sd_energy_grad_dev(J, D, P));
+    else e += M * (double)sd_energy_grad(J, P);
+    float Kt[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Kt[i][j] = (float)M * (W[0][i] * W[0][j] + W[1][i] * W[1][j] + W[2][i] * W[2][j]);
+    float s2 = 1.0f;
+    if (ro.mode == REF_CUBE) { float sc = (float)ro.invs[(size_t)c * nH + h]; s2 = sc * sc; }
+    const float ks[6] = {Kt[0][0], Kt[1][1], Kt[2][2], Kt[0][1], Kt[0][2], Kt[1][2]};
+#pragma unroll
+    for (int j = 0; j < 6; ++j) K[k_at<float>(c * 6 + j, h, nH)] = ks[j] * s2;
+    // -dE/dx / 2 of the tet's corners: row k + 1 gets P B_k, row 0 minus their sum
+    const float hm = -0.5f * (float)M;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float f = hm * (P[d][0] * B[k][0] + P[d][1] * B[k][1] + P[d][2] * B[k][2]);
+        out[tc[k + 1]][d] += f;
+        out[T0][d] -= f;
+      }
+    }
+    // Jacobi diagonal
+    const float g0[3] = {-(B[0][0] + B[1][0] + B[2][0]), -(B[0][1] + B[1][1] + B[2][1]),
+                         -(B[0][2] + B[1][2] + B[2][2])};
+    const float n0 = g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[T0][3 + d] += Kt[d][d] * n0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float nk = B[k][0] * B[k][0] + B[k][1] * B[k][1] + B[k][2] * B[k][2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) out[tc[k + 1]][3 + d] += Kt[d][d] * nk;
+    }
+  }
+
   __device__ double hex(int h, float out[8][6]) const {
     int vid[8];
     double pd[8][3];
     gather8(X, nV, conn, nH, h, vid, pd);
     double e = 0.0;
     int inv = 0;
-    // Per corner tet, FP64 first: the Jacobian J = E B from the FP64 corners, its orientation
-    // (sign of det J; for a cube reference B = invs * (a permutation with determinant +1 for
-    // every corner) and invs > 0, so the sign is that of det E), and its deviation from
-    // isometry D = J J^T - I. Then FP32 for the rest: the weights (the reference's m = 1
-    // branch, |s_i - 1| < 1e-8 (slim_m.cpp:351-353), is taken for the tet when
-    // |D|_F < 2e-8 puts every singular value inside it, a threshold FP32 cannot resolve),
-    // and the energy and dE/dJ in their deviation form, which keeps FP32's relative
-    // precision where J is near a rotation.
+    // the ring's first tet pattern on rows turned r steps is tet r (rotate_rings)
 #pragma unroll 1
-    for (int c = 0; c < 8; ++c) {
-      double E64[3][3], B64[3][3], J64[3][3], D64[3][3];
-      tet_edges(pd, c, E64);
-      tet_B(ro, nH, h, c, B64);
-      matmul3(E64, B64, J64);
-      inv += (ro.mode == REF_CUBE ? det3(E64) : det3(J64)) < 0.0;
-      const double dev2 = isometry_deviation(J64, D64);
-      const bool iso = dev2 < 4e-16;
-      float B[3][3], J[3][3], D[3][3], W[3][3], P[3][3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j2 = 0; j2 < 3; ++j2) {
-          B[i][j2] = (float)B64[i][j2];
-          J[i][j2] = (float)J64[i][j2];
-          D[i][j2] = (float)D64[i][j2];
-        }
-      if (iso) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int j2 = 0; j2 < 3; ++j2) W[i][j2] = i == j2 ? 1.0f : 0.0f;
-      } else {
-        slim_weights_sd_stable(J, W);
-      }
-      // the deviation form while I + D is well conditioned (|D|_F < 0.5); the direct form
-      // else, where inverting J J^T would square J's condition number
-      if (dev2 < 0.25) e += M * (6.0 + (double)sd_energy_grad_dev(J, D, P));
-      else e += M * (double)sd_energy_grad(J, P);
-      float Kt[3][3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) Kt[i][j] = (float)M * (W[0][i] * W[0][j] + W[1][i] * W[1][j] + W[2][i] * W[2][j]);
-      float s2 = 1.0f;
-      if (ro.mode == REF_CUBE) { float s = (float)ro.invs[(size_t)c * nH + h]; s2 = s * s; }
-      const float ks[6] = {Kt[0][0], Kt[1][1], Kt[2][2], Kt[0][1], Kt[0][2], Kt[1][2]};
-#pragma unroll
-      for (int j = 0; j < 6; ++j) K[k_at<float>(c * 6 + j, h, nH)] = ks[j] * s2;
-      const float hm = -0.5f * (float)M;
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) P[i][j] *= hm;
-      float tmp[8][3];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) tmp[k][0] = tmp[k][1] = tmp[k][2] = 0.0f;
-      scatter_tet(P, B, c, tmp);
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) out[k][d] += tmp[k][d];
-      float g0[3] = {-(B[0][0] + B[1][0] + B[2][0]), -(B[0][1] + B[1][1] + B[2][1]), -(B[0][2] + B[1][2] + B[2][2])};
-      float n0 = g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2];
-      {
-        std::remove_reference_t<decltype(Kt[0][0])> v[3] = {Kt[0][0] * n0, Kt[1][1] * n0, Kt[2][2] * n0};
-        add_row<3, 3>(out, kHexTet[c][0], v);
-      }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float nk = B[k][0] * B[k][0] + B[k][1] * B[k][1] + B[k][2] * B[k][2];
-        {
-          std::remove_reference_t<decltype(Kt[0][0])> v[3] = {Kt[0][0] * nk, Kt[1][1] * nk, Kt[2][2] * nk};
-          add_row<3, 3>(out, kHexTet[c][k + 1], v);
-        }
-      }
+    for (int r = 0; r < 4; ++r) {
+      tet<0, 3, 4, 1>(h, r, pd, out, e, inv);
+      rotate_rings(pd);
+      rotate_rings(out);
+    }
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+      tet<4, 7, 5, 0>(h, 4 + r, pd, out, e, inv);
+      rotate_rings(pd);
+      rotate_rings(out);
     }
     if (inv) atomicAdd(n_inverted, inv);
     return e;
@@ -1942,37 +2000,67 @@ template <class T, class P>
 struct GradOp {
   RefOp ro; int nV, nH; const int* conn; const P* X; P* G; double M;
   SoftDev sd; const double* aL; double* epart_unused;
+  // corner tet c on rows T0..T3 of pd / o (compile-time indices, see rotate_rings)
+  template <int T0, int T1, int T2, int T3>
+  __device__ __forceinline__ void tet(int h, int c, const double pd[8][3], T o[8][3], double& e) const {
+    constexpr int tc[4] = {T0, T1, T2, T3};
+    T B[3][3], J[3][3], Pm[3][3];
+    if constexpr (sizeof(T) == 4) {
+      // FP32: J and D = J J^T - I formed in FP64 from the corners, the energy and dE/dJ in
+      // the deviation form (FP32 keeps its relative precision near a rotation)
+      double B64[3][3], E64[3][3], J64[3][3], D64[3][3];
+      tet_B(ro, nH, h, c, B64);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) E64[d][k] = pd[tc[k + 1]][d] - pd[T0][d];
+      matmul3(E64, B64, J64);
+      const double dev2 = isometry_deviation(J64, D64);
+      T D[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) { B[i][j] = (T)B64[i][j]; J[i][j] = (T)J64[i][j]; D[i][j] = (T)D64[i][j]; }
+      if (dev2 < 0.25) e += M * (6.0 + (double)sd_energy_grad_dev(J, D, Pm));   // see AdjLocalOp32
+      else e += M * (double)sd_energy_grad(J, Pm);
+    } else {
+      T E[3][3];
+      tet_B(ro, nH, h, c, B);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) E[d][k] = (T)pd[tc[k + 1]][d] - (T)pd[T0][d];
+      matmul3(E, B, J);
+      e += M * (double)sd_energy_grad(J, Pm);
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Pm[i][j] *= (T)M;
+    // scatter_tet's sums in its order: row k + 1 gets F B_k, row 0 minus it
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T f[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) f[d] = Pm[d][0] * B[k][0] + Pm[d][1] * B[k][1] + Pm[d][2] * B[k][2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) o[tc[k + 1]][d] += f[d];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) o[T0][d] += -f[d];
+    }
+  }
   __device__ double hex(int h, T o[8][3]) const {
     int vid[8];
     double pd[8][3];
     gather8(X, nV, conn, nH, h, vid, pd);
     double e = 0.0;
 #pragma unroll 1
-    for (int c = 0; c < 8; ++c) {
-      T B[3][3], J[3][3], Pm[3][3];
-      if constexpr (sizeof(T) == 4) {
-        // FP32: J and D = J J^T - I formed in FP64 from the corners, the energy and dE/dJ in
-        // the deviation form (FP32 keeps its relative precision near a rotation)
-        double B64[3][3], E64[3][3], J64[3][3], D64[3][3];
-        tet_B(ro, nH, h, c, B64);
-        tet_edges(pd, c, E64);
-        matmul3(E64, B64, J64);
-        const double dev2 = isometry_deviation(J64, D64);
-        T D[3][3];
-        for (int i = 0; i < 3; ++i)
-          for (int j = 0; j < 3; ++j) { B[i][j] = (T)B64[i][j]; J[i][j] = (T)J64[i][j]; D[i][j] = (T)D64[i][j]; }
-        if (dev2 < 0.25) e += M * (6.0 + (double)sd_energy_grad_dev(J, D, Pm));   // see AdjLocalOp32
-        else e += M * (double)sd_energy_grad(J, Pm);
-      } else {
-        T E[3][3];
-        tet_B(ro, nH, h, c, B);
-        tet_edges(pd, c, E);
-        matmul3(E, B, J);
-        e += M * (double)sd_energy_grad(J, Pm);
-      }
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) Pm[i][j] *= (T)M;
-      scatter_tet(Pm, B, c, o);
+    for (int r = 0; r < 4; ++r) {
+      tet<0, 3, 4, 1>(h, r, pd, o, e);
+      rotate_rings(pd);
+      rotate_rings(o);
+    }
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+      tet<4, 7, 5, 0>(h, 4 + r, pd, o, e);
+      rotate_rings(pd);
+      rotate_rings(o);
     }
     return e;
   }
