@@ -439,6 +439,33 @@ int64_t hexclean_drop_small_pieces(int64_t nV, int64_t nH, const int32_t* H, uin
   return guarded(nV, nH, H, H_flag, [&] { drop_small(nV, nH, H, H_flag); });
 }
 
+int64_t hexclean_medial_surface(int64_t nV, int64_t nH, const int32_t* H, const uint8_t* H_flag, uint8_t* v_flag,
+                                int32_t* quads) {
+  return catching([&]() -> int64_t {
+    check_hexes(nV, nH, H);
+    if (nH > 0 && !H_flag) fail("H_flag is null");
+    // the faces of the whole mesh in build_connectivity's numbering, then the loop that ends
+    // clean_hex_mesh (grid_hex_meshing.cpp:1966-1981) over them in that order
+    Conn c;
+    std::vector<int32_t> all(nH);
+    for (int64_t h = 0; h < nH; ++h) all[h] = (int32_t)h;
+    build(c, nV, H, all, false);
+    if (v_flag) std::memset(v_flag, 0, (size_t)nV);
+    int64_t n = 0;
+    for (int64_t f = 0; f < c.nF; ++f) {
+      const int32_t a = c.fh[2 * f], b = c.fh[2 * f + 1];
+      const bool on = c.fbnd[f] ? H_flag[a] != 0 : (H_flag[a] != 0) != (H_flag[b] != 0);
+      if (!on) continue;
+      for (int q = 0; q < 4; ++q) {
+        if (v_flag) v_flag[c.fvs[4 * f + q]] = 1;
+        if (quads) quads[4 * n + q] = c.fvs[4 * f + q];
+      }
+      ++n;
+    }
+    return n;
+  });
+}
+
 int64_t hexclean_clean_flags(int64_t nV, int64_t nH, const int32_t* H, uint8_t* H_flag) {
   return guarded(nV, nH, H, H_flag, [&] {
     tagging(nV, nH, H, H_flag);

@@ -11,7 +11,8 @@ import numpy as np
 from . import capi
 
 EXPORTS = ["hexclean_error", "hexclean_reorder_hex_mesh", "hexclean_tagging_uneven_element",
-           "hexclean_clean_non_manifold_ve", "hexclean_drop_small_pieces", "hexclean_clean_flags"]
+           "hexclean_clean_non_manifold_ve", "hexclean_drop_small_pieces", "hexclean_clean_flags",
+           "hexclean_medial_surface"]
 
 _READY = False
 
@@ -28,9 +29,11 @@ def _lib():
         L.hexclean_error.restype = C.c_char_p
         L.hexclean_reorder_hex_mesh.argtypes = [i64, vp, i64, vp, vp]
         L.hexclean_reorder_hex_mesh.restype = i64
-        for n in EXPORTS[2:]:
+        for n in EXPORTS[2:6]:
             getattr(L, n).argtypes = [i64, i64, vp, vp]
             getattr(L, n).restype = i64
+        L.hexclean_medial_surface.argtypes = [i64, i64, vp, vp, vp, vp]
+        L.hexclean_medial_surface.restype = i64
         _READY = True
     return L
 
@@ -82,3 +85,20 @@ def clean_flags(nV, H, flag):
     """tagging_uneven_element, then (unless nothing is flagged) clean_non_manifold_ve and
     drop_small_pieces: clean_hex_mesh's flags from the inside flags signed_dis < 0."""
     return _run("hexclean_clean_flags", nV, H, flag)
+
+
+def medial_surface(nV, H, flag):
+    """The medial-surface tags closing clean_hex_mesh (grid_hex_meshing.cpp:1966-1981): the
+    vertex flags (nV bools) and the faces on it (m x 4 vertex ids, in the reference's face
+    order) of the flagged region's boundary."""
+    L = _lib()
+    H = _hexes(H)
+    f = np.ascontiguousarray(np.asarray(flag, dtype=bool), dtype=np.uint8)
+    if f.shape != (len(H),):
+        raise ValueError("one flag per hex")
+    v = np.zeros(int(nV), dtype=np.uint8)
+    q = np.zeros((max(1, 6 * len(H)), 4), dtype=np.int32)
+    n = L.hexclean_medial_surface(int(nV), len(H), H.ctypes.data, f.ctypes.data, v.ctypes.data, q.ctypes.data)
+    if n < 0:
+        raise HexCleanError(L.hexclean_error().decode(errors="replace"))
+    return v.astype(bool), q[:n].copy()

@@ -54,8 +54,19 @@ int64_t hexclean_drop_small_pieces(int64_t nV, int64_t nH, const int32_t* H, uin
  * tagging_uneven_element, then, unless no hex is flagged, clean_non_manifold_ve and
  * drop_small_pieces. H_flag comes in as signed_dis < 0 of each hex's box centre
  * (hexopt_hex_inside). The scaffold layers and the medial-surface tags that follow in the
- * reference are not part of it. */
+ * reference are not part of it (the tags: hexclean_medial_surface). */
 int64_t hexclean_clean_flags(int64_t nV, int64_t nH, const int32_t* H, uint8_t* H_flag);
+
+/* The medial-surface tags that end clean_hex_mesh (grid_hex_meshing.cpp:1966-1981): over the
+ * faces of the whole mesh in build_connectivity's order, a face is on the medial surface when
+ * it is a boundary face of a flagged hex or an inner face between a flagged and an unflagged
+ * hex, and so are its 4 vertices. v_flag (nullable, nV bytes) gets 1 for the vertices on it;
+ * quads (nullable, room for 6 nH x 4 ids) gets the faces on it in face order, each with the
+ * corners of its first occurrence, as the reference's Hybrid::vs. Returns the number of faces
+ * on the medial surface (the scaffold layers of scaffold_type 2 and 3, which come before these
+ * tags in the reference, are not built). */
+int64_t hexclean_medial_surface(int64_t nV, int64_t nH, const int32_t* H, const uint8_t* H_flag, uint8_t* v_flag,
+                                int32_t* quads);
 
 #ifdef __cplusplus
 }

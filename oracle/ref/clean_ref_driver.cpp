@@ -4,7 +4,9 @@
 // (:2006-2080) and drop_small_pieces (:2081-2124), all the reference's own code compiled from
 // its sources (oracle/ref/Makefile). Writes the flags after each pass.
 // Input: nV, nH (int32), H (nH x 8 int32), flags (nH int32). Output (refio format): "tag",
-// "manifold", "final" (nH doubles, 0 / 1).
+// "manifold", "final" (nH doubles, 0 / 1), then the medial-surface tags of the final flags
+// (:1966-1981): "medial_v" (nV doubles, 0 / 1) and "medial_f" (4 vertex ids per tagged face,
+// in face order).
 // Test infrastructure only; built in the development container, never shipped.
 #include <cstdio>
 #include <cstdint>
@@ -73,6 +75,33 @@ int main(int argc, char** argv) {
     put(out, "manifold", md.H_flag);
   }
   put(out, "final", md.H_flag);
+  // the medial-surface tags that end clean_hex_mesh (grid_hex_meshing.cpp:1966-1981), on the
+  // reference's own faces of the whole mesh; written as vertex flags and the tagged faces'
+  // vertex ids in face order
+  std::vector<bool> vmed(nV, false);
+  std::vector<double> fmed;
+  for (auto& f : mi.Fs) {
+    if ((f.boundary && md.H_flag[f.neighbor_hs[0]]) ||
+        (!f.boundary && (md.H_flag[f.neighbor_hs[0]] != md.H_flag[f.neighbor_hs[1]]))) {
+      f.on_medial_surface = true;
+      for (auto& vid : f.vs) {
+        mi.Vs[vid].on_medial_surface = true;
+        vmed[vid] = true;
+        fmed.push_back(vid);
+      }
+    }
+  }
+  put(out, "medial_v", vmed);
+  {
+    uint32_t ln = 8;
+    fwrite(&ln, 4, 1, out);
+    fwrite("medial_f", 1, 8, out);
+    char kind = 'd';
+    fwrite(&kind, 1, 1, out);
+    uint64_t nb = fmed.size() * 8;
+    fwrite(&nb, 8, 1, out);
+    if (nb) fwrite(fmed.data(), 1, nb, out);
+  }
   fclose(out);
   return 0;
 }

@@ -23,6 +23,15 @@
 //      iteration is logged, then checked against one slim_solve2(data, -1, 1) call.
 //   6  point_line_projection (global_functions.cpp:3454-3465), the segment projection of the
 //      feature-line branch, on given (segment, point) triples.
+//   7  the feature-alignment loop body (grid_hex_meshing.cpp:3851-3870) with every boundary
+//      vertex clean: per outer iteration dirty_graph_projection (:3954-4082) -- CORNER rows
+//      keep their input targets, LINE rows are projected onto their curve (the LINE branch,
+//      :3967-4019, written out around the reference's point_line_projection), REGULAR rows
+//      onto their own patch's tree (:4050-4080) -- then compute_referenceMesh(cube, H_flag),
+//      slim_m_opt(ts, 1, -1) and lamda_T = lamda_L = lamda_C updated together (:3866).
+//      Extra input after the surface: the patch of each surface face, the patch of each plane
+//      row, the curves (vertex lists into the surface, circle flags) and the curve of each
+//      line row.
 #include "slim_m.h"
 #include "global_functions.h"
 #include "igl/flip_avoiding_line_search.h"
@@ -579,6 +588,98 @@ int main(int argc, char** argv) {
       log.push_back(data.energy); log.push_back(data.engery_quality); log.push_back(data.engery_soft);
       log.push_back(fc.lamda_T);
       fc.lamda_T = std::min(LAMDA0 * std::max(data.engery_quality / data.engery_soft * fc.lamda_T, 1.0), lamT_bound);
+      auto f = flat3(ts.UV);
+      Vtrace.insert(Vtrace.end(), f.begin(), f.end());
+    }
+    out.putd("t_iter", t_iter);
+  } else if (mode == 7) {
+    auto fpatch = in.arr<int32_t>(nsf);
+    auto patchT = in.arr<int32_t>(nTc);
+    const int ncurve = in.get<int32_t>();
+    auto clen = in.arr<int32_t>(ncurve);
+    std::vector<std::vector<int>> curves(ncurve);
+    for (int c = 0; c < ncurve; ++c) {
+      auto cv = in.arr<int32_t>(clen[c]);
+      curves[c].assign(cv.begin(), cv.end());
+    }
+    auto circles = in.arr<int32_t>(ncurve);
+    auto curveL = in.arr<int32_t>(nL);
+    // one tree per patch over that patch's faces (the tri_Trees of Qfg.Rs)
+    int npatch = 0;
+    for (int f = 0; f < nsf; ++f) npatch = std::max(npatch, fpatch[f] + 1);
+    std::vector<Surface> ps(npatch);
+    for (int p = 0; p < npatch; ++p) {
+      std::vector<int> fl;
+      for (int f = 0; f < nsf; ++f) if (fpatch[f] == p) fl.push_back(f);
+      std::vector<int> remap(nsv, -1);
+      int nu = 0;
+      for (int f : fl) for (int k = 0; k < 3; ++k) if (remap[surf.F(f, k)] < 0) remap[surf.F(f, k)] = nu++;
+      ps[p].V.resize(nu, 3);
+      for (int v = 0; v < nsv; ++v) if (remap[v] >= 0) ps[p].V.row(remap[v]) = surf.V.row(v);
+      ps[p].F.resize((int)fl.size(), 3);
+      for (size_t i = 0; i < fl.size(); ++i) for (int k = 0; k < 3; ++k) ps[p].F((int)i, k) = remap[surf.F(fl[i], k)];
+      ps[p].build();
+    }
+    double LAMDA0 = lamT;
+    fc.lamda_C = fc.lamda_L = fc.lamda_T = LAMDA0;
+    std::vector<double> t_iter;
+    for (int it = 0; it < n_outer; ++it) {
+      const auto t0 = std::chrono::steady_clock::now();
+      // LINE rows: nearest segment of the vertex's curve (ties to the lower segment, as the
+      // sort of (distance, index) pairs leaves them)
+      for (int i = 0; i < nL; ++i) {
+        const Eigen::Vector3d v = ts.UV.row(fc.ids_L(i)).transpose();
+        const std::vector<int>& curve = curves[curveL[i]];
+        int curve_len = (int)curve.size();
+        if (!circles[curveL[i]]) curve_len--;
+        std::vector<Eigen::Vector3d> pvs, tangents;
+        std::vector<std::pair<double, uint32_t>> dis_ids;
+        for (int j = 0; j < curve_len; ++j) {
+          const int p0 = curve[j], p1 = curve[(j + 1) % curve.size()];
+          Eigen::Vector3d pv;
+          double t;
+          point_line_projection(surf.V.row(p0).transpose(), surf.V.row(p1).transpose(), v, pv, t);
+          const Eigen::Vector3d tangent = (surf.V.row(p1) - surf.V.row(p0)).transpose().normalized();
+          dis_ids.push_back(std::make_pair((v - pv).norm(), (uint32_t)pvs.size()));
+          pvs.push_back(pv);
+          tangents.push_back(tangent);
+        }
+        std::sort(dis_ids.begin(), dis_ids.end());
+        const uint32_t k = dis_ids[0].second;
+        fc.origin_L.row(i) = pvs[k].transpose();
+        fc.Axa_L.row(i) = tangents[k].transpose();
+      }
+      // REGULAR rows onto their own patch
+      for (int p = 0; p < npatch; ++p) {
+        std::vector<int> rows;
+        for (int j = 0; j < nTc; ++j) if (patchT[j] == p) rows.push_back(j);
+        if (rows.empty()) continue;
+        Eigen::MatrixXd Ps((int)rows.size(), 3);
+        for (size_t j = 0; j < rows.size(); ++j) Ps.row((int)j) = ts.UV.row(fc.ids_T(rows[j]));
+        Eigen::VectorXd S; Eigen::VectorXi I; Eigen::MatrixXd C, N;
+        igl::signed_distance_pseudonormal(Ps, ps[p].V, ps[p].F, ps[p].tree, ps[p].FN, ps[p].VN, ps[p].EN, ps[p].EMAP,
+                                          S, I, C, N);
+        for (size_t j = 0; j < rows.size(); ++j) {
+          fc.normal_T.row(rows[j]) = N.row((int)j);
+          fc.dis_T[rows[j]] = N.row((int)j).dot(C.row((int)j));
+        }
+      }
+      ts.fc = fc;
+      compute_referenceMesh(ts.UV, m.Hs, H_flag, Hids, ts.RT, true);
+      SLIMData data;
+      data.stop_threshold = 1.e-7;
+      data.soft_const_p = 1e1;
+      data.exp_factor = 5.0;
+      data.weight_opt = weight_opt;
+      slim_precompute(ts.V, ts.T, ts.UV, data, ts.energy_type, ts);
+      slim_solve2(data, 1, -1, false);
+      ts.UV = data.V_o;
+      ts.V = ts.UV;
+      t_iter.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+      log.push_back(data.energy); log.push_back(data.engery_quality); log.push_back(data.engery_soft);
+      log.push_back(fc.lamda_C);
+      fc.lamda_T = fc.lamda_L = fc.lamda_C =
+          std::min(LAMDA0 * std::max(data.engery_quality / data.engery_soft * fc.lamda_C, 1.0), lamT_bound);
       auto f = flat3(ts.UV);
       Vtrace.insert(Vtrace.end(), f.begin(), f.end());
     }

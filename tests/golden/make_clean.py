@@ -3,7 +3,8 @@
 grid_hex_meshing.cpp:1983-2124, compiled from the reference sources into oracle/_ref/clean_ref
 by oracle/ref/Makefile) on 40 seeded random blobs of 6^3 to 14^3 cell grids: a smoothed random
 field thresholded to the inside flags, half of the meshes with their vertices and hexes
-renumbered at random. The flags after each pass are stored with the inputs.
+renumbered at random. The flags after each pass are stored with the inputs, and the
+medial-surface tags of the final flags (:1966-1981): vertex flags and the tagged faces.
 
 Run in the development container:  python tests/golden/make_clean.py
 """
@@ -49,7 +50,9 @@ def run(nV, H, flag):
             f.write(np.ascontiguousarray(flag, dtype="<i4").tobytes())
         subprocess.run([BIN, pin, pout], check=True, stdout=subprocess.DEVNULL)
         r = refio.read_out(pout)
-    return {k: v.astype(bool) for k, v in r.items()}
+    out = {k: v.astype(bool) for k, v in r.items() if k != "medial_f"}
+    out["medial_f"] = r["medial_f"].astype(np.int32).reshape(-1, 4)
+    return out
 
 
 def main():
@@ -68,7 +71,7 @@ def main():
         out["m%d_nV" % i] = nV
         out["m%d_H" % i] = H
         out["m%d_in" % i] = flag
-        for k in ("tag", "manifold", "final"):
+        for k in ("tag", "manifold", "final", "medial_v", "medial_f"):
             out["m%d_%s" % (i, k)] = r[k]
     out["count"] = 40
     path = os.path.join(HERE, "clean_blobs.npz")

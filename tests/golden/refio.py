@@ -45,8 +45,9 @@ def read_out(path):
 def write_slim_input(path, mode, V, H, energy=2, n_outer=1, n_inner=1, cube=True, projection=False, weight_opt=1.0,
                      lamT=0.0, lamT_bound=1e8, lamC=0.0, lamL=0.0, lamProj=1e5, hflag=None, b=None, bc=None,
                      idsT=None, normalT=None, disT=None, idsC=None, Cpos=None, idsL=None, axaL=None, originL=None,
-                     surfV=None, surfF=None):
-    """The driver's input file (slim_ref_driver.cpp's main, modes 0, 1 and 3)."""
+                     surfV=None, surfF=None, tail=b""):
+    """The driver's input file (slim_ref_driver.cpp's main, modes 0, 1, 3 and 7; `tail`: the
+    bytes mode 7 reads after the surface)."""
     nV, nH = V.shape[0], H.shape[0]
     e3 = np.zeros((0, 3)); ei = np.zeros(0, dtype=np.int32)
     hflag = np.ones(nH, dtype=np.int32) if hflag is None else hflag
@@ -66,6 +67,7 @@ def write_slim_input(path, mode, V, H, energy=2, n_outer=1, n_inner=1, cube=True
             _w(f, "ii", 0, 0)
         else:
             _w(f, "ii", surfV.shape[0], surfF.shape[0]); _arr(f, surfV, "<f8"); _arr(f, surfF, "<i4")
+        f.write(tail)
 
 
 def run_slim(mode, V, H, **kw):
@@ -74,6 +76,25 @@ def run_slim(mode, V, H, **kw):
         write_slim_input(pin, mode, V, H, **kw)
         subprocess.run([REF_BIN, pin, pout], check=True, stdout=subprocess.DEVNULL)
         return read_out(pout)
+
+
+def run_feature(V, H, n_outer, weight_opt, lam0, lam_bound, idsT, idsC, Cpos, idsL, axaL, originL, surfV, surfF,
+                fpatch, patchT, curves, circles, curveL):
+    """Mode 7: the feature-alignment loop body with every vertex clean (lines re-projected
+    onto their curves, planes onto their own patch, corners fixed, the three weights grown
+    together from lam0)."""
+    import io
+    t = io.BytesIO()
+    _arr(t, fpatch, "<i4"); _arr(t, patchT, "<i4")
+    _w(t, "i", len(curves)); _arr(t, [len(c) for c in curves], "<i4")
+    for c in curves:
+        _arr(t, c, "<i4")
+    _arr(t, circles, "<i4"); _arr(t, curveL, "<i4")
+    nT = len(idsT)
+    return run_slim(7, V, H, energy=2, n_outer=n_outer, n_inner=1, weight_opt=weight_opt, lamT=lam0,
+                    lamT_bound=lam_bound, lamC=lam0, lamL=lam0, idsT=idsT, normalT=np.zeros((nT, 3)),
+                    disT=np.zeros(nT), idsC=idsC, Cpos=Cpos, idsL=idsL, axaL=axaL, originL=originL, surfV=surfV,
+                    surfF=surfF, tail=t.getvalue())
 
 
 def run_glue(V, H, fixed, cubes, energy=1, weight_opt=1.0):

@@ -1,8 +1,9 @@
 """clean_hex_mesh's flag passes (include/hexclean.h, host code) against the reference's own
 tagging_uneven_element, clean_non_manifold_ve and drop_small_pieces compiled from its
 sources (tests/golden/clean_blobs.npz, written by tests/golden/make_clean.py): 40 seeded
-random blobs of 6^3 to 14^3 cells, half renumbered at random, flag for flag after each pass.
-CPU only."""
+random blobs of 6^3 to 14^3 cells, half renumbered at random, flag for flag after each pass,
+then the medial-surface tags of the final flags (vertex flags, and the tagged faces with
+their corners in the reference's face order). CPU only."""
 import numpy as np
 import pytest
 
@@ -22,3 +23,6 @@ def test_cleaning_passes_match_compiled_reference(i):
     fin = hexclean.drop_small_pieces(nV, H, man)
     assert np.array_equal(fin, C["m%d_final" % i])
     assert np.array_equal(hexclean.clean_flags(nV, H, flag), C["m%d_final" % i])
+    vmed, fmed = hexclean.medial_surface(nV, H, fin)
+    assert np.array_equal(vmed, C["m%d_medial_v" % i])
+    assert np.array_equal(fmed, C["m%d_medial_f" % i])
