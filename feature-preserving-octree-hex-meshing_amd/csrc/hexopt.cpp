@@ -553,16 +553,14 @@ int hexopt_slim_solve(hexopt_ctx* c, const hexopt_params* prm, int iters, hexopt
     if (nL > 0 && !prm->projection)
       HIPCHK(c, hipMemcpyAsync(aL, x + 3 * (size_t)nV, nL * sizeof(double), hipMemcpyDeviceToDevice, st));
     // flip-avoiding step bound and, outside the glue, the energy at the start of the line
-    // search in the same pass over the hexes
+    // search, read back together
     double* scal = c->d_scal.as<double>();
     double q0 = 0, s0 = 0;
-    if (c->glue) {
-      launch_max_step(cur, x, nV, c->tm.conn, nH, c->d_part[0].as<double>(), scal + 4, st);
-    } else {
-      launch_bound_energy(ro, cur, x, nV, c->tm.conn, nH, prm->energy, M, prm->exp_factor, c->sd, aL,
-                          c->d_part[0].as<double>(), c->d_part[3].as<double>(), c->d_part[1].as<double>(),
-                          nblocks(c->sd.nb + c->sd.nC + c->sd.nT + c->sd.nL), scal + 4, scal, st);
-    }
+    launch_max_step(cur, x, nV, c->tm.conn, nH, c->d_part[0].as<double>(), scal + 4, st);
+    if (!c->glue)
+      launch_energy(ro, cur, nullptr, 0.0, nV, c->tm.conn, nH, prm->energy, M, prm->exp_factor, c->sd, aL,
+                    c->d_part[0].as<double>(), c->d_part[1].as<double>(),
+                    nblocks(c->sd.nb + c->sd.nC + c->sd.nT + c->sd.nL), scal, st);
     std::vector<unsigned char> cgs(cg_state_bytes());
     double hs[5] = {0, 0, 0, 0, 0};
     HIPCHK(c, hipMemcpyAsync(hs, scal, 5 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -885,10 +883,11 @@ int hexopt_opt32_solve(hexopt_ctx* c, const hexopt_params* prm, const hexopt_opt
     // a_L is the line unknowns' solution (as slim_m.cpp:501-503), used by the soft energy
     double* aL = c->d_aL.as<double>();
     if (nL > 0) launch_f2d(nL, d + 3 * (size_t)nV, aL, st);
-    // FP64 flip-free bound and the FP64 energy at the current positions (one pass), then the
-    // reference's bisection on FP64 energies along the FP32 direction
-    launch_bound_energy_dir32(ro, cur, d, nV, c->tm.conn, nH, M, sd, aL, c->d_part[0].as<double>(),
-                              c->d_part[3].as<double>(), c->d_part[1].as<double>(), nsb, scal + 4, scal, st);
+    // FP64 flip-free bound, then the reference's bisection on FP64 energies along the FP32
+    // direction
+    launch_max_step_dir32(cur, d, nV, c->tm.conn, nH, c->d_part[0].as<double>(), scal + 4, st);
+    launch_energy_dir32(ro, cur, d, 0.0, nV, c->tm.conn, nH, M, sd, aL, c->d_part[0].as<double>(),
+                        c->d_part[1].as<double>(), nsb, scal, st);
     double hs[10];
     int ninv = 0;
     std::vector<unsigned char> cgs(cg_state_bytes());

@@ -94,15 +94,6 @@ void launch_cg_update(int n, void* state, double* x, double* r, double* z, const
 void launch_cg_zero_if(int n, void* state, double* x, int nb, hipStream_t st);
 size_t cg_state_bytes();
 void read_cg_state(const void* host_state, int* iters, double* tol_error, int* done);
-// max_step + energy at the current positions in one pass over the hexes (plus the soft
-// energy): out_min = the flip-free bound, out2 = (element, soft) energies.
-void launch_bound_energy(const RefOp& ro, const double* cur, const double* dst, int nV, const int* conn, int nH,
-                         int energy, double M, double exp_f, const SoftDev& sd, const double* aL, double* part_min,
-                         double* part_q, double* part_s, int ns_blocks, double* out_min, double* out2,
-                         hipStream_t st);
-void launch_bound_energy_dir32(const RefOp& ro, const double* cur, const float* d, int nV, const int* conn, int nH,
-                               double M, const SoftDev& sd, const double* aL, double* part_min, double* part_q,
-                               double* part_s, int ns_blocks, double* out_min, double* out2, hipStream_t st);
 void launch_max_step(const double* cur, const double* dst, int nV, const int* conn, int nH, double* part, double* out,
                      hipStream_t st);
 void launch_energy(const RefOp& ro, const double* cur, const double* dst, double step, int nV, const int* conn, int nH,

@@ -1897,48 +1897,6 @@ __global__ __launch_bounds__(256) void max_step_kernel(const double* cur, const 
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// The line search's two passes before its first trial step in one (both read every hex's
-// corners and the direction): per block the flip-free bound (max_step_kernel's minimum
-// root) and the element energy at the current positions (energy_kernel<double> with no
-// direction), each term computed and added in the same order as those kernels do.
-template <bool DIFF, class DT>
-__global__ __launch_bounds__(256) void max_step_energy_kernel(RefOp ro, const double* cur, const DT* dst, int nV,
-                                                              const int* conn, int nH, int energy, double M,
-                                                              double exp_f, double* part_min, double* part_e) {
-  __shared__ double red[16];
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  double m = INFINITY, e = 0;
-  if (h < nH) {
-    int vid[8];
-    double p[8][3], q[8][3];
-    gather8(cur, nV, conn, nH, h, vid, p);
-    for (int k = 0; k < 8; ++k)
-      for (int d = 0; d < 3; ++d) q[k][d] = line_dir<DIFF>(p[k][d], dst[(size_t)d * nV + vid[k]]);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      double x[4][3], dd[4][3];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        pick_row<double, 3>(p, kHexTet[c][j], x[j]);
-        pick_row<double, 3>(q, kHexTet[c][j], dd[j]);
-      }
-      m = fmin(m, tet_min_root(x, dd));
-    }
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      double B[3][3], E[3][3], J[3][3];
-      tet_B(ro, nH, h, c, B);
-      tet_edges(p, c, E);
-      matmul3(E, B, J);
-      e += M * (double)tet_energy(J, energy, exp_f);
-    }
-  }
-  const double mn = block_min(m, red);
-  if (threadIdx.x == 0) part_min[blockIdx.x] = mn;
-  const double se = block_sum(e, red);
-  if (threadIdx.x == 0) part_e[blockIdx.x] = se;
-}
-
 __global__ void min_fin_kernel(const double* part, int n, double* out) {
   __shared__ double red[16];
   double m = INFINITY;
@@ -2632,37 +2590,6 @@ void launch_energy_dir32(const RefOp& ro, const double* cur, const float* d, dou
     hipLaunchKernelGGL((energy_kernel<double, false, float>), dim3(nb), dim3(256), 0, st, ro, cur, d, step, nV, conn, nH,
                        (int)E_SYMMETRIC_DIRICHLET, M, 1.0, part_q);
   hipLaunchKernelGGL((soft_energy_kernel<false, float>), dim3(ns_blocks), dim3(256), 0, st, sd, cur, d, step, nV, aL,
-                     part_s);
-  hipLaunchKernelGGL(energy_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part_q, nb, (const double*)part_s,
-                     ns_blocks, out2);
-}
-
-void launch_bound_energy(const RefOp& ro, const double* cur, const double* dst, int nV, const int* conn, int nH,
-                         int energy, double M, double exp_f, const SoftDev& sd, const double* aL, double* part_min,
-                         double* part_q, double* part_s, int ns_blocks, double* out_min, double* out2,
-                         hipStream_t st) {
-  const int nb = (nH + 255) / 256;
-  if (nb > 0) {
-    hipLaunchKernelGGL((max_step_energy_kernel<true, double>), dim3(nb), dim3(256), 0, st, ro, cur, dst, nV, conn, nH,
-                       energy, M, exp_f, part_min, part_q);
-    hipLaunchKernelGGL(min_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part_min, nb, out_min);
-  }
-  hipLaunchKernelGGL((soft_energy_kernel<true, double>), dim3(ns_blocks), dim3(256), 0, st, sd, cur,
-                     (const double*)nullptr, 0.0, nV, aL, part_s);
-  hipLaunchKernelGGL(energy_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part_q, nb, (const double*)part_s,
-                     ns_blocks, out2);
-}
-
-void launch_bound_energy_dir32(const RefOp& ro, const double* cur, const float* d, int nV, const int* conn, int nH,
-                               double M, const SoftDev& sd, const double* aL, double* part_min, double* part_q,
-                               double* part_s, int ns_blocks, double* out_min, double* out2, hipStream_t st) {
-  const int nb = (nH + 255) / 256;
-  if (nb > 0) {
-    hipLaunchKernelGGL((max_step_energy_kernel<false, float>), dim3(nb), dim3(256), 0, st, ro, cur, d, nV, conn, nH,
-                       (int)E_SYMMETRIC_DIRICHLET, M, 1.0, part_min, part_q);
-    hipLaunchKernelGGL(min_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part_min, nb, out_min);
-  }
-  hipLaunchKernelGGL((soft_energy_kernel<false, float>), dim3(ns_blocks), dim3(256), 0, st, sd, cur, d, 0.0, nV, aL,
                      part_s);
   hipLaunchKernelGGL(energy_fin_kernel, dim3(1), dim3(1024), 0, st, (const double*)part_q, nb, (const double*)part_s,
                      ns_blocks, out2);
