@@ -38,7 +38,7 @@ def test_library_exports_every_cleaning_symbol():
     from pyhexopt import hexclean
     lib = ctypes.CDLL(capi.LIB_PATH)
     names = declared("hexclean.h", "hexclean")
-    assert len(names) == 6
+    assert len(names) == 7
     for n in names:
         assert hasattr(lib, n), n
     assert sorted(names) == sorted(hexclean.EXPORTS)
