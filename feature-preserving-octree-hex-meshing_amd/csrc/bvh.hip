@@ -90,7 +90,10 @@ __device__ __forceinline__ unsigned fr_state(unsigned f) { return (f >> 26) & 7u
 __device__ __forceinline__ unsigned fr_seen(unsigned f) { return (f >> 29) & 3u; }
 __device__ __forceinline__ unsigned fr_make(int n, unsigned st, unsigned seen) { return (unsigned)n | (st << 26) | (seen << 29); }
 
-__global__ __launch_bounds__(kBvhBlock) void closest_kernel(SurfaceDev s0, const PatchSeg* __restrict__ segs, int nseg,
+// six waves per SIMD (80 registers, 2 values spilled) instead of the five that 82 registers
+// leave: more walks in flight to cover the dependent node fetches (the four patch walks of the
+// 1.1M-hex torus refresh: 275-279 against 294-295 us; eight waves with 33 spills, 284-287)
+__global__ __launch_bounds__(kBvhBlock) __attribute__((amdgpu_waves_per_eu(6))) void closest_kernel(SurfaceDev s0, const PatchSeg* __restrict__ segs, int nseg,
                                                             int nq, const double* P, int soa, int stride,
                                                             const int* ids, const int* order, double* S, int* I,
                                                             double* Cout, double* Nout, double* sqrDout, int* status) {
