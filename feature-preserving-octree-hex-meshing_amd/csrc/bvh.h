@@ -82,7 +82,7 @@ void launch_metro_ep(const SurfaceDev& s, const double* ep, int nq, const double
                      int* status, hipStream_t st);
 
 struct SoftDev;
-void launch_refold_planes(const SoftDev& sd, int nT, const double* Cp, double* t_d, const double* S0, const double* b0,
-                          double* S, double* b, double lam, hipStream_t st);
+void launch_refold_planes(const SoftDev& sd, const int* trow, int nrows, const double* Cp, double* t_d,
+                          const double* S0, const double* b0, double* S, double* b, double lam, hipStream_t st);
 
 }  // namespace hexopt

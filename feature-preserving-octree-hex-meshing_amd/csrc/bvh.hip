@@ -283,15 +283,17 @@ __global__ __launch_bounds__(kBvhBlock) __attribute__((amdgpu_waves_per_eu(6))) 
 }
 
 // S = S0 + lamT n n^T, b = b0 + lamT n d with d = n . C, for the plane rows (one row per
-// plane vertex; rows of other constraint kinds stay S0/b0)
-__global__ void refold_planes_kernel(const int* t_ids, const int* cidx, int nT, const double* Cp, const double* t_n,
-                                     double* t_d, const double* S0, const double* b0, double* S, double* b, double lam) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nT) return;
+// plane vertex; rows of other constraint kinds stay S0/b0). One thread per row, in row
+// (vertex) order: trow[r] is the plane index of row r, or -1.
+__global__ void refold_planes_kernel(const int* trow, int nrows, const double* Cp, const double* t_n, double* t_d,
+                                     const double* S0, const double* b0, double* S, double* b, double lam) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int i = trow[r];
+  if (i < 0) return;
   const double n0 = t_n[(size_t)i * 3], n1 = t_n[(size_t)i * 3 + 1], n2 = t_n[(size_t)i * 3 + 2];
   const double d = n0 * Cp[(size_t)i * 3] + n1 * Cp[(size_t)i * 3 + 1] + n2 * Cp[(size_t)i * 3 + 2];
   t_d[i] = d;
-  const int r = cidx[t_ids[i]];
   const double* s0 = S0 + (size_t)r * 6;
   const double* c0 = b0 + (size_t)r * 3;
   double* s1 = S + (size_t)r * 6;
@@ -303,11 +305,11 @@ __global__ void refold_planes_kernel(const int* t_ids, const int* cidx, int nT, 
 
 }  // namespace
 
-void launch_refold_planes(const SoftDev& sd, int nT, const double* Cp, double* t_d, const double* S0, const double* b0,
-                          double* S, double* b, double lam, hipStream_t st) {
-  if (nT <= 0) return;
-  hipLaunchKernelGGL(refold_planes_kernel, dim3((nT + 255) / 256), dim3(256), 0, st, sd.t_ids, sd.cidx, nT, Cp, sd.t_n,
-                     t_d, S0, b0, S, b, lam);
+void launch_refold_planes(const SoftDev& sd, const int* trow, int nrows, const double* Cp, double* t_d,
+                          const double* S0, const double* b0, double* S, double* b, double lam, hipStream_t st) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(refold_planes_kernel, dim3((nrows + 255) / 256), dim3(256), 0, st, trow, nrows, Cp, sd.t_n, t_d,
+                     S0, b0, S, b, lam);
 }
 
 // Vertex-sampled metro statistics of per-query squared distances: a query counts when its

@@ -85,6 +85,8 @@ struct hexopt_ctx {
   DevBuf d_X, d_G, d_v;
   // tangent-plane refresh: folded rows without the plane terms
   DevBuf d_S0, d_b0;
+  DevBuf d_trow;  // per constraint row: its plane (T) index, or -1
+  int nrows = 0;
   bool planes_refreshable = false;
   std::vector<int> h_tids;          // plane rows' vertices (Morton numbering), ids_T order
   // per-patch projection: the last patch assignment, its rows grouped by patch (each group in
@@ -361,6 +363,18 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
     }
     return cidx[v];
   };
+  // rows numbered in the device's vertex order (whatever order the caller lists the
+  // constraints in), so the vertex passes, which walk the vertices in that order, read the
+  // 48-byte rows of neighbouring vertices from neighbouring places; the values of a row do
+  // not depend on its number
+  {
+    std::vector<char> has(nV, 0);
+    for (int i = 0; i < cs->nC; ++i) has[L.vinv[cs->ids_C[i]]] = 1;
+    for (int i = 0; i < cs->nT; ++i) has[L.vinv[cs->ids_T[i]]] = 1;
+    for (int i = 0; i < cs->nL; ++i) has[L.vinv[cs->ids_L[i]]] = 1;
+    for (int v = 0; v < nV; ++v)
+      if (has[v]) row(v);
+  }
   for (int i = 0; i < cs->nC; ++i) {
     int r = row(L.vinv[cs->ids_C[i]]);
     for (int d = 0; d < 3; ++d) { S[r * 6 + d] += cs->lamda_C; b[r * 3 + d] += cs->lamda_C * cs->C[i * 3 + d]; }
@@ -415,6 +429,12 @@ int hexopt_set_constraints(hexopt_ctx* c, const hexopt_constraints* cs) {
   rc |= upload(c, c->d_S0, S0_pre);
   rc |= upload(c, c->d_b0, b0_pre);
   c->planes_refreshable = !dupT;
+  c->nrows = (int)cline.size();
+  if (!dupT) {
+    std::vector<int> trow(std::max<size_t>(1, cline.size()), -1);
+    for (int i = 0; i < cs->nT; ++i) trow[cidx[L.vinv[cs->ids_T[i]]]] = i;
+    rc |= upload(c, c->d_trow, trow);
+  }
   {
     std::vector<char> other(nV, 0);
     for (int i = 0; i < cs->nC; ++i) other[L.vinv[cs->ids_C[i]]] = 1;
@@ -1212,7 +1232,7 @@ int hexopt_project_planes(hexopt_ctx* c, hexopt_surface* s, double lamda_T) {
   launch_closest(s->view, nT, c->d_cur.as<double>(), 1, c->L.nV, c->d_tids.as<int>(), nullptr, nullptr,
                  s->dC.as<double>(), const_cast<double*>(c->sd.t_n), nullptr, s->dstat.as<int>(), st,
                  c->d_torder.as<int>());
-  launch_refold_planes(c->sd, nT, s->dC.as<double>(), const_cast<double*>(c->sd.t_d), c->d_S0.as<double>(),
+  launch_refold_planes(c->sd, c->d_trow.as<int>(), c->nrows, s->dC.as<double>(), const_cast<double*>(c->sd.t_d), c->d_S0.as<double>(),
                        c->d_b0.as<double>(), const_cast<double*>(c->sd.S), const_cast<double*>(c->sd.b), lamda_T, st);
   c->sd.lamT = lamda_T;
   // no host round trip: the walk's stack holds the tree's depth plus two frames, and trees
@@ -1283,7 +1303,7 @@ int hexopt_project_patches(hexopt_ctx* c, int npatch, hexopt_surface* const* pat
   launch_closest_segs(c->d_segs.as<PatchSeg>(), (int)segs.size(), nblocks, max_depth, c->d_cur.as<double>(), 1,
                       c->L.nV, c->d_tids.as<int>(), c->d_porder.as<int>(), c->d_pC.as<double>(),
                       const_cast<double*>(c->sd.t_n), c->d_pstat.as<int>(), st);
-  launch_refold_planes(c->sd, nT, c->d_pC.as<double>(), const_cast<double*>(c->sd.t_d), c->d_S0.as<double>(),
+  launch_refold_planes(c->sd, c->d_trow.as<int>(), c->nrows, c->d_pC.as<double>(), const_cast<double*>(c->sd.t_d), c->d_S0.as<double>(),
                        c->d_b0.as<double>(), const_cast<double*>(c->sd.S), const_cast<double*>(c->sd.b), lamda_T, st);
   c->sd.lamT = lamda_T;
   HIPCHK(c, hipGetLastError());
